@@ -1,0 +1,151 @@
+"""ctypes mirror of include/auto_hip.h and the loader for libauto_hip.so.
+
+The shared library is the product; this module only describes its C ABI.
+Loading fails loudly when the library is missing: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+MAX_CHANNELS = 3
+MAX_KERNELS = 9
+MAX_RINGS = 3
+
+OK = 0
+ERR_INVALID = 1
+ERR_DEVICE = 2
+ERR_ALLOC = 3
+ERR_UNKNOWN_FIELD = 4
+ERR_SIZE = 5
+ERR_UNSUPPORTED = 6
+ERR_DIVERGED = 7
+
+PATH_FFT_FP32 = 0
+PATH_DIRECT_BF16 = 1
+
+INIT_ORBIUM = 0
+INIT_PATCH = 1
+INIT_BLOBS = 2
+INIT_ZERO = 3
+
+PATHS = {"fft_fp32": PATH_FFT_FP32, "direct_bf16": PATH_DIRECT_BF16}
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [
+        ("radius", ctypes.c_float),
+        ("rings", ctypes.c_uint32),
+        ("beta", ctypes.c_float * MAX_RINGS),
+        ("mu", ctypes.c_float),
+        ("sigma", ctypes.c_float),
+        ("weight", ctypes.c_float),
+        ("src", ctypes.c_uint32),
+        ("dst", ctypes.c_uint32),
+    ]
+
+
+class LeniaDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("n_kernels", ctypes.c_uint32),
+        ("kernels", Kernel * MAX_KERNELS),
+        ("dt", ctypes.c_float),
+        ("path", ctypes.c_uint32),
+        ("init", ctypes.c_uint32),
+        ("init_size", ctypes.c_uint32),
+        ("init_count", ctypes.c_uint32),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("n_kernels", ctypes.c_uint32),
+        ("path", ctypes.c_uint32),
+        ("cells_per_step", ctypes.c_uint64),
+        ("generations_per_step", ctypes.c_uint64),
+        ("total_bytes", ctypes.c_uint64),
+        ("generation", ctypes.c_uint64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("mass", ctypes.c_double * MAX_CHANNELS),
+        ("min", ctypes.c_float * MAX_CHANNELS),
+        ("max", ctypes.c_float * MAX_CHANNELS),
+        ("nonfinite", ctypes.c_uint64),
+        ("generation", ctypes.c_uint64),
+    ]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 40),
+        ("launches", ctypes.c_uint64),
+        ("total_ms", ctypes.c_double),
+    ]
+
+
+LIB_NAME = "libauto_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+# Every symbol include/auto_hip.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "ah_lenia_build": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(LeniaDesc)]),
+    "ah_sim_new": (ctypes.c_int, [ctypes.POINTER(LeniaDesc), ctypes.c_uint32, ctypes.POINTER(_P)]),
+    "ah_sim_step": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "ah_sim_wait": (ctypes.c_int, [_P]),
+    "ah_sim_read_field": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
+    "ah_sim_write_field": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
+    "ah_sim_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "ah_sim_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
+    "ah_sim_destroy": (None, [_P]),
+    "ah_last_error": (ctypes.c_char_p, []),
+    "ah_sim_profile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ah_sim_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(KernelTime), ctypes.c_uint32,
+                                           ctypes.POINTER(ctypes.c_uint32)]),
+}
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    """A non-OK ah_status, carrying the code and ah_last_error()'s message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"auto_hip error {code}: {message}")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load libauto_hip.so from the package directory; raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != OK:
+        msg = load().ah_last_error()
+        raise EngineError(code, msg.decode() if msg else "")
+
+
+def symbols() -> list[str]:
+    return list(_SIGNATURES)

@@ -1,0 +1,474 @@
+// Host engine behind include/auto_hip.h.
+//
+// One ah_sim owns its device buffers and one HIP stream (default priority).
+// It plays the part of auto-core's Sim for the Lenia program
+// (crates/auto-core/src/sim.rs:202-309,559-595): creation builds the kernels
+// and seeds the state in one go (Sim::new runs the init passes, :286-294),
+// stepping is asynchronous (Sim::step, :297-309), reads block.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "ah_internal.h"
+
+namespace ah {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+// ------------------------------------------------------------ profiler
+hipEvent_t Profiler::get() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    AH_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Profiler::reset() {
+    for (auto& r : recs) {
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    recs.clear();
+}
+
+Profiler::~Profiler() {
+    reset();
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+Launch::Launch(Profiler* p_, hipStream_t s_, const char* n_) : p(p_), s(s_), name(n_) {
+    if (p && p->enabled) {
+        b = p->get();
+        AH_HIP(hipEventRecord(b, s));
+    }
+}
+
+Launch::~Launch() {
+    if (p && p->enabled && b) {
+        hipEvent_t e = p->get();
+        if (hipEventRecord(e, s) == hipSuccess) p->recs.push_back({name, b, e});
+    }
+}
+
+}  // namespace ah
+
+using namespace ah;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ah_sim {
+    ah_lenia_desc d{};
+    uint32_t seed = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Profiler prof;
+    std::vector<DevBuf> bufs;
+    size_t total_bytes = 0;
+    uint64_t generation = 0;
+
+    float* state = nullptr;  // [C][H][W]
+    float* state_alt = nullptr;
+    KernelParams kp[AH_MAX_KERNELS]{};
+    int ksrc[AH_MAX_KERNELS]{};
+    int reach[AH_MAX_KERNELS]{};
+
+    // FFT path
+    float2* spec_src = nullptr;  // [C][W/8][H][4]
+    float2* spec_k = nullptr;    // [K][W/8][H][4] (aliases spec_src when K == C == 1)
+    float* khat = nullptr;       // [K][W/8][H/2+1][4]
+    float2* kpack = nullptr;     // [K][H/2+1]
+    float2* twW = nullptr;
+    float2* twH = nullptr;
+
+    // direct path
+    uint16_t* taps_bf16[AH_MAX_KERNELS]{};
+    float tap_scale[AH_MAX_KERNELS]{};
+
+    // stats
+    double* pmass = nullptr;
+    float* pmin = nullptr;
+    float* pmax = nullptr;
+    unsigned long long* pnf = nullptr;
+    int stat_blocks = 0;
+
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            throw Error{AH_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e)};
+        bufs.push_back({p, bytes});
+        total_bytes += bytes;
+        return p;
+    }
+
+    ~ah_sim() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& b : bufs) (void)hipFree(b.p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t cells() const { return (size_t)d.width * d.height; }
+};
+
+namespace {
+
+bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+// Reject what the reference asserts on (spectral.rs:103-116) and what this build cannot run.
+void validate(const ah_lenia_desc& d) {
+    auto bad = [](const std::string& m) { throw Error{AH_ERR_INVALID, m}; };
+    if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
+    if (d.width < 16 || d.height < 16 || d.width > 8192 || d.height > 8192) bad("width and height must be in 16..8192");
+    if (d.channels < 1 || d.channels > AH_MAX_CHANNELS) bad("channels must be 1..3");
+    if (d.n_kernels < 1 || d.n_kernels > AH_MAX_KERNELS) bad("n_kernels must be 1..9");
+    if (!std::isfinite(d.dt) || d.dt <= 0.0f) bad("dt must be a positive finite number");
+    const uint32_t side = std::min(d.width, d.height);
+    float wsum[AH_MAX_CHANNELS] = {0, 0, 0};
+    for (uint32_t k = 0; k < d.n_kernels; ++k) {
+        const ah_kernel& kd = d.kernels[k];
+        if (!(kd.radius > 0.0f) || !std::isfinite(kd.radius)) bad("kernel radius must be positive");
+        int reach = (int)std::ceil((double)kd.radius);
+        if (2 * reach >= (int)side) bad("world smaller than its kernel");
+        if (reach > 127) throw Error{AH_ERR_UNSUPPORTED, "kernel radius above 127 cells"};
+        if (kd.rings < 1 || kd.rings > AH_MAX_RINGS) bad("rings must be 1..3");
+        if (!(kd.sigma > 0.0f) || !std::isfinite(kd.mu)) bad("sigma must be positive, mu finite");
+        if (!(kd.weight > 0.0f)) bad("kernel weight must be positive");
+        if (kd.src >= d.channels || kd.dst >= d.channels) bad("kernel src/dst out of range");
+        wsum[kd.dst] += kd.weight;
+    }
+    if (d.path != AH_PATH_FFT_FP32 && d.path != AH_PATH_DIRECT_BF16) bad("unknown path");
+    if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
+        throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
+    if (d.init > AH_INIT_ZERO) bad("unknown init");
+    if (d.init == AH_INIT_ORBIUM && d.channels != 1) bad("Orbium init is single-channel");
+    if (d.init == AH_INIT_PATCH && (d.init_size == 0 || d.init_size > side)) bad("patch size must be 1..side");
+    if (d.init == AH_INIT_BLOBS && (d.init_size == 0 || d.init_size > side || d.init_count > 65536))
+        bad("blob size must be 1..side and count <= 65536");
+    (void)wsum;
+}
+
+// SplitMix64, crates/auto-life-gpu/src/world.rs:153-159.
+uint64_t splitmix64(uint64_t& s) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void setup(ah_sim* s) {
+    const ah_lenia_desc& d = s->d;
+    const int W = (int)d.width, H = (int)d.height, C = (int)d.channels, K = (int)d.n_kernels;
+    const size_t cells = s->cells();
+    AH_HIP(hipGetDevice(&s->device));
+    AH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    hipStream_t st = s->stream;
+
+    s->state = (float*)s->alloc(sizeof(float) * cells * C);
+
+    // Normalised growth parameters.
+    float wsum[AH_MAX_CHANNELS] = {0, 0, 0};
+    for (int k = 0; k < K; ++k) wsum[d.kernels[k].dst] += d.kernels[k].weight;
+    for (int k = 0; k < K; ++k) {
+        const ah_kernel& kd = d.kernels[k];
+        s->kp[k] = KernelParams{kd.mu, 1.0f / kd.sigma, kd.weight / wsum[kd.dst], kd.src, kd.dst};
+        s->ksrc[k] = (int)kd.src;
+        s->reach[k] = (int)std::ceil((double)kd.radius);
+    }
+
+    // FP64 taps of every kernel (scratch for set-up).
+    int max_reach = 0;
+    for (int k = 0; k < K; ++k) max_reach = std::max(max_reach, s->reach[k]);
+    const int max_side = 2 * max_reach + 1;
+    std::vector<double*> taps(K);
+    double* tap_sums = nullptr;
+    std::vector<void*> scratch;
+    auto tmp = [&](size_t bytes) {
+        void* p = nullptr;
+        AH_HIP(hipMalloc(&p, bytes));
+        scratch.push_back(p);
+        return p;
+    };
+    try {
+        tap_sums = (double*)tmp(sizeof(double) * K);
+        for (int k = 0; k < K; ++k) {
+            const int side = 2 * s->reach[k] + 1;
+            taps[k] = (double*)tmp(sizeof(double) * side * side);
+            launch_kernel_taps64(st, &s->prof, d.kernels[k], s->reach[k], taps[k], tap_sums + k);
+        }
+        (void)max_side;
+
+        if (d.path == AH_PATH_FFT_FP32) {
+            const size_t spec_elems = (size_t)(W / 2) * H;  // complex per channel / kernel
+            s->spec_src = (float2*)s->alloc(sizeof(float2) * spec_elems * C);
+            bool alias = (K == 1 && C == 1);
+            s->spec_k = alias ? s->spec_src : (float2*)s->alloc(sizeof(float2) * spec_elems * K);
+            const size_t np = (size_t)H / 2 + 1;
+            s->khat = (float*)s->alloc(sizeof(float) * (size_t)(W / 8) * np * BC * K);
+            s->kpack = (float2*)s->alloc(sizeof(float2) * np * K);
+            AH_HIP(hipMemsetAsync(s->khat, 0, sizeof(float) * (size_t)(W / 8) * np * BC * K, st));
+            s->twW = (float2*)s->alloc(sizeof(float2) * W);
+            s->twH = (float2*)s->alloc(sizeof(float2) * H);
+            launch_twiddles(st, &s->prof, s->twW, W);
+            launch_twiddles(st, &s->prof, s->twH, H);
+            double* cosW = (double*)tmp(sizeof(double) * W);
+            double* cosH = (double*)tmp(sizeof(double) * H);
+            launch_cos_table64(st, &s->prof, cosW, W);
+            launch_cos_table64(st, &s->prof, cosH, H);
+            for (int k = 0; k < K; ++k) {
+                const int side = 2 * s->reach[k] + 1;
+                double* T = (double*)tmp(sizeof(double) * side * (W / 2 + 1));
+                launch_khat64(st, &s->prof, W, H, s->reach[k], taps[k], cosW, cosH, T,
+                              s->khat + (size_t)k * (W / 8) * np * BC, s->kpack + (size_t)k * np);
+            }
+        } else {
+            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path not built yet"};
+        }
+
+        // Seed.
+        uint32_t* corners = nullptr;
+        if (d.init == AH_INIT_BLOBS && d.init_count > 0) {
+            std::vector<uint32_t> host(2 * d.init_count);
+            uint64_t rs = s->seed;
+            for (uint32_t i = 0; i < d.init_count; ++i) {
+                host[2 * i] = (uint32_t)(splitmix64(rs) % d.width);
+                host[2 * i + 1] = (uint32_t)(splitmix64(rs) % d.height);
+            }
+            corners = (uint32_t*)tmp(sizeof(uint32_t) * host.size());
+            AH_HIP(hipMemcpyAsync(corners, host.data(), sizeof(uint32_t) * host.size(), hipMemcpyHostToDevice, st));
+        }
+        launch_seed(st, &s->prof, d, s->seed, corners, s->state);
+
+        s->stat_blocks = stats_blocks(cells);
+        size_t np = (size_t)s->stat_blocks * C;
+        s->pmass = (double*)s->alloc(sizeof(double) * np);
+        s->pmin = (float*)s->alloc(sizeof(float) * np);
+        s->pmax = (float*)s->alloc(sizeof(float) * np);
+        s->pnf = (unsigned long long*)s->alloc(sizeof(unsigned long long) * np);
+
+        AH_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        for (void* p : scratch) (void)hipFree(p);
+        throw;
+    }
+    for (void* p : scratch) (void)hipFree(p);
+}
+
+void step_fft(ah_sim* s, float* conv_out, int conv_k) {
+    const ah_lenia_desc& d = s->d;
+    const int W = (int)d.width, H = (int)d.height, C = (int)d.channels, K = (int)d.n_kernels;
+    launch_row_r2c(s->stream, &s->prof, W, H, C, s->state, s->spec_src, s->twW);
+    launch_col_conv(s->stream, &s->prof, W, H, C, s->spec_src, s->spec_k, s->khat, s->kpack, s->twH, s->ksrc, K);
+    launch_row_c2r(s->stream, &s->prof, W, H, C, K, s->spec_k, s->state, s->state, s->kp, d.dt, s->twW, conv_out,
+                   conv_k);
+}
+
+template <typename F>
+ah_status guard(F&& f) {
+    try {
+        f();
+        return AH_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return AH_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return AH_ERR_DEVICE;
+    } catch (...) {
+        set_error("unknown failure");
+        return AH_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ah_last_error(void) { return ah::g_error.c_str(); }
+
+ah_status ah_lenia_build(const char* id, uint32_t width, uint32_t height, ah_lenia_desc* out) {
+    return guard([&] {
+        if (!id || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        if (std::strcmp(id, "lenia") != 0) throw Error{AH_ERR_INVALID, std::string("unknown automaton id: ") + id};
+        // automata::fit_size rounds to powers of two (mod.rs:51-61).
+        auto p2 = [](uint32_t v) {
+            uint32_t r = 1;
+            while (r < v) r <<= 1;
+            return r;
+        };
+        ah_lenia_desc d{};
+        d.width = p2(width);
+        d.height = p2(height);
+        d.channels = 1;
+        d.n_kernels = 1;
+        // lenia(w, h, 13.0) (spectral.rs:286-352,489).
+        d.kernels[0] = ah_kernel{13.0f, 1, {1.0f, 0.0f, 0.0f}, 0.15f, 0.015f, 1.0f, 0, 0};
+        d.dt = 0.1f;
+        d.path = AH_PATH_FFT_FP32;
+        d.init = AH_INIT_ORBIUM;
+        *out = d;
+    });
+}
+
+ah_status ah_sim_new(const ah_lenia_desc* desc, uint32_t seed, ah_sim** out) {
+    return guard([&] {
+        if (!desc || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        *out = nullptr;
+        validate(*desc);
+        std::unique_ptr<ah_sim> s(new ah_sim());
+        s->d = *desc;
+        s->seed = seed;
+        setup(s.get());
+        *out = s.release();
+    });
+}
+
+ah_status ah_sim_step(ah_sim* s, uint32_t steps, uint64_t* generations) {
+    return guard([&] {
+        if (!s) throw Error{AH_ERR_INVALID, "null sim"};
+        AH_HIP(hipSetDevice(s->device));
+        for (uint32_t i = 0; i < steps; ++i) step_fft(s, nullptr, 0);
+        s->generation += steps;
+        if (generations) *generations = steps;
+    });
+}
+
+ah_status ah_sim_wait(ah_sim* s) {
+    return guard([&] {
+        if (!s) throw Error{AH_ERR_INVALID, "null sim"};
+        AH_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+ah_status ah_sim_read_field(ah_sim* s, const char* name, void* dst, size_t bytes) {
+    return guard([&] {
+        if (!s || !name || !dst) throw Error{AH_ERR_INVALID, "null argument"};
+        AH_HIP(hipSetDevice(s->device));
+        const size_t plane = sizeof(float) * s->cells();
+        if (std::strcmp(name, "state") == 0) {
+            if (bytes != plane * s->d.channels) throw Error{AH_ERR_SIZE, "state is channels*height*width*4 bytes"};
+            AH_HIP(hipMemcpyAsync(dst, s->state, bytes, hipMemcpyDeviceToHost, s->stream));
+            AH_HIP(hipStreamSynchronize(s->stream));
+            return;
+        }
+        if (std::strncmp(name, "conv", 4) == 0 && name[4] >= '0' && name[4] <= '9' && name[5] == 0) {
+            int k = name[4] - '0';
+            if (k >= (int)s->d.n_kernels) throw Error{AH_ERR_UNKNOWN_FIELD, std::string("no kernel for ") + name};
+            if (bytes != plane) throw Error{AH_ERR_SIZE, "conv fields are height*width*4 bytes"};
+            float* u = nullptr;
+            AH_HIP(hipMallocAsync((void**)&u, plane, s->stream));
+            step_fft(s, u, k);
+            hipError_t e = hipMemcpyAsync(dst, u, bytes, hipMemcpyDeviceToHost, s->stream);
+            (void)hipFreeAsync(u, s->stream);
+            AH_HIP(e);
+            AH_HIP(hipStreamSynchronize(s->stream));
+            return;
+        }
+        throw Error{AH_ERR_UNKNOWN_FIELD, std::string("unknown field: ") + name};
+    });
+}
+
+ah_status ah_sim_write_field(ah_sim* s, const char* name, const void* src, size_t bytes) {
+    return guard([&] {
+        if (!s || !name || !src) throw Error{AH_ERR_INVALID, "null argument"};
+        AH_HIP(hipSetDevice(s->device));
+        if (std::strcmp(name, "state") != 0) throw Error{AH_ERR_UNKNOWN_FIELD, std::string("not writable: ") + name};
+        if (bytes != sizeof(float) * s->cells() * s->d.channels)
+            throw Error{AH_ERR_SIZE, "state is channels*height*width*4 bytes"};
+        AH_HIP(hipMemcpyAsync(s->state, src, bytes, hipMemcpyHostToDevice, s->stream));
+        AH_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+ah_status ah_sim_stats(ah_sim* s, ah_stats* out) {
+    return guard([&] {
+        if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        AH_HIP(hipSetDevice(s->device));
+        const int C = (int)s->d.channels, nb = s->stat_blocks;
+        launch_stats(s->stream, &s->prof, s->state, s->cells(), C, s->pmass, s->pmin, s->pmax, s->pnf, nb);
+        ah_stats r{};
+        for (int c = 0; c < C; ++c) {
+            size_t o = (size_t)c * nb;
+            unsigned long long nf = 0;
+            AH_HIP(hipMemcpyAsync(&r.mass[c], s->pmass + o, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            AH_HIP(hipMemcpyAsync(&r.min[c], s->pmin + o, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+            AH_HIP(hipMemcpyAsync(&r.max[c], s->pmax + o, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+            AH_HIP(hipMemcpyAsync(&nf, s->pnf + o, sizeof(nf), hipMemcpyDeviceToHost, s->stream));
+            AH_HIP(hipStreamSynchronize(s->stream));
+            r.nonfinite += nf;
+        }
+        r.generation = s->generation;
+        *out = r;
+    });
+}
+
+ah_status ah_sim_info(const ah_sim* s, ah_info* out) {
+    return guard([&] {
+        if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        ah_info i{};
+        i.width = s->d.width;
+        i.height = s->d.height;
+        i.channels = s->d.channels;
+        i.n_kernels = s->d.n_kernels;
+        i.path = s->d.path;
+        i.cells_per_step = (uint64_t)s->cells();
+        i.generations_per_step = 1;
+        i.total_bytes = s->total_bytes;
+        i.generation = s->generation;
+        *out = i;
+    });
+}
+
+void ah_sim_destroy(ah_sim* s) { delete s; }
+
+ah_status ah_sim_profile(ah_sim* s, int enable) {
+    return guard([&] {
+        if (!s) throw Error{AH_ERR_INVALID, "null sim"};
+        AH_HIP(hipStreamSynchronize(s->stream));
+        s->prof.reset();
+        s->prof.enabled = enable != 0;
+    });
+}
+
+ah_status ah_sim_profile_read(ah_sim* s, ah_kernel_time* out, uint32_t capacity, uint32_t* count) {
+    return guard([&] {
+        if (!s || !count || (capacity && !out)) throw Error{AH_ERR_INVALID, "null argument"};
+        AH_HIP(hipStreamSynchronize(s->stream));
+        std::vector<ah_kernel_time> agg;
+        for (auto& r : s->prof.recs) {
+            float ms = 0.0f;
+            AH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            auto it = std::find_if(agg.begin(), agg.end(),
+                                   [&](const ah_kernel_time& t) { return std::strcmp(t.name, r.name) == 0; });
+            if (it == agg.end()) {
+                ah_kernel_time t{};
+                std::snprintf(t.name, sizeof(t.name), "%s", r.name);
+                agg.push_back(t);
+                it = agg.end() - 1;
+            }
+            it->launches += 1;
+            it->total_ms += ms;
+        }
+        uint32_t n = (uint32_t)std::min<size_t>(agg.size(), capacity);
+        for (uint32_t i = 0; i < n; ++i) out[i] = agg[i];
+        *count = (uint32_t)agg.size();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,513 @@
+// Plain FP32 FFT step for gfx950: three fused passes per step.
+//
+// The reference runs Lenia as 4*log2(N)+5 dispatches per step, one radix-2
+// Stockham stage each, ping-ponging a full complex field through memory
+// (crates/auto-core/src/automata/spectral.rs:14-64 fft_pass, :199-264 step).
+// Here a step is three passes, each holding whole transforms in LDS:
+//
+//   1. row_r2c   state rows -> real FFT along x (length-W/2 complex FFT plus
+//                the even/odd split) -> half spectrum, 4 rows per workgroup so
+//                every 32-byte spectrum block gets a 128-byte contiguous write.
+//   2. col_conv  one 4-column block of the half spectrum, all H rows:
+//                forward FFT along y, multiply by the real kernel spectrum
+//                (scaled by 1/(W*H)), inverse FFT along y, written in place.
+//   3. row_c2r   inverse real FFT along x, then the growth map, dt and clip
+//                (spectral.rs:236-264,337-345) written straight to the state.
+//
+// Inside LDS the transforms are Stockham autosort stages of radix 8 (with a
+// final radix 2 or 4 stage), the same stage recurrence as fft_pass
+// generalised to radix R: thread j, k = j mod Ls, loads v[r] = x[j + r*M/R],
+// twiddles by w^(r*k) with w = exp(sign*2*pi*i/(Ls*R)), runs a radix-R DFT in
+// registers and stores at (j-k)*R + k + r*Ls.  Twiddles come from a table
+// generated in FP64 (init.hip), not from per-thread f32 cos/sin.
+//
+// Half-spectrum layout (complex64): spec[kb][y][c], kx = 4*kb + c.  Bin kx=0
+// packs the two real bins X[0] and X[W/2] as (re, im); the column pass treats
+// that column as two real sequences sharing one complex transform.
+#include "ah_internal.h"
+
+namespace ah {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conj2(float2 a) { return make_float2(a.x, -a.y); }
+// a * (SIGN * i)
+template <int SIGN>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+    return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// LDS index padding: one float2 every 32 breaks the power-of-two strides.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr size_t padded(size_t n) { return n + (n >> 5) + 1; }
+
+template <int SIGN>
+__device__ __forceinline__ void dft2(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft8(float2* v) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<SIGN>(e);
+    dft4<SIGN>(o);
+    const float c = 0.70710678118654752f;
+    // w8^k = exp(SIGN*2*pi*i*k/8)
+    float2 o1 = cmul(o[1], make_float2(c, SIGN * c));
+    float2 o2 = mul_si<SIGN>(o[2]);
+    float2 o3 = cmul(o[3], make_float2(-c, SIGN * c));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) dft2<SIGN>(v);
+    else if constexpr (R == 4) dft4<SIGN>(v);
+    else dft8<SIGN>(v);
+}
+
+// One radix-R Stockham stage over `nseq` sequences of length M stored back
+// to back in LDS.  Every thread owns E/R butterflies; all loads happen before
+// the first barrier, all stores after it.  tw[t] = exp(-2*pi*i*t/NT);
+// the twiddle for r*k over Ls*R points is tw[r*k*(NT/(Ls*R))].
+template <int R, int E, int SIGN>
+__device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, const float2* __restrict__ tw, int NT) {
+    constexpr int NB = E / R;
+    const int per_seq = M / R;
+    const int nthreads = blockDim.x, tid = threadIdx.x;
+    const int tstep = NT / (Ls * R);
+    float2 v[NB][R];
+    int base[NB], kk[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        int b = tid + i * nthreads;
+        int seq = b / per_seq, j = b - seq * per_seq;
+        int off = seq * M;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = lds[pad(off + j + r * per_seq)];
+        int k = j & (Ls - 1);
+        kk[i] = k;
+        base[i] = off + (j - k) * R + k;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        if (Ls > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[r * kk[i] * tstep];
+                if (SIGN > 0) w.y = -w.y;
+                v[i][r] = cmul(v[i][r], w);
+            }
+        }
+        dft<R, SIGN>(v[i]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[pad(base[i] + r * Ls)] = v[i][r];
+    }
+    __syncthreads();
+}
+
+// Complex FFT of length M (power of two, >= 8) on nseq sequences in LDS.
+// E = elements per thread = nseq*M/blockDim.x (8 or 16).
+template <int E, int SIGN>
+__device__ void fft_lds(float2* lds, int M, int nseq, const float2* __restrict__ tw, int NT) {
+    int Ls = 1;
+    while (Ls * 8 <= M) {
+        stage<8, E, SIGN>(lds, M, nseq, Ls, tw, NT);
+        Ls *= 8;
+    }
+    if (Ls * 4 == M) stage<4, E, SIGN>(lds, M, nseq, Ls, tw, NT);
+    else if (Ls * 2 == M) stage<2, E, SIGN>(lds, M, nseq, Ls, tw, NT);
+}
+
+// ------------------------------------------------------------ pass 1
+// Grid: H/rows workgroups, rows consecutive rows each; block = rows*M/E threads.
+template <int E>
+__global__ void __launch_bounds__(1024) k_row_r2c(int W, int H, int rows, const float* __restrict__ state,
+                                                  float2* __restrict__ spec, const float2* __restrict__ twW) {
+    extern __shared__ float2 lds[];
+    const int M = W >> 1;
+    const int y0 = blockIdx.x * rows;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ch = blockIdx.y;
+    state += (size_t)ch * W * H;
+    spec += (size_t)ch * (W >> 1) * H;
+    // rows consecutive rows are one contiguous run of rows*M float2.
+    const float4* src = reinterpret_cast<const float4*>(state + (size_t)y0 * W);
+    const int n4 = rows * M / 2;
+    for (int i = tid; i < n4; i += nt) {
+        float4 v = src[i];
+        lds[pad(2 * i)] = make_float2(v.x, v.y);
+        lds[pad(2 * i + 1)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
+    // Even/odd split: X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[M-k])/2,
+    // O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output float4 q
+    // holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
+    float4* dst = reinterpret_cast<float4*>(spec);
+    for (int q = tid; q < n4; q += nt) {
+        int kb = q / (2 * rows), rem = q - kb * 2 * rows;
+        int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
+        const float2* z = lds;  // padded access below
+        int off = row * M;
+        float2 out[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            int k = kx + t;
+            if (k == 0) {
+                float2 z0 = z[pad(off)];
+                out[t] = make_float2(z0.x + z0.y, z0.x - z0.y);
+            } else {
+                float2 a = z[pad(off + k)], b = conj2(z[pad(off + M - k)]);
+                float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+                float2 d = csub(a, b);
+                float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // -i/2 * d
+                out[t] = cadd(e, cmul(twW[k], o));
+            }
+        }
+        size_t o4 = ((size_t)kb * H + y0 + row) * 2 + (rem & 1);
+        dst[o4] = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
+    }
+}
+
+// ------------------------------------------------------------ pass 2
+// Grid: (W/8 blocks, source channels); block = 4*H/E threads.
+// spec_in: [channels][W/8][H][4]; spec_out: [n_kernels][W/8][H][4] (may alias
+// spec_in when one kernel reads one channel).  khat: [n_kernels][W/8][H/2+1][4],
+// kpack: [n_kernels][H/2+1].
+struct ColArgs {
+    int W, H;
+    const float2* spec_in;
+    float2* spec_out;
+    const float* khat;
+    const float2* kpack;
+    const float2* twH;
+    int n_kernels;
+    int ksrc[AH_MAX_KERNELS];
+};
+
+// Multiply the transformed block in LDS by kernel k's spectrum, in place.
+// Pairs (ky, H-ky) are owned by one thread, which the packed kx=0 column
+// needs: D[ky] = A C[ky] + B conj C[-ky] with A=(K0+KN)/2, B=(K0-KN)/2.
+__device__ __forceinline__ void multiply_block(float2* lds, const ColArgs& a, int kb, int k) {
+    const int H = a.H, half = H >> 1;
+    const int npairs = half + 1;
+    const float* kh = a.khat + ((size_t)k * (a.W >> 3) + kb) * (size_t)npairs * BC;
+    for (int t = threadIdx.x; t < npairs * BC; t += blockDim.x) {
+        int p = t / BC, c = t - p * BC;
+        int y1 = p, y2 = (H - p) & (H - 1);
+        float2 c1 = lds[pad(c * H + y1)], c2 = lds[pad(c * H + y2)];
+        float2 d1, d2;
+        if (kb == 0 && c == 0) {
+            float2 ab = a.kpack[(size_t)k * npairs + p];
+            d1 = cadd(make_float2(ab.x * c1.x, ab.x * c1.y), make_float2(ab.y * c2.x, -ab.y * c2.y));
+            d2 = cadd(make_float2(ab.x * c2.x, ab.x * c2.y), make_float2(ab.y * c1.x, -ab.y * c1.y));
+        } else {
+            float kv = kh[t];
+            d1 = make_float2(kv * c1.x, kv * c1.y);
+            d2 = make_float2(kv * c2.x, kv * c2.y);
+        }
+        lds[pad(c * H + y1)] = d1;
+        if (y2 != y1) lds[pad(c * H + y2)] = d2;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void store_block(const float2* lds, float2* dst, int H) {
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int q = threadIdx.x; q < 2 * H; q += blockDim.x) {
+        int y = q >> 1, cp = (q & 1) * 2;
+        float2 u = lds[pad(cp * H + y)], v = lds[pad((cp + 1) * H + y)];
+        d4[q] = make_float4(u.x, u.y, v.x, v.y);
+    }
+}
+
+template <int E>
+__global__ void __launch_bounds__(1024) k_col_conv(ColArgs a) {
+    extern __shared__ float2 lds[];
+    const int H = a.H;
+    const int kb = blockIdx.x, ch = blockIdx.y;
+    const size_t blk = (size_t)H * BC;  // float2 per block
+    const float2* src = a.spec_in + ((size_t)ch * (a.W >> 3) + kb) * blk;
+    {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        for (int q = threadIdx.x; q < 2 * H; q += blockDim.x) {
+            float4 v = s4[q];
+            int y = q >> 1, cp = (q & 1) * 2;
+            lds[pad(cp * H + y)] = make_float2(v.x, v.y);
+            lds[pad((cp + 1) * H + y)] = make_float2(v.z, v.w);
+        }
+    }
+    __syncthreads();
+    fft_lds<E, -1>(lds, H, BC, a.twH, H);
+    int users = 0, last = -1;
+    for (int k = 0; k < a.n_kernels; ++k)
+        if (a.ksrc[k] == ch) { ++users; last = k; }
+    if (users == 1) {
+        multiply_block(lds, a, kb, last);
+        fft_lds<E, 1>(lds, H, BC, a.twH, H);
+        store_block(lds, a.spec_out + ((size_t)last * (a.W >> 3) + kb) * blk, H);
+        return;
+    }
+    // Several kernels read this channel: keep its spectrum in registers and
+    // rebuild the LDS copy before each kernel's multiply.
+    float2 keep[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) keep[i] = lds[pad(threadIdx.x + i * blockDim.x)];
+    bool first = true;
+    for (int k = 0; k < a.n_kernels; ++k) {
+        if (a.ksrc[k] != ch) continue;
+        if (!first) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
+            __syncthreads();
+        }
+        first = false;
+        multiply_block(lds, a, kb, k);
+        fft_lds<E, 1>(lds, H, BC, a.twH, H);
+        store_block(lds, a.spec_out + ((size_t)k * (a.W >> 3) + kb) * blk, H);
+    }
+}
+
+// ------------------------------------------------------------ pass 3
+struct RowArgs {
+    int W, H, rows;
+    const float2* spec;  // [n_kernels][W/8][H][4]
+    const float* state_in;
+    float* state_out;
+    const float2* twW;
+    float dt;
+    int n_kernels;
+    KernelParams kp[AH_MAX_KERNELS];
+    float* conv_out;  // when set: write kernel conv_k's potential instead of updating
+    int conv_k;
+};
+
+// Load kernel k's spectrum rows into LDS and turn them into the length-M
+// complex sequence whose inverse FFT interleaves the real output:
+// Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) conj(w^k).
+template <int E>
+__device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
+    const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
+    const float4* s4 = reinterpret_cast<const float4*>(a.spec + (size_t)k * M * H);
+    const int n4 = rows * M / 2;
+    for (int q = threadIdx.x; q < n4; q += blockDim.x) {
+        int kb = q / (2 * rows), rem = q - kb * 2 * rows;
+        int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
+        float4 v = s4[((size_t)kb * H + y0 + row) * 2 + (rem & 1)];
+        lds[pad(row * M + kx)] = make_float2(v.x, v.y);
+        lds[pad(row * M + kx + 1)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    const int half = M >> 1;
+    for (int t = threadIdx.x; t < rows * (half + 1); t += blockDim.x) {
+        int row = t / (half + 1), kk = t - row * (half + 1);
+        int off = row * M;
+        if (kk == 0) {
+            float2 p = lds[pad(off)];
+            lds[pad(off)] = make_float2(p.x + p.y, p.x - p.y);
+            continue;
+        }
+        int k2 = M - kk;
+        float2 x1 = lds[pad(off + kk)], x2 = lds[pad(off + k2)];
+        // Z[kk] from (x1, x2); Z[k2] from (x2, x1) with w^k2 = -conj(w^kk).
+        float2 w1 = a.twW[kk];
+        float2 s1 = cadd(x1, conj2(x2)), d1 = csub(x1, conj2(x2));
+        float2 z1 = cadd(s1, cmul(make_float2(-d1.y, d1.x), conj2(w1)));
+        lds[pad(off + kk)] = z1;
+        if (k2 != kk) {
+            float2 w2 = make_float2(-w1.x, -w1.y);  // conj(w^k2) = conj(-conj(w^kk)) = -w^kk
+            float2 s2 = cadd(x2, conj2(x1)), d2 = csub(x2, conj2(x1));
+            float2 z2 = cadd(s2, cmul(make_float2(-d2.y, d2.x), w2));
+            lds[pad(off + k2)] = z2;
+        }
+    }
+    __syncthreads();
+}
+
+template <int E>
+__global__ void __launch_bounds__(1024) k_row_c2r(RowArgs a) {
+    extern __shared__ float2 lds[];
+    const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
+    const int y0 = blockIdx.x * rows;
+    const int ch = blockIdx.y;  // destination channel
+    const int nt = blockDim.x, tid = threadIdx.x;
+    constexpr int NQ = E / 2;  // float4 (4 output cells) per thread
+    float acc[NQ][4];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k = 0; k < a.n_kernels; ++k) {
+        const KernelParams kp = a.kp[k];
+        if (a.conv_out ? (k != a.conv_k) : ((int)kp.dst != ch)) continue;
+        load_c2r<E>(lds, a, k, y0);
+        fft_lds<E, 1>(lds, M, rows, a.twW, W);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            int q = tid + i * nt;  // float4 index within the rows*W/4 cells
+            float2 u0 = lds[pad(2 * q)], u1 = lds[pad(2 * q + 1)];
+            float u[4] = {u0.x, u0.y, u1.x, u1.y};
+            if (a.conv_out) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = u[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float d = (u[j] - kp.mu) * kp.inv_sigma;
+                    acc[i][j] += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const size_t cell0 = (size_t)y0 * W;
+    if (a.conv_out) {
+        float4* o4 = reinterpret_cast<float4*>(a.conv_out + cell0);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) o4[tid + i * nt] = make_float4(acc[i][0], acc[i][1], acc[i][2], acc[i][3]);
+        return;
+    }
+    const size_t plane = (size_t)W * H * ch;
+    const float4* s4 = reinterpret_cast<const float4*>(a.state_in + plane + cell0);
+    float4* o4 = reinterpret_cast<float4*>(a.state_out + plane + cell0);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        int q = tid + i * nt;
+        float4 s = s4[q];
+        float r[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = fminf(fmaxf(r[j] + a.dt * acc[i][j], 0.0f), 1.0f);
+        o4[q] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// Dynamic LDS above 64 KiB must be allowed per kernel; done once per kernel.
+template <typename K>
+void set_lds(K kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    AH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+// Rows per row-pass workgroup: at least RW, enough to give >= 256 threads.
+int row_rows(int W, int H) {
+    int M = W / 2;
+    int rows = RW;
+    while (rows * M < 2048 && rows < H) rows *= 2;
+    return rows;
+}
+
+}  // namespace
+
+size_t fft_lds_limit() { return 160 * 1024; }
+
+void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, const float* state, float2* spec,
+                    const float2* twW) {
+    const int M = W / 2, rows = row_rows(W, H);
+    const int elems = rows * M;
+    const int E = elems > 8 * 1024 ? 16 : 8;
+    const int threads = elems / E;
+    const size_t lds = padded(elems) * sizeof(float2);
+    Launch L(p, s, "row_r2c");
+    if (E == 8) {
+        set_lds(k_row_r2c<8>, lds);
+        hipLaunchKernelGGL(k_row_r2c<8>, dim3(H / rows, channels), dim3(threads), lds, s, W, H, rows, state, spec,
+                           twW);
+    } else {
+        set_lds(k_row_r2c<16>, lds);
+        hipLaunchKernelGGL(k_row_r2c<16>, dim3(H / rows, channels), dim3(threads), lds, s, W, H, rows, state, spec,
+                           twW);
+    }
+    AH_HIP(hipGetLastError());
+}
+
+void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
+                     float2* spec_out, const float* khat, const float2* kpack, const float2* twH,
+                     const int* ksrc, int n_kernels) {
+    ColArgs a;
+    a.W = W;
+    a.H = H;
+    a.spec_in = spec_in;
+    a.spec_out = spec_out;
+    a.khat = khat;
+    a.kpack = kpack;
+    a.twH = twH;
+    a.n_kernels = n_kernels;
+    for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
+    const int elems = BC * H;
+    const int E = elems > 8 * 1024 ? 16 : 8;
+    const int threads = elems / E;
+    const size_t lds = padded(elems) * sizeof(float2);
+    Launch L(p, s, "col_conv");
+    if (E == 8) {
+        set_lds(k_col_conv<8>, lds);
+        hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 8, channels), dim3(threads), lds, s, a);
+    } else {
+        set_lds(k_col_conv<16>, lds);
+        hipLaunchKernelGGL(k_col_conv<16>, dim3(W / 8, channels), dim3(threads), lds, s, a);
+    }
+    AH_HIP(hipGetLastError());
+}
+
+void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
+                    const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
+                    float* conv_out, int conv_k) {
+    RowArgs a;
+    a.W = W;
+    a.H = H;
+    a.rows = row_rows(W, H);
+    a.spec = spec;
+    a.state_in = state_in;
+    a.state_out = state_out;
+    a.twW = twW;
+    a.dt = dt;
+    a.n_kernels = n_kernels;
+    for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
+    a.conv_out = conv_out;
+    a.conv_k = conv_k;
+    const int M = W / 2;
+    const int elems = a.rows * M;
+    const int E = elems > 8 * 1024 ? 16 : 8;
+    const int threads = elems / E;
+    const size_t lds = padded(elems) * sizeof(float2);
+    const int gy = conv_out ? 1 : channels;
+    Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
+    if (E == 8) {
+        set_lds(k_row_c2r<8>, lds);
+        hipLaunchKernelGGL(k_row_c2r<8>, dim3(H / a.rows, gy), dim3(threads), lds, s, a);
+    } else {
+        set_lds(k_row_c2r<16>, lds);
+        hipLaunchKernelGGL(k_row_c2r<16>, dim3(H / a.rows, gy), dim3(threads), lds, s, a);
+    }
+    AH_HIP(hipGetLastError());
+}
+
+}  // namespace ah

@@ -1,0 +1,371 @@
+// Set-up kernels (FP64 where it matters) and the state statistics reduction.
+//
+//  - twiddle tables: exp(-2*pi*i*t/n) evaluated in FP64, rounded to FP32
+//    (the reference forms cos/sin of an f32 angle per butterfly instead,
+//    crates/auto-core/src/automata/spectral.rs:47-48);
+//  - kernel taps: Chan's multi-ring shell of exp-bump cores, normalised to
+//    sum 1 in FP64 (the reference's core and box sum: spectral.rs:75-83,
+//    288-303; one ring with beta={1} is exactly its kernel);
+//  - kernel spectrum for the FFT path: the kernel is even in x and in y, so
+//    its DFT is real and separable into two cosine sums, done in FP64 and
+//    stored in FP32 already divided by W*H (which the reference applies after
+//    the inverse transform instead, spectral.rs:215,227-234);
+//  - seeds: the Orbium lattice (spectral.rs:183-197,313-335) and the
+//    patch/blob inits, all from the PCG hash of program.rs:140-147;
+//  - statistics: FP64 mass, min, max and non-finite count per channel, with
+//    wavefront shuffles and a fixed-order second pass (bitwise reproducible).
+#include "ah_internal.h"
+
+namespace ah {
+
+namespace {
+
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ uint32_t rand3(uint32_t a, uint32_t b, uint32_t c) { return pcg(pcg(pcg(a) ^ b) ^ c); }
+__device__ __forceinline__ float unit_f(uint32_t r) { return (float)r * (1.0f / 4294967296.0f); }
+
+__constant__ uint8_t c_orbium[400] = {
+    0, 0, 0, 0, 0, 0, 0, 13, 4, 0, 0, 0, 0, 0, 0, 60, 0, 0, 0, 0,
+    0, 0, 0, 0, 0, 0, 35, 53, 54, 21, 18, 25, 26, 18, 1, 65, 0, 0, 0, 0,
+    0, 0, 0, 0, 0, 22, 68, 91, 98, 87, 48, 47, 44, 47, 45, 27, 89, 0, 0, 0,
+    0, 0, 0, 0, 3, 17, 89, 116, 119, 97, 33, 20, 14, 16, 31, 55, 180, 0, 0, 0,
+    0, 0, 0, 9, 33, 47, 87, 103, 98, 72, 34, 0, 0, 0, 0, 12, 102, 84, 0, 0,
+    1, 0, 4, 35, 43, 34, 28, 63, 69, 67, 53, 0, 0, 0, 0, 0, 5, 220, 0, 0,
+    60, 0, 26, 44, 20, 0, 0, 51, 79, 94, 95, 61, 0, 0, 0, 0, 0, 116, 40, 0,
+    0, 31, 47, 28, 0, 0, 0, 69, 109, 129, 137, 130, 0, 0, 0, 0, 0, 0, 132, 0,
+    0, 150, 55, 8, 0, 0, 0, 48, 135, 162, 174, 174, 107, 0, 0, 0, 0, 0, 109, 0,
+    0, 141, 60, 0, 0, 0, 0, 7, 158, 193, 214, 218, 206, 27, 0, 0, 0, 0, 72, 25,
+    0, 0, 152, 0, 0, 0, 0, 0, 170, 223, 245, 253, 248, 143, 0, 0, 0, 0, 57, 36,
+    0, 0, 214, 0, 0, 0, 0, 0, 129, 247, 255, 255, 255, 233, 71, 0, 0, 6, 56, 34,
+    0, 0, 141, 19, 0, 0, 0, 0, 85, 255, 255, 250, 255, 248, 142, 40, 13, 30, 62, 22,
+    0, 0, 8, 114, 0, 0, 0, 0, 45, 217, 255, 228, 220, 221, 166, 85, 50, 55, 59, 10,
+    0, 0, 0, 108, 29, 0, 0, 0, 29, 158, 224, 210, 199, 189, 156, 104, 75, 68, 42, 0,
+    0, 0, 0, 20, 85, 19, 0, 0, 30, 108, 172, 184, 173, 160, 134, 103, 79, 57, 16, 0,
+    0, 0, 0, 0, 42, 66, 38, 30, 44, 86, 127, 142, 139, 127, 109, 86, 62, 30, 0, 0,
+    0, 0, 0, 0, 0, 37, 59, 60, 66, 81, 99, 108, 105, 92, 78, 58, 32, 5, 0, 0,
+    0, 0, 0, 0, 0, 0, 18, 43, 58, 64, 70, 71, 66, 59, 42, 24, 5, 0, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 0, 15, 26, 33, 34, 30, 20, 11, 0, 0, 0, 0, 0};
+
+__global__ void k_cos_table64(double* tab, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) tab[t] = cos(2.0 * M_PI * (double)t / (double)n);
+}
+
+__global__ void k_twiddles(float2* tw, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        double sn, cs;
+        sincos(2.0 * M_PI * (double)t / (double)n, &sn, &cs);
+        tw[t] = make_float2((float)cs, (float)-sn);
+    }
+}
+
+struct ShellArgs {
+    double radius;
+    int rings;
+    double beta[AH_MAX_RINGS];
+    int reach;
+};
+
+__device__ double shell64(double r, const ShellArgs& a) {
+    if (r <= 0.0 || r >= a.radius) return 0.0;
+    double b = (double)a.rings * r / a.radius;
+    int k = (int)floor(b);
+    if (k >= a.rings) return 0.0;
+    double q = b - (double)k;
+    if (q <= 0.0 || q >= 1.0) return 0.0;
+    return a.beta[k] * exp(4.0 - 1.0 / (q * (1.0 - q)));
+}
+
+// One workgroup: taps of the (2*reach+1)^2 box, FP64 sum (fixed order), normalise.
+__global__ void __launch_bounds__(256) k_kernel_taps64(ShellArgs a, double* taps, double* sum_out) {
+    __shared__ double part[256];
+    const int side = 2 * a.reach + 1, n = side * side;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        int dy = i / side - a.reach, dx = i % side - a.reach;
+        double v = shell64(sqrt((double)(dx * dx + dy * dy)), a);
+        taps[i] = v;
+        acc += v;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double sum = part[0];
+    for (int i = threadIdx.x; i < n; i += 256) taps[i] /= sum;
+    if (threadIdx.x == 0) *sum_out = sum;
+}
+
+// T[dy][kx] = sum_dx K[dy][dx] cos(2 pi kx dx / W), dy in [0, 2*reach], kx in [0, W/2].
+__global__ void k_khat_rows64(int W, int reach, const double* taps, const double* cosW, double* T) {
+    const int nk = W / 2 + 1, side = 2 * reach + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk * side) return;
+    int dyi = i / nk, kx = i - dyi * nk;
+    const double* row = taps + (size_t)dyi * side + reach;
+    double acc = row[0];
+    for (int dx = 1; dx <= reach; ++dx) acc += 2.0 * row[dx] * cosW[(kx * dx) & (W - 1)];
+    T[i] = acc;
+}
+
+// khat[kb][p][c] = (sum_dy T[dy][kx] cos(2 pi p dy / H)) / (W*H), p in [0, H/2];
+// kx = 0 and kx = W/2 go to kpack[p] = ((K0+KN)/2, (K0-KN)/2).
+__global__ void k_khat64(int W, int H, int reach, const double* T, const double* cosH, float* khat, float2* kpack) {
+    const int nk = W / 2 + 1, np = H / 2 + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk * np) return;
+    int p = i / nk, kx = i - p * nk;
+    double acc = T[(size_t)reach * nk + kx];
+    for (int dy = 1; dy <= reach; ++dy)
+        acc += (T[(size_t)(reach + dy) * nk + kx] + T[(size_t)(reach - dy) * nk + kx]) * cosH[(p * dy) & (H - 1)];
+    const double scale = 1.0 / ((double)W * (double)H);
+    acc *= scale;
+    if (kx == 0 || kx == W / 2) {
+        // kx = 0 owns the pair; recompute the partner bin so one thread writes kpack[p].
+        if (kx != 0) return;
+        double accN = T[(size_t)reach * nk + W / 2];
+        for (int dy = 1; dy <= reach; ++dy)
+            accN += (T[(size_t)(reach + dy) * nk + W / 2] + T[(size_t)(reach - dy) * nk + W / 2]) *
+                    cosH[(p * dy) & (H - 1)];
+        accN *= scale;
+        kpack[p] = make_float2((float)(0.5 * (acc + accN)), (float)(0.5 * (acc - accN)));
+        return;
+    }
+    int kb = kx >> 2, c = kx & 3;
+    khat[((size_t)kb * np + p) * BC + c] = (float)acc;
+}
+
+struct SeedArgs {
+    int W, H, C, init;
+    uint32_t seed, size, count;
+    const uint32_t* corners;  // BLOBS: (x, y) per blob
+};
+
+__global__ void k_seed(SeedArgs a, float* state) {
+    const size_t cells = (size_t)a.W * a.H;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * a.C) return;
+    int c = (int)(i / cells);
+    size_t ci = i - (size_t)c * cells;
+    uint32_t x = (uint32_t)(ci % a.W), y = (uint32_t)(ci / a.W);
+    float v = 0.0f;
+    if (a.init == AH_INIT_ORBIUM) {
+        if (c == 0) {
+            uint32_t r = rand3(x / 64u, y / 64u, a.seed);
+            uint32_t px = x % 64u, py = y % 64u;
+            if ((r & 1u) == 0u && px < 20u && py < 20u) {
+                uint32_t ix = px, iy = py;
+                switch ((r >> 8u) & 3u) {
+                    case 1u: ix = py; iy = 19u - px; break;
+                    case 2u: ix = 19u - px; iy = 19u - py; break;
+                    case 3u: ix = 19u - py; iy = px; break;
+                    default: break;
+                }
+                v = (float)c_orbium[iy * 20u + ix] / 255.0f;
+            }
+        }
+    } else if (a.init == AH_INIT_PATCH) {
+        uint32_t x0 = (a.W - a.size) / 2, y0 = (a.H - a.size) / 2;
+        if (x >= x0 && x < x0 + a.size && y >= y0 && y < y0 + a.size) v = unit_f(rand3(x, y, a.seed + c));
+    } else if (a.init == AH_INIT_BLOBS) {
+        for (uint32_t b = 0; b < a.count; ++b) {
+            uint32_t dx = (x + a.W - a.corners[2 * b]) % a.W, dy = (y + a.H - a.corners[2 * b + 1]) % a.H;
+            if (dx < a.size && dy < a.size) {
+                v = unit_f(rand3(x, y, a.seed + c));
+                break;
+            }
+        }
+    }
+    state[i] = v;
+}
+
+// ---------------------------------------------------------------- stats
+constexpr int STATS_THREADS = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sumu(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Grid (nblocks, channels): each block reduces a strided slice of one channel.
+__global__ void __launch_bounds__(STATS_THREADS) k_stats_partial(const float* state, size_t cells, double* pm,
+                                                                 float* pmin, float* pmax, unsigned long long* pnf) {
+    const int c = blockIdx.y;
+    const float4* s4 = reinterpret_cast<const float4*>(state + (size_t)c * cells);
+    const size_t n4 = cells / 4;
+    double m = 0.0;
+    float lo = INFINITY, hi = -INFINITY;
+    unsigned long long nf = 0;
+    for (size_t i = (size_t)blockIdx.x * STATS_THREADS + threadIdx.x; i < n4; i += (size_t)gridDim.x * STATS_THREADS) {
+        float4 v = s4[i];
+        float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!isfinite(e[j])) {
+                ++nf;
+                continue;
+            }
+            m += (double)e[j];
+            lo = fminf(lo, e[j]);
+            hi = fmaxf(hi, e[j]);
+        }
+    }
+    m = wave_sum(m);
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    nf = wave_sumu(nf);
+    __shared__ double sm[STATS_THREADS / 64];
+    __shared__ float slo[STATS_THREADS / 64], shi[STATS_THREADS / 64];
+    __shared__ unsigned long long snf[STATS_THREADS / 64];
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        sm[w] = m;
+        slo[w] = lo;
+        shi[w] = hi;
+        snf[w] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < STATS_THREADS / 64; ++j) {
+            m += sm[j];
+            lo = fminf(lo, slo[j]);
+            hi = fmaxf(hi, shi[j]);
+            nf += snf[j];
+        }
+        size_t o = (size_t)c * gridDim.x + blockIdx.x;
+        pm[o] = m;
+        pmin[o] = lo;
+        pmax[o] = hi;
+        pnf[o] = nf;
+    }
+}
+
+// One block per channel folds the partials in a fixed order.
+__global__ void __launch_bounds__(64) k_stats_final(int nblocks, double* pm, float* pmin, float* pmax,
+                                                   unsigned long long* pnf) {
+    const int c = blockIdx.x;
+    double m = 0.0;
+    float lo = INFINITY, hi = -INFINITY;
+    unsigned long long nf = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 64) {
+        size_t o = (size_t)c * nblocks + i;
+        m += pm[o];
+        lo = fminf(lo, pmin[o]);
+        hi = fmaxf(hi, pmax[o]);
+        nf += pnf[o];
+    }
+    m = wave_sum(m);
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    nf = wave_sumu(nf);
+    if (threadIdx.x == 0) {
+        // Results land in slot 0 of each channel's partial range.
+        size_t o = (size_t)c * nblocks;
+        pm[o] = m;
+        pmin[o] = lo;
+        pmax[o] = hi;
+        pnf[o] = nf;
+    }
+}
+
+}  // namespace
+
+void launch_cos_table64(hipStream_t s, Profiler* p, double* tab, int n) {
+    Launch L(p, s, "cos_table64");
+    hipLaunchKernelGGL(k_cos_table64, dim3((n + 255) / 256), dim3(256), 0, s, tab, n);
+    AH_HIP(hipGetLastError());
+}
+
+void launch_twiddles(hipStream_t s, Profiler* p, float2* tw, int n) {
+    Launch L(p, s, "twiddles64");
+    hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, tw, n);
+    AH_HIP(hipGetLastError());
+}
+
+void launch_kernel_taps64(hipStream_t s, Profiler* p, const ah_kernel& k, int reach, double* taps, double* sum) {
+    ShellArgs a;
+    a.radius = (double)k.radius;
+    a.rings = (int)k.rings;
+    for (int i = 0; i < AH_MAX_RINGS; ++i) a.beta[i] = (double)k.beta[i];
+    a.reach = reach;
+    Launch L(p, s, "kernel_taps64");
+    hipLaunchKernelGGL(k_kernel_taps64, dim3(1), dim3(256), 0, s, a, taps, sum);
+    AH_HIP(hipGetLastError());
+}
+
+void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const double* taps, const double* cosW,
+                   const double* cosH, double* T, float* khat, float2* kpack) {
+    const int side = 2 * reach + 1, nk = W / 2 + 1, np = H / 2 + 1;
+    {
+        Launch L(p, s, "khat_rows64");
+        int n = nk * side;
+        hipLaunchKernelGGL(k_khat_rows64, dim3((n + 255) / 256), dim3(256), 0, s, W, reach, taps, cosW, T);
+        AH_HIP(hipGetLastError());
+    }
+    {
+        Launch L(p, s, "khat64");
+        int n = nk * np;
+        hipLaunchKernelGGL(k_khat64, dim3((n + 255) / 256), dim3(256), 0, s, W, H, reach, T, cosH, khat, kpack);
+        AH_HIP(hipGetLastError());
+    }
+}
+
+void launch_seed(hipStream_t s, Profiler* p, const ah_lenia_desc& d, uint32_t seed, const uint32_t* corners,
+                 float* state) {
+    SeedArgs a;
+    a.W = (int)d.width;
+    a.H = (int)d.height;
+    a.C = (int)d.channels;
+    a.init = (int)d.init;
+    a.seed = seed;
+    a.size = d.init_size;
+    a.count = d.init_count;
+    a.corners = corners;
+    size_t n = (size_t)d.width * d.height * d.channels;
+    Launch L(p, s, "seed");
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, state);
+    AH_HIP(hipGetLastError());
+}
+
+int stats_blocks(size_t cells) {
+    size_t b = (cells / 4 + STATS_THREADS * 16 - 1) / (STATS_THREADS * 16);
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return (int)b;
+}
+
+void launch_stats(hipStream_t s, Profiler* p, const float* state, size_t cells, int channels, double* pm,
+                  float* pmin, float* pmax, unsigned long long* pnf, int nblocks) {
+    {
+        Launch L(p, s, "stats_partial");
+        hipLaunchKernelGGL(k_stats_partial, dim3(nblocks, channels), dim3(STATS_THREADS), 0, s, state, cells, pm, pmin,
+                           pmax, pnf);
+        AH_HIP(hipGetLastError());
+    }
+    {
+        Launch L(p, s, "stats_final");
+        hipLaunchKernelGGL(k_stats_final, dim3(channels), dim3(64), 0, s, nblocks, pm, pmin, pmax, pnf);
+        AH_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace ah

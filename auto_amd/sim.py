@@ -1,0 +1,119 @@
+"""Sim: the reference's engine interface over libauto_hip.so.
+
+Mirrors crates/auto-core/src/sim.rs: `Sim::new(gpu, program, seed)` (:202),
+`step(gpu, steps) -> generations` (:297-309, asynchronous), `read_field` /
+`write_field` (:559-595) with row-major little-endian f32 bytes, and the
+Program metadata `cells_per_step`, `generations_per_step`, `total_bytes()`
+(crates/auto-core/src/program.rs:101-129).  Errors raise EngineError with the
+engine's message, where the reference returns Err(String) or panics.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import LeniaDesc, Info, Stats, KernelTime, check, load
+from . import workloads
+
+
+def available() -> bool:
+    """True when the built library loads (it does not imply a GPU is present)."""
+    try:
+        load()
+        return True
+    except (ImportError, OSError):
+        return False
+
+
+def build(id: str, width: int, height: int) -> LeniaDesc:
+    """automata::build (crates/auto-core/src/automata/mod.rs:63-95) through the C registry."""
+    d = LeniaDesc()
+    check(load().ah_lenia_build(id.encode(), width, height, ctypes.byref(d)))
+    return d
+
+
+class Sim:
+    def __init__(self, desc: LeniaDesc, seed: int = 1):
+        lib = load()
+        self._lib = lib
+        self.desc = desc
+        h = ctypes.c_void_p()
+        check(lib.ah_sim_new(ctypes.byref(desc), seed & 0xFFFFFFFF, ctypes.byref(h)))
+        self._h = h
+        self.seed = seed
+        info = self.info()
+        self.width, self.height, self.channels = info.width, info.height, info.channels
+        self.cells_per_step = info.cells_per_step
+        self.generations_per_step = info.generations_per_step
+
+    # -- lifecycle --------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ah_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, steps: int = 1) -> int:
+        gens = ctypes.c_uint64()
+        check(self._lib.ah_sim_step(self._h, steps, ctypes.byref(gens)))
+        return gens.value
+
+    def wait(self) -> None:
+        check(self._lib.ah_sim_wait(self._h))
+
+    # -- fields -----------------------------------------------------------
+    def field_shape(self, name: str) -> tuple:
+        if name == "state":
+            return (self.channels, self.height, self.width)
+        return (self.height, self.width)
+
+    def read_field(self, name: str = "state") -> np.ndarray:
+        out = np.empty(self.field_shape(name), np.float32)
+        check(self._lib.ah_sim_read_field(self._h, name.encode(), out.ctypes.data, out.nbytes))
+        return out
+
+    def write_field(self, name: str, data: np.ndarray) -> None:
+        a = np.ascontiguousarray(data, np.float32)
+        check(self._lib.ah_sim_write_field(self._h, name.encode(), a.ctypes.data, a.nbytes))
+
+    # -- metadata ---------------------------------------------------------
+    def info(self) -> Info:
+        i = Info()
+        check(self._lib.ah_sim_info(self._h, ctypes.byref(i)))
+        return i
+
+    def total_bytes(self) -> int:
+        return self.info().total_bytes
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(self._lib.ah_sim_stats(self._h, ctypes.byref(s)))
+        return s
+
+    # -- kernel timing ----------------------------------------------------
+    def profile(self, enable: bool = True) -> None:
+        check(self._lib.ah_sim_profile(self._h, int(enable)))
+
+    def profile_read(self) -> dict:
+        cap = 32
+        arr = (KernelTime * cap)()
+        n = ctypes.c_uint32()
+        check(self._lib.ah_sim_profile_read(self._h, arr, cap, ctypes.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+
+__all__ = ["Sim", "build", "available", "workloads", "_abi"]

@@ -1,0 +1,165 @@
+/*
+ * auto_hip.h -- C ABI of the MI355X (gfx950) Lenia world-step engine.
+ *
+ * The engine replaces the Lenia path of auto-core: the registry entry that
+ * builds the Lenia program (crates/auto-core/src/automata/mod.rs:63-95,
+ * crates/auto-core/src/automata/spectral.rs:286-352,482-490) and the `Sim`
+ * object that runs it (crates/auto-core/src/sim.rs:202-309,559-595).  The
+ * reference describes a rule as WGSL text; a HIP engine cannot consume that,
+ * so the rule crosses this boundary as data (ah_lenia_desc).
+ *
+ * Conventions
+ *   - Every entry point returns ah_status; AH_OK is 0.  On failure the
+ *     message is available from ah_last_error() (thread-local), mirroring the
+ *     reference's Result<_, String> (sim.rs:202) and replacing its panics on
+ *     invalid descriptors (spectral.rs:103-116) with AH_ERR_INVALID.
+ *   - The engine owns all device memory; callers own host buffers.
+ *   - State is float32, row-major [channel][y][x], little-endian, exactly as
+ *     read_field/write_field move it in the reference (sim.rs:559-595).
+ *   - Stepping is asynchronous on the engine's own HIP stream; read_field,
+ *     stats and wait synchronise.  One handle must not be used from two
+ *     threads at once; distinct handles may be used from distinct threads.
+ *   - No exceptions cross this boundary.
+ */
+#ifndef AUTO_HIP_H
+#define AUTO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    AH_OK = 0,
+    AH_ERR_INVALID = 1,       /* bad descriptor or argument            */
+    AH_ERR_DEVICE = 2,        /* HIP runtime or launch failure         */
+    AH_ERR_ALLOC = 3,         /* device allocation failed              */
+    AH_ERR_UNKNOWN_FIELD = 4, /* read_field/write_field name unknown   */
+    AH_ERR_SIZE = 5,          /* host buffer size does not match       */
+    AH_ERR_UNSUPPORTED = 6,   /* valid request this build cannot serve */
+    AH_ERR_DIVERGED = 7       /* surrogate judge exceeded its bound    */
+} ah_status;
+
+enum { AH_MAX_CHANNELS = 3, AH_MAX_KERNELS = 9, AH_MAX_RINGS = 3 };
+
+/* How the periodic convolution is computed. */
+typedef enum {
+    AH_PATH_FFT_FP32 = 0,    /* fused three-pass FP32 real FFT                  */
+    AH_PATH_DIRECT_BF16 = 1  /* direct box convolution on BF16 MFMA, FP32 acc.  */
+} ah_path;
+
+/* How the state is seeded at creation. */
+typedef enum {
+    AH_INIT_ORBIUM = 0, /* reference Orbium lattice (spectral.rs:313-335)             */
+    AH_INIT_PATCH = 1,  /* centred init_size^2 patch, unit_f(rand3(x,y,seed+c))       */
+    AH_INIT_BLOBS = 2,  /* init_count blobs of init_size^2, corners from SplitMix64    */
+    AH_INIT_ZERO = 3
+} ah_init;
+
+/*
+ * One convolution kernel: a radial shell of `rings` exp-bump rings with ring
+ * weights beta (Chan's multi-ring Lenia; rings = 1, beta = {1} is exactly the
+ * reference's core, spectral.rs:288-303), normalised to sum 1 on the torus,
+ * with its own Gaussian growth (mu, sigma) and mixing weight into `dst`.
+ */
+typedef struct {
+    float radius;             /* R in cells, 2*ceil(R) < min(width, height)   */
+    uint32_t rings;           /* 1..AH_MAX_RINGS                              */
+    float beta[AH_MAX_RINGS]; /* ring weights                                 */
+    float mu, sigma;          /* growth centre and width                      */
+    float weight;             /* mixing weight h; normalised per dst at create */
+    uint32_t src, dst;        /* source and destination channel               */
+} ah_kernel;
+
+typedef struct {
+    uint32_t width, height;   /* powers of two, 16..8192                      */
+    uint32_t channels;        /* 1..AH_MAX_CHANNELS                           */
+    uint32_t n_kernels;       /* 1..AH_MAX_KERNELS                            */
+    ah_kernel kernels[AH_MAX_KERNELS];
+    float dt;                 /* time step                                    */
+    uint32_t path;            /* ah_path                                      */
+    uint32_t init;            /* ah_init                                      */
+    uint32_t init_size;       /* patch side (PATCH) or blob side (BLOBS)      */
+    uint32_t init_count;      /* blob count (BLOBS)                           */
+} ah_lenia_desc;
+
+typedef struct {
+    uint32_t width, height, channels, n_kernels, path;
+    uint64_t cells_per_step;       /* program.rs:117                      */
+    uint64_t generations_per_step; /* program.rs:119                      */
+    uint64_t total_bytes;          /* device bytes held (program.rs:123)  */
+    uint64_t generation;           /* generations advanced so far         */
+} ah_info;
+
+/* Mass and range of the current state: FP64 sums, per channel. */
+typedef struct {
+    double mass[AH_MAX_CHANNELS];
+    float min[AH_MAX_CHANNELS];
+    float max[AH_MAX_CHANNELS];
+    uint64_t nonfinite;            /* NaN/Inf cells over all channels     */
+    uint64_t generation;
+} ah_stats;
+
+typedef struct ah_sim ah_sim;
+
+/*
+ * Fill `out` with a registry rule, as automata::build(id, w, h) does
+ * (mod.rs:63-95): sizes are rounded up to powers of two (mod.rs:51-61).
+ * Known ids: "lenia" (the reference rule: R=13, one ring, mu=0.15,
+ * sigma=0.015, dt=0.1, Orbium seed; spectral.rs:286-352,482-490).
+ */
+ah_status ah_lenia_build(const char* id, uint32_t width, uint32_t height, ah_lenia_desc* out);
+
+/* Sim::new (sim.rs:202-282): allocate, build kernels, seed the state. */
+ah_status ah_sim_new(const ah_lenia_desc* desc, uint32_t seed, ah_sim** out);
+
+/* Sim::step (sim.rs:297-309): enqueue `steps` steps; asynchronous. */
+ah_status ah_sim_step(ah_sim* sim, uint32_t steps, uint64_t* generations);
+
+/* Gpu::wait (gpu.rs:133-137): block until queued work is done. */
+ah_status ah_sim_wait(ah_sim* sim);
+
+/*
+ * Sim::read_field / write_field (sim.rs:559-595).  Fields:
+ *   "state"  float32 [channels][height][width], read/write;
+ *   "conv<k>" float32 [height][width], read-only: kernel k convolved with its
+ *            source channel of the current state, computed by the sim's path
+ *            (the reference's conv_i field, spectral.rs:227-234).
+ * `bytes` must equal the field size exactly.
+ */
+ah_status ah_sim_read_field(ah_sim* sim, const char* name, void* dst, size_t bytes);
+ah_status ah_sim_write_field(ah_sim* sim, const char* name, const void* src, size_t bytes);
+
+/* Mass/min/max/non-finite reduction of the current state (synchronises). */
+ah_status ah_sim_stats(ah_sim* sim, ah_stats* out);
+
+ah_status ah_sim_info(const ah_sim* sim, ah_info* out);
+
+void ah_sim_destroy(ah_sim* sim);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ah_last_error(void);
+
+/*
+ * Kernel timing.  While enabled, every kernel the sim launches is bracketed
+ * by HIP events on the sim's stream; ah_sim_profile_read returns per-kernel
+ * launch counts and summed device milliseconds (it synchronises).  Enabling
+ * resets the counters.  This replaces the reference's unused TIMESTAMP_QUERY
+ * (gpu.rs:83) and complements the wall-clock method of bench.rs:126-158.
+ */
+typedef struct {
+    char name[40];
+    uint64_t launches;
+    double total_ms;
+} ah_kernel_time;
+
+ah_status ah_sim_profile(ah_sim* sim, int enable);
+ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacity, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AUTO_HIP_H */

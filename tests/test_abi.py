@@ -1,0 +1,99 @@
+"""CPU checks of the C ABI: the library loads, exports every declared
+symbol, and the host-side logic (registry, validation) behaves without a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from auto_amd import _abi, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "auto_hip.h")).read()
+    return sorted(set(re.findall(r"\b(ah_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    names = declared_symbols()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_abi.symbols())
+
+
+def test_struct_sizes_match_header(tmp_path):
+    """ctypes layouts equal the C compiler's sizeof for every ABI struct."""
+    import shutil
+    import subprocess
+    want = [40, 396, 64, 56, 56]
+    gcc = shutil.which("gcc")
+    if gcc:
+        src = tmp_path / "sz.c"
+        src.write_text('#include "include/auto_hip.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu",'
+                       'sizeof(ah_kernel),sizeof(ah_lenia_desc),sizeof(ah_stats),sizeof(ah_info),'
+                       'sizeof(ah_kernel_time));return 0;}')
+        exe = tmp_path / "sz"
+        subprocess.run([gcc, f"-I{ROOT}", str(src), "-o", str(exe)], check=True)
+        want = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    got = [ctypes.sizeof(t) for t in (_abi.Kernel, _abi.LeniaDesc, _abi.Stats, _abi.Info, _abi.KernelTime)]
+    assert got == want
+
+
+def test_registry_matches_reference_entry():
+    """ah_lenia_build("lenia") == the reference entry (spectral.rs:286-352,482-490),
+    sizes rounded up to powers of two (mod.rs:51-61)."""
+    lib = _abi.load()
+    d = _abi.LeniaDesc()
+    assert lib.ah_lenia_build(b"lenia", 100, 60, ctypes.byref(d)) == 0
+    ref = W.lenia_reference(100, 60)
+    assert (d.width, d.height) == (128, 64) == (ref.width, ref.height)
+    assert bytes(d) == bytes(ref)
+    assert lib.ah_lenia_build(b"life", 64, 64, ctypes.byref(d)) == _abi.ERR_INVALID
+    assert b"unknown" in lib.ah_last_error()
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (lambda d: setattr(d, "width", 100), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[0], "radius", 40.0), _abi.ERR_INVALID),  # world smaller than its kernel
+    (lambda d: setattr(d, "n_kernels", 0), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[0], "sigma", 0.0), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[0], "src", 2), _abi.ERR_INVALID),
+    (lambda d: setattr(d, "path", 7), _abi.ERR_INVALID),
+    (lambda d: setattr(d, "dt", float("nan")), _abi.ERR_INVALID),
+])
+def test_invalid_descriptors_are_errors_not_crashes(mutate, code):
+    """The reference panics on these (spectral.rs:103-116); the ABI returns an error."""
+    lib = _abi.load()
+    d = W.lenia_reference(64, 64)
+    mutate(d)
+    h = ctypes.c_void_p()
+    assert lib.ah_sim_new(ctypes.byref(d), 1, ctypes.byref(h)) == code
+    assert not h.value
+    assert lib.ah_last_error()
+
+
+def test_null_arguments():
+    lib = _abi.load()
+    assert lib.ah_sim_step(None, 1, None) == _abi.ERR_INVALID
+    assert lib.ah_sim_wait(None) == _abi.ERR_INVALID
+    lib.ah_sim_destroy(None)
+
+
+def test_workload_shapes():
+    d = W.rings3_4096()
+    assert (d.width, d.height, d.n_kernels, d.kernels[0].rings) == (4096, 4096, 1, 3)
+    assert list(d.kernels[0].beta) == pytest.approx([1, 2 / 3, 1 / 3])
+    m = W.multichannel_1024(seed=1)
+    assert m.channels == 3 and m.n_kernels == 9
+    for k in range(9):
+        kd = m.kernels[k]
+        assert (kd.src, kd.dst) == (k // 3, k % 3)
+        assert 8 <= kd.radius <= 20 and 1 <= kd.rings <= 3
+        assert 0.1 <= kd.mu <= 0.35 and 0.01 <= kd.sigma <= 0.05 and 0.2 <= kd.weight <= 1
+    ws = W.uneven_worlds(seed=1)
+    assert len(ws) == 384
+    assert {d.width for d, _ in ws} <= {128, 256, 512, 1024}
