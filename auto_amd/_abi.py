@@ -92,6 +92,28 @@ class KernelTime(ctypes.Structure):
     ]
 
 
+BATCH_MAX_CANDIDATES = 8
+
+
+class BatchTune(ctypes.Structure):
+    _fields_ = [("tile_rows", ctypes.c_uint32), ("grid", ctypes.c_uint32), ("ms_per_step", ctypes.c_double)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_worlds", ctypes.c_uint32),
+        ("tile_rows", ctypes.c_uint32),
+        ("grid", ctypes.c_uint32),
+        ("ntiles", ctypes.c_uint32),
+        ("graph_steps", ctypes.c_uint32),
+        ("total_cells", ctypes.c_uint64),
+        ("total_bytes", ctypes.c_uint64),
+        ("generation", ctypes.c_uint64),
+        ("n_tune", ctypes.c_uint32),
+        ("tune", BatchTune * BATCH_MAX_CANDIDATES),
+    ]
+
+
 LIB_NAME = "libauto_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
@@ -111,6 +133,17 @@ _SIGNATURES = {
     "ah_sim_profile": (ctypes.c_int, [_P, ctypes.c_int]),
     "ah_sim_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(KernelTime), ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32)]),
+    "ah_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "ah_batch_new": (ctypes.c_int, [ctypes.POINTER(LeniaDesc), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(_P)]),
+    "ah_batch_step": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "ah_batch_wait": (ctypes.c_int, [_P]),
+    "ah_batch_read_world": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_size_t]),
+    "ah_batch_write_world": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_size_t]),
+    "ah_batch_info_get": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
+    "ah_batch_measure": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "ah_batch_destroy": (None, [_P]),
 }
 
 _lib = None

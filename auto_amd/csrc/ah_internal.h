@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -18,6 +20,33 @@ struct Error {
     ah_status code;
     std::string msg;
 };
+
+// Runs f and turns exceptions into an ah_status plus ah_last_error() text.
+template <typename F>
+ah_status guard(F&& f) {
+    try {
+        f();
+        return AH_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return AH_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return AH_ERR_DEVICE;
+    } catch (...) {
+        set_error("unknown failure");
+        return AH_ERR_DEVICE;
+    }
+}
+
+uint64_t splitmix64(uint64_t& state);
+uint16_t f32_to_bf16(float f);
+double bf16_to_f64(uint16_t h);
+double build_fragments(const std::vector<double>& taps, int R, std::vector<int>& sched, std::vector<uint16_t>& frags,
+                       int& nfrag);
 
 #define AH_HIP(call)                                                                  \
     do {                                                                              \
@@ -91,5 +120,57 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
                     float* conv_out, int conv_k);
 size_t fft_lds_limit();
+
+// direct_bf16.hip
+struct DirectKernel {  // one kernel as the direct path reads it
+    int reach;         // R: rows dy in [-R, R]
+    int frag0, nfrag;  // fragments of this kernel in the fragment table
+    float scale;       // 1 / sum of the bf16-rounded taps
+    int src, dst;
+    float mu, inv_sigma, weight;
+};
+
+struct DirectArgs {
+    int W, H;
+    int halo_y, halo_x;  // max R over kernels; max R rounded up to a multiple of 8
+    int tr, tc, tstride; // LDS tile rows, columns, row stride (bf16 elements)
+    int n_kernels;
+    DirectKernel k[AH_MAX_KERNELS];
+    const int4* sched;   // per kernel, per |dy| in [0, R]: (e, chunks, first fragment, 0)
+    int sched_base[AH_MAX_KERNELS];
+    const uint4* frags;  // [fragment][64 lanes] x 8 bf16
+    int nfrag_max;
+    const float* state_in;
+    float* state_out;
+    float dt;
+    float* conv_out;     // set: write kernel conv_k's potential instead of updating
+    int conv_k;
+};
+struct BatchWorld {  // one world of a batch as the batch kernel reads it
+    int W, H;
+    size_t off;      // first cell in both state buffers
+    int reach, frag0, nfrag, sched0;
+    float scale, mu, inv_sigma, dt;
+};
+
+struct BatchArgs {
+    const BatchWorld* worlds;
+    const int4* tiles;  // (world, x0, y0, 0); tile = 64 columns x 32*NB rows
+    const int* seg;     // workgroup b runs tiles [seg[b], seg[b+1])
+    int ntiles;
+    float* buf0;
+    float* buf1;
+    int parity;         // 0: buf0 -> buf1, 1: buf1 -> buf0
+    const int4* sched;
+    const uint4* frags;
+    int halo_y, halo_x, tr, tc, tstride, nfrag_max;
+};
+size_t batch_lds_bytes(const BatchArgs& a);
+void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int nb, int grid);
+int direct_rows_per_tile(int H);
+size_t direct_lds_bytes(const DirectArgs& a);
+void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels);
+
+void launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n4);
 
 }  // namespace ah

@@ -1,0 +1,450 @@
+// Batched uneven worlds: hundreds of single-channel Lenia worlds of different
+// sizes and rules advanced together, one persistent launch per step.
+//
+// The reference runs one world per Sim (crates/auto-core/src/sim.rs:202-309);
+// it has no batching.  Here every world's state lives in one pair of
+// ping-pong buffers; a tile list (64 columns x 32*NB rows per tile, grouped by
+// world) is cut into contiguous segments of near-equal MFMA cost, one per
+// workgroup of the persistent direct-BF16 kernel (direct_bf16.hip
+// k_direct_batch).  Tile rows NB and the grid size are chosen by an autotuner
+// that times every admissible candidate on the batch itself at creation.
+// Steps run as hipGraph replays of `graph_steps` launches (an even count, so
+// the ping-pong parity is unchanged by a replay), with single eager launches
+// for the remainder.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ah_internal.h"
+
+using namespace ah;
+
+struct ah_batch {
+    std::vector<ah_lenia_desc> d;
+    std::vector<uint32_t> seeds;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Profiler prof;
+    std::vector<void*> bufs;
+    size_t total_bytes = 0;
+    float* buf[2] = {nullptr, nullptr};
+    std::vector<BatchWorld> worlds_h;
+    BatchWorld* worlds = nullptr;
+    std::vector<int> cost;  // MFMA chunks per 16x16 output block, per world
+    int4* sched = nullptr;
+    uint4* frags = nullptr;
+    int halo_y = 0, halo_x = 0, nfrag_max = 0;
+    uint64_t total_cells = 0;
+
+    // chosen tiling
+    int nb = 0, grid = 0, ntiles = 0;
+    int4* tiles = nullptr;
+    int* seg = nullptr;
+    ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES]{};
+    uint32_t n_tune = 0;
+
+    int parity = 0;
+    uint64_t generation = 0;
+    uint32_t graph_steps = 100;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            throw Error{AH_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e)};
+        bufs.push_back(p);
+        total_bytes += bytes;
+        return p;
+    }
+    void release(void* p) {
+        auto it = std::find(bufs.begin(), bufs.end(), p);
+        if (it != bufs.end()) {
+            (void)hipFree(p);
+            bufs.erase(it);
+        }
+    }
+
+    ~ah_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (void* p : bufs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    BatchArgs args(int par) const {
+        BatchArgs a{};
+        a.worlds = worlds;
+        a.tiles = tiles;
+        a.seg = seg;
+        a.ntiles = ntiles;
+        a.buf0 = buf[0];
+        a.buf1 = buf[1];
+        a.parity = par;
+        a.sched = sched;
+        a.frags = frags;
+        a.halo_y = halo_y;
+        a.halo_x = halo_x;
+        a.tr = 32 * nb + 2 * halo_y;
+        a.tc = 64 + 2 * halo_x + 32;
+        a.tstride = a.tc + 8;
+        a.nfrag_max = nfrag_max;
+        return a;
+    }
+};
+
+namespace {
+
+bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+void validate_world(const ah_lenia_desc& d, uint32_t i) {
+    auto bad = [&](const std::string& m) { throw Error{AH_ERR_INVALID, "world " + std::to_string(i) + ": " + m}; };
+    if (d.path != AH_PATH_DIRECT_BF16) bad("batched worlds run the direct BF16 path (path must be AH_PATH_DIRECT_BF16)");
+    if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
+    if (d.width < 64 || d.height < 32 || d.width > 8192 || d.height > 8192) bad("width 64..8192, height 32..8192");
+    if (d.channels != 1 || d.n_kernels != 1) bad("batched worlds are single-channel with one kernel");
+    const ah_kernel& k = d.kernels[0];
+    if (!(k.radius > 0.0f) || !std::isfinite(k.radius)) bad("kernel radius must be positive");
+    const int reach = (int)std::ceil((double)k.radius);
+    if (2 * reach >= (int)std::min(d.width, d.height)) bad("world smaller than its kernel");
+    if (reach > 31) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": radius above 31"};
+    if (k.rings < 1 || k.rings > AH_MAX_RINGS) bad("rings must be 1..3");
+    if (!(k.sigma > 0.0f) || !std::isfinite(k.mu)) bad("sigma must be positive, mu finite");
+    if (!std::isfinite(d.dt) || d.dt <= 0.0f) bad("dt must be a positive finite number");
+    if (d.init > AH_INIT_ZERO || d.init == AH_INIT_BLOBS) bad("batched worlds take ORBIUM, PATCH or ZERO inits");
+    if (d.init == AH_INIT_PATCH && (d.init_size == 0 || d.init_size > std::min(d.width, d.height)))
+        bad("patch size must be 1..side");
+}
+
+// Tiles of every world for row-block count nb, and contiguous segments of
+// near-equal cost for `grid` workgroups.
+void build_tiling(ah_batch* b, int nb, int grid, std::vector<int4>& tiles, std::vector<int>& seg) {
+    tiles.clear();
+    std::vector<double> tcost;
+    for (size_t w = 0; w < b->d.size(); ++w) {
+        const int W = (int)b->d[w].width, H = (int)b->d[w].height;
+        for (int y0 = 0; y0 < H; y0 += 32 * nb)
+            for (int x0 = 0; x0 < W; x0 += 64) {
+                tiles.push_back(make_int4((int)w, x0, y0, 0));
+                tcost.push_back((double)b->cost[w]);
+            }
+    }
+    double total = 0.0;
+    for (double c : tcost) total += c;
+    seg.assign(grid + 1, (int)tiles.size());
+    seg[0] = 0;
+    double acc = 0.0;
+    int g = 1;
+    for (size_t t = 0; t < tiles.size() && g < grid; ++t) {
+        acc += tcost[t];
+        while (g < grid && acc >= total * g / grid) seg[g++] = (int)t + 1;
+    }
+}
+
+void install_tiling(ah_batch* b, int nb, int grid) {
+    std::vector<int4> tiles;
+    std::vector<int> seg;
+    build_tiling(b, nb, grid, tiles, seg);
+    if (b->tiles) b->release(b->tiles);
+    if (b->seg) b->release(b->seg);
+    b->tiles = (int4*)b->alloc(sizeof(int4) * tiles.size());
+    b->seg = (int*)b->alloc(sizeof(int) * seg.size());
+    AH_HIP(hipMemcpy(b->tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    AH_HIP(hipMemcpy(b->seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
+    b->nb = nb;
+    b->grid = grid;
+    b->ntiles = (int)tiles.size();
+    for (auto& g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+void launch_step(ah_batch* b, Profiler* p) {
+    launch_direct_batch(b->stream, p, b->args(b->parity), b->nb, b->grid);
+    b->parity ^= 1;
+}
+
+// Times each admissible (rows per tile, grid) on the batch and keeps the fastest.
+void autotune(ah_batch* b) {
+    int min_h = 1 << 30;
+    for (auto& d : b->d) min_h = std::min(min_h, (int)d.height);
+    hipDeviceProp_t prop;
+    AH_HIP(hipGetDeviceProperties(&prop, b->device));
+    const int cus = prop.multiProcessorCount;
+    hipEvent_t e0, e1;
+    AH_HIP(hipEventCreate(&e0));
+    AH_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_nb = 0, best_grid = 0;
+    b->n_tune = 0;
+    try {
+        for (int nb : {1, 2, 4}) {
+            if (32 * nb > min_h) continue;
+            b->nb = nb;
+            const size_t lds = batch_lds_bytes(b->args(0));
+            if (lds > 160 * 1024) continue;
+            const int per_cu = std::max(1, (int)((160 * 1024) / lds));
+            for (int k = 1; k <= std::min(per_cu, 2); ++k) {
+                const int grid = cus * k;
+                install_tiling(b, nb, grid);
+                launch_step(b, nullptr);  // warm-up
+                const int reps = 4;
+                AH_HIP(hipEventRecord(e0, b->stream));
+                for (int r = 0; r < reps; ++r) launch_step(b, nullptr);
+                AH_HIP(hipEventRecord(e1, b->stream));
+                AH_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                AH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double per = ms / reps;
+                if (b->n_tune < AH_BATCH_MAX_CANDIDATES)
+                    b->tune[b->n_tune++] = ah_batch_tune{(uint32_t)(32 * nb), (uint32_t)grid, per};
+                if (per < best) {
+                    best = per;
+                    best_nb = nb;
+                    best_grid = grid;
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!best_nb) throw Error{AH_ERR_UNSUPPORTED, "no tile shape fits the batch in LDS"};
+    install_tiling(b, best_nb, best_grid);
+    b->parity = 0;
+}
+
+void setup(ah_batch* b) {
+    const size_t n = b->d.size();
+    AH_HIP(hipGetDevice(&b->device));
+    AH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    // World layout and kernels.
+    std::vector<int> sched;
+    std::vector<uint16_t> frags;
+    b->worlds_h.resize(n);
+    b->cost.resize(n);
+    size_t off = 0;
+    int rmax = 0, nfrag_total = 0;
+    std::vector<void*> scratch;
+    double* taps = nullptr;
+    double* sum = nullptr;
+    AH_HIP(hipMalloc((void**)&taps, sizeof(double) * 63 * 63));
+    scratch.push_back(taps);
+    AH_HIP(hipMalloc((void**)&sum, sizeof(double)));
+    scratch.push_back(sum);
+    try {
+        for (size_t w = 0; w < n; ++w) {
+            const ah_lenia_desc& d = b->d[w];
+            const int R = (int)std::ceil((double)d.kernels[0].radius), side = 2 * R + 1;
+            rmax = std::max(rmax, R);
+            launch_kernel_taps64(b->stream, nullptr, d.kernels[0], R, taps, sum);
+            std::vector<double> t((size_t)side * side);
+            AH_HIP(hipMemcpyAsync(t.data(), taps, sizeof(double) * t.size(), hipMemcpyDeviceToHost, b->stream));
+            AH_HIP(hipStreamSynchronize(b->stream));
+            BatchWorld& bw = b->worlds_h[w];
+            bw.W = (int)d.width;
+            bw.H = (int)d.height;
+            bw.off = off;
+            bw.reach = R;
+            bw.frag0 = nfrag_total;
+            bw.sched0 = (int)(sched.size() / 4);
+            bw.scale = (float)(1.0 / build_fragments(t, R, sched, frags, bw.nfrag));
+            bw.mu = d.kernels[0].mu;
+            bw.inv_sigma = 1.0f / d.kernels[0].sigma;
+            bw.dt = d.dt;
+            int chunks = 0;
+            for (int dy = -R; dy <= R; ++dy) chunks += sched[(bw.sched0 + std::abs(dy)) * 4 + 1];
+            b->cost[w] = chunks;
+            nfrag_total += bw.nfrag;
+            b->nfrag_max = std::max(b->nfrag_max, bw.nfrag);
+            off += (size_t)d.width * d.height;
+        }
+    } catch (...) {
+        for (void* p : scratch) (void)hipFree(p);
+        throw;
+    }
+    for (void* p : scratch) (void)hipFree(p);
+    b->total_cells = off;
+    b->halo_y = rmax;
+    b->halo_x = 8 * ((rmax + 7) / 8);
+    b->buf[0] = (float*)b->alloc(sizeof(float) * off);
+    b->buf[1] = (float*)b->alloc(sizeof(float) * off);
+    AH_HIP(hipMemsetAsync(b->buf[0], 0, sizeof(float) * off, b->stream));
+    AH_HIP(hipMemsetAsync(b->buf[1], 0, sizeof(float) * off, b->stream));
+    b->worlds = (BatchWorld*)b->alloc(sizeof(BatchWorld) * n);
+    b->sched = (int4*)b->alloc(sizeof(int) * sched.size());
+    b->frags = (uint4*)b->alloc(sizeof(uint16_t) * frags.size());
+    AH_HIP(hipMemcpy(b->worlds, b->worlds_h.data(), sizeof(BatchWorld) * n, hipMemcpyHostToDevice));
+    AH_HIP(hipMemcpy(b->sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
+    AH_HIP(hipMemcpy(b->frags, frags.data(), sizeof(uint16_t) * frags.size(), hipMemcpyHostToDevice));
+    AH_HIP(hipStreamSynchronize(b->stream));
+
+    // The tuner steps zero states (the work is data-independent), then the
+    // worlds are seeded.
+    autotune(b);
+    for (size_t w = 0; w < n; ++w)
+        launch_seed(b->stream, nullptr, b->d[w], b->seeds[w], nullptr, b->buf[0] + b->worlds_h[w].off);
+    AH_HIP(hipStreamSynchronize(b->stream));
+    b->parity = 0;
+    b->generation = 0;
+}
+
+hipGraphExec_t graph_for(ah_batch* b) {
+    hipGraphExec_t& g = b->graph[b->parity];
+    if (g) return g;
+    hipGraph_t graph = nullptr;
+    AH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int start = b->parity;
+    try {
+        for (uint32_t i = 0; i < b->graph_steps; ++i) launch_step(b, nullptr);
+    } catch (...) {
+        (void)hipStreamEndCapture(b->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        b->parity = start;
+        throw;
+    }
+    b->parity = start;  // an even number of steps returns to the same parity
+    AH_HIP(hipStreamEndCapture(b->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g = nullptr;
+        throw Error{AH_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+    }
+    return g;
+}
+
+void run_steps(ah_batch* b, uint32_t steps) {
+    while (steps) {
+        if (steps >= b->graph_steps) {
+            AH_HIP(hipGraphLaunch(graph_for(b), b->stream));
+            steps -= b->graph_steps;
+            b->generation += b->graph_steps;
+        } else {
+            launch_step(b, &b->prof);
+            --steps;
+            ++b->generation;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+ah_status ah_batch_new(const ah_lenia_desc* descs, uint32_t n_worlds, const uint32_t* seeds, ah_batch** out) {
+    return guard([&] {
+        if (!descs || !seeds || !out || n_worlds == 0) throw Error{AH_ERR_INVALID, "null argument or no worlds"};
+        *out = nullptr;
+        for (uint32_t i = 0; i < n_worlds; ++i) validate_world(descs[i], i);
+        std::unique_ptr<ah_batch> b(new ah_batch());
+        b->d.assign(descs, descs + n_worlds);
+        b->seeds.assign(seeds, seeds + n_worlds);
+        setup(b.get());
+        *out = b.release();
+    });
+}
+
+ah_status ah_batch_step(ah_batch* b, uint32_t steps) {
+    return guard([&] {
+        if (!b) throw Error{AH_ERR_INVALID, "null batch"};
+        AH_HIP(hipSetDevice(b->device));
+        run_steps(b, steps);
+    });
+}
+
+ah_status ah_batch_wait(ah_batch* b) {
+    return guard([&] {
+        if (!b) throw Error{AH_ERR_INVALID, "null batch"};
+        AH_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+ah_status ah_batch_read_world(ah_batch* b, uint32_t world, float* dst, size_t bytes) {
+    return guard([&] {
+        if (!b || !dst) throw Error{AH_ERR_INVALID, "null argument"};
+        if (world >= b->d.size()) throw Error{AH_ERR_INVALID, "world index out of range"};
+        const BatchWorld& w = b->worlds_h[world];
+        if (bytes != sizeof(float) * (size_t)w.W * w.H) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
+        AH_HIP(hipMemcpyAsync(dst, b->buf[b->parity] + w.off, bytes, hipMemcpyDeviceToHost, b->stream));
+        AH_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+ah_status ah_batch_write_world(ah_batch* b, uint32_t world, const float* src, size_t bytes) {
+    return guard([&] {
+        if (!b || !src) throw Error{AH_ERR_INVALID, "null argument"};
+        if (world >= b->d.size()) throw Error{AH_ERR_INVALID, "world index out of range"};
+        const BatchWorld& w = b->worlds_h[world];
+        if (bytes != sizeof(float) * (size_t)w.W * w.H) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
+        AH_HIP(hipMemcpyAsync(b->buf[b->parity] + w.off, src, bytes, hipMemcpyHostToDevice, b->stream));
+        AH_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+ah_status ah_batch_info_get(const ah_batch* b, ah_batch_info* out) {
+    return guard([&] {
+        if (!b || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        ah_batch_info i{};
+        i.n_worlds = (uint32_t)b->d.size();
+        i.tile_rows = (uint32_t)(32 * b->nb);
+        i.grid = (uint32_t)b->grid;
+        i.ntiles = (uint32_t)b->ntiles;
+        i.graph_steps = b->graph_steps;
+        i.total_cells = b->total_cells;
+        i.total_bytes = b->total_bytes;
+        i.generation = b->generation;
+        i.n_tune = b->n_tune;
+        for (uint32_t k = 0; k < b->n_tune; ++k) i.tune[k] = b->tune[k];
+        *out = i;
+    });
+}
+
+ah_status ah_batch_measure(ah_batch* b, uint32_t steps, double* kernel_ms, double* graph_wall_ms,
+                           double* eager_wall_ms) {
+    return guard([&] {
+        if (!b || !kernel_ms || !graph_wall_ms || !eager_wall_ms) throw Error{AH_ERR_INVALID, "null argument"};
+        if (steps == 0 || steps % b->graph_steps) throw Error{AH_ERR_INVALID, "steps must be a multiple of graph_steps"};
+        AH_HIP(hipSetDevice(b->device));
+        // Eager launches with per-launch events: summed device time and wall time.
+        AH_HIP(hipStreamSynchronize(b->stream));
+        b->prof.reset();
+        b->prof.enabled = true;
+        auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < steps; ++i) launch_step(b, &b->prof);
+        AH_HIP(hipStreamSynchronize(b->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        double ksum = 0.0;
+        for (auto& r : b->prof.recs) {
+            float ms = 0.0f;
+            AH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            ksum += ms;
+        }
+        b->prof.reset();
+        b->prof.enabled = false;
+        b->generation += steps;
+        // The same number of steps through graph replays.
+        (void)graph_for(b);
+        AH_HIP(hipStreamSynchronize(b->stream));
+        auto t2 = std::chrono::steady_clock::now();
+        run_steps(b, steps);
+        AH_HIP(hipStreamSynchronize(b->stream));
+        auto t3 = std::chrono::steady_clock::now();
+        *kernel_ms = ksum;
+        *eager_wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        *graph_wall_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    });
+}
+
+void ah_batch_destroy(ah_batch* b) { delete b; }
+
+}  // extern "C"

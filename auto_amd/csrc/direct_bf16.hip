@@ -1,0 +1,273 @@
+// Direct periodic convolution on BF16 MFMA (FP32 accumulate), fused with the
+// growth map, dt and clip.  This replaces the reference's whole FFT pass chain
+// (crates/auto-core/src/automata/spectral.rs:199-264) for kernels of radius
+// <= 31 with one launch per step.
+//
+// Formulation.  For one kernel row dy the x-convolution of a strip of 16
+// output columns is a banded (Toeplitz) matrix product:
+//     u[y][x0+i] += sum_j A_dy[i][j] * S[y+dy][xw+j],  A_dy[i][j] = K[dy][xw+j-x0-i]
+// v_mfma_f32_16x16x32_bf16 computes D(16x16) += A(16x32) B(32x16) with
+//   A rows = the 16 output columns i, A cols = 32 consecutive input columns j,
+//   B rows = the same 32 input columns, B cols = 16 output rows y,
+// so every lane's B operand is 8 consecutive bf16 of one state row (one
+// aligned 16-byte LDS read) and its A operand is a precomputed fragment of the
+// kernel row.  Each kernel row only spans its chord of the disc, |dx| <= dxm(dy),
+// so the number of 32-column chunks varies per row (3 near the centre of an
+// R=26 kernel, 2 further out, 1 at the rim).  A fragment is reused for every
+// 16-row block a wave owns; K[dy] = K[-dy] halves the fragment table.
+//
+// Lane layouts assumed for the 16x16x32 bf16 MFMA (wave64, lane l, q = l/16):
+//   A: row l%16, k = 8q..8q+7     B: col l%16, k = 8q..8q+7
+//   D: col l%16, rows 4q..4q+3
+//
+// The state tile is staged in LDS as bf16; kernel taps are rounded to bf16
+// and the accumulated sum is rescaled by 1/sum(bf16 taps) (an FP64 sum done
+// on the host), so the rounded kernel still has unit mass.
+#include <algorithm>
+
+#include "ah_internal.h"
+
+namespace ah {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int DX = 64;          // output tile width: 4 strips of 16 columns
+constexpr int DTHREADS = 512;   // 8 waves: strip = wave & 3, row half = wave >> 2
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                      // round to nearest even
+    return u >> 16;
+}
+
+// Stage rows [y0-hy, y0+DY+hy) x cols [x0-hx, x0-hx+tc) of one channel as
+// bf16, wrapped on the torus.  Columns outside [x0-hy, x0+DX+hy) are zero:
+// no nonzero tap reaches them, and zeros keep non-finite data out of range.
+__device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int W, int H, int x0, int y0,
+                                           int halo_y, int halo_x, int tr, int tc, int tstride) {
+    const int groups = tc >> 2;
+    for (int i = threadIdx.x; i < tr * groups; i += DTHREADS) {
+        int r = i / groups, j = (i - r * groups) * 4;
+        int gy = (y0 - halo_y + r) % H;
+        if (gy < 0) gy += H;
+        int cx = j - halo_x;  // column relative to x0
+        int gx = (x0 + cx) % W;
+        if (gx < 0) gx += W;
+        float4 v = *reinterpret_cast<const float4*>(src + (size_t)gy * W + gx);
+        float e[4] = {v.x, v.y, v.z, v.w};
+        uint32_t h[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int c = cx + t;
+            h[t] = (c >= -halo_y && c < DX + halo_y) ? bf16_bits(e[t]) : 0u;
+        }
+        *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    }
+}
+
+// All MFMAs of one kernel over this wave's 16 columns x 16*NB rows.
+template <int NB>
+__device__ __forceinline__ void conv_tile(f32x4 (&acc)[NB], const uint16_t* tile, const uint4* frag_lds,
+                                          const int4* sched, int reach, int halo_y, int halo_x, int tstride) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = wave & 3, half = wave >> 2;
+    const int q = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int row0 = half * 16 * NB + n + halo_y;
+    const int colS = 16 * strip + halo_x + 8 * q;
+    for (int dy = -reach; dy <= reach; ++dy) {
+        const int4 sc = sched[dy < 0 ? -dy : dy];  // (e, chunks, first fragment)
+        const uint16_t* brow = tile + (size_t)(row0 + dy) * tstride + colS + sc.x;
+        for (int c = 0; c < sc.y; ++c) {
+            const bf16x8 av = __builtin_bit_cast(bf16x8, frag_lds[(sc.z + c) * 64 + lane]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const uint4 bv = *reinterpret_cast<const uint4*>(brow + (size_t)16 * b * tstride + 32 * c);
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bv), acc[b], 0, 0, 0);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float growth(float u, float mu, float inv_sigma) {
+    float d = (u - mu) * inv_sigma;
+    return 2.0f * __expf(-0.5f * d * d) - 1.0f;
+}
+
+template <int NB>
+__global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
+    constexpr int DY = 32 * NB;
+    const int W = a.W, H = a.H;
+    const int x0 = blockIdx.x * DX, y0 = blockIdx.y * DY;
+    const int dst = blockIdx.z;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int strip = wave & 3, half = wave >> 2;
+    const int q = lane >> 4, n = lane & 15;
+    uint16_t* tile = lds_u16;
+    uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)a.tr * a.tstride);
+
+    float g[NB][4];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) g[b][r] = 0.0f;
+
+    int staged = -1;
+    for (int k = 0; k < a.n_kernels; ++k) {
+        const DirectKernel kk = a.k[k];
+        if (a.conv_out ? (k != a.conv_k) : (kk.dst != dst)) continue;
+        __syncthreads();  // previous kernel's reads of tile / fragments are done
+        if (kk.src != staged) {
+            stage_tile(tile, a.state_in + (size_t)kk.src * W * H, W, H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc,
+                       a.tstride);
+            staged = kk.src;
+        }
+        for (int i = tid; i < kk.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)kk.frag0 * 64 + i];
+        __syncthreads();
+        f32x4 acc[NB];
+        conv_tile<NB>(acc, tile, frag_lds, a.sched + a.sched_base[k], kk.reach, a.halo_y, a.halo_x, a.tstride);
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float u = acc[b][r] * kk.scale;
+                if (a.conv_out) g[b][r] = u;
+                else g[b][r] += kk.weight * growth(u, kk.mu, kk.inv_sigma);
+            }
+    }
+
+    // Lane holds 4 consecutive output columns of one output row per block.
+    const int xo = x0 + 16 * strip + 4 * q;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int yo = y0 + half * 16 * NB + 16 * b + n;
+        const size_t o = (size_t)yo * W + xo;
+        if (a.conv_out) {
+            *reinterpret_cast<float4*>(a.conv_out + o) = make_float4(g[b][0], g[b][1], g[b][2], g[b][3]);
+        } else {
+            const size_t plane = (size_t)dst * W * H;
+            float4 s = *reinterpret_cast<const float4*>(a.state_in + plane + o);
+            float r[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) r[t] = fminf(fmaxf(r[t] + a.dt * g[b][t], 0.0f), 1.0f);
+            *reinterpret_cast<float4*>(a.state_out + plane + o) = make_float4(r[0], r[1], r[2], r[3]);
+        }
+    }
+}
+
+// Persistent batch kernel: workgroup b walks tiles [seg[b], seg[b+1]) of a
+// list grouped by world, cut into segments of near-equal MFMA cost; a tile
+// names its world, whose kernel fragments are reloaded only when the world
+// changes.  States ping-pong between buf[parity] and
+// buf[parity^1]; world w's cells live at the same offset in both.
+template <int NB>
+__global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
+    constexpr int DY = 32 * NB;
+    uint16_t* tile = lds_u16;
+    uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)a.tr * a.tstride);
+    const float* in = a.parity ? a.buf1 : a.buf0;
+    float* out = a.parity ? a.buf0 : a.buf1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int strip = wave & 3, half = wave >> 2;
+    const int q = lane >> 4, n = lane & 15;
+    int loaded = -1;
+    const int t_end = a.seg[blockIdx.x + 1];
+    for (int t = a.seg[blockIdx.x]; t < t_end; ++t) {
+        const int4 tj = a.tiles[t];  // (world, x0, y0, -)
+        const BatchWorld w = a.worlds[tj.x];
+        const int x0 = tj.y, y0 = tj.z;
+        __syncthreads();
+        stage_tile(tile, in + w.off, w.W, w.H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride);
+        if (tj.x != loaded) {
+            for (int i = tid; i < w.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)w.frag0 * 64 + i];
+            loaded = tj.x;
+        }
+        __syncthreads();
+        f32x4 acc[NB];
+        conv_tile<NB>(acc, tile, frag_lds, a.sched + w.sched0, w.reach, a.halo_y, a.halo_x, a.tstride);
+        const int xo = x0 + 16 * strip + 4 * q;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int yo = y0 + half * 16 * NB + 16 * b + n;
+            const size_t o = w.off + (size_t)yo * w.W + xo;
+            float4 s = *reinterpret_cast<const float4*>(in + o);
+            float r[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                r[e] = fminf(fmaxf(r[e] + w.dt * growth(acc[b][e] * w.scale, w.mu, w.inv_sigma), 0.0f), 1.0f);
+            *reinterpret_cast<float4*>(out + o) = make_float4(r[0], r[1], r[2], r[3]);
+        }
+    }
+    (void)DY;
+}
+
+// Dynamic LDS above 64 KiB must be allowed per kernel.  Done once per kernel
+// symbol (up to the 160 KiB a gfx950 workgroup can hold), never inside a
+// stream capture.
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    static std::vector<const void*> done;
+    const void* key = reinterpret_cast<const void*>(kernel);
+    if (bytes <= 64 * 1024 || std::find(done.begin(), done.end(), key) != done.end()) return;
+    AH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done.push_back(key);
+}
+
+}  // namespace
+
+int direct_rows_per_tile(int H) {
+    if (H >= 128) return 128;
+    if (H >= 64) return 64;
+    return 32;
+}
+
+size_t direct_lds_bytes(const DirectArgs& a) {
+    return (size_t)a.tr * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4);
+}
+
+size_t batch_lds_bytes(const BatchArgs& a) {
+    return (size_t)a.tr * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4);
+}
+
+void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int nb, int grid) {
+    const size_t lds = batch_lds_bytes(a);
+    Launch L(p, s, "direct_bf16_batch");
+    if (nb == 4) {
+        allow_lds(k_direct_batch<4>, lds);
+        hipLaunchKernelGGL(k_direct_batch<4>, dim3(grid), dim3(DTHREADS), lds, s, a);
+    } else if (nb == 2) {
+        allow_lds(k_direct_batch<2>, lds);
+        hipLaunchKernelGGL(k_direct_batch<2>, dim3(grid), dim3(DTHREADS), lds, s, a);
+    } else {
+        allow_lds(k_direct_batch<1>, lds);
+        hipLaunchKernelGGL(k_direct_batch<1>, dim3(grid), dim3(DTHREADS), lds, s, a);
+    }
+    AH_HIP(hipGetLastError());
+}
+
+void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels) {
+    const int DY = direct_rows_per_tile(a.H);
+    const size_t lds = direct_lds_bytes(a);
+    dim3 grid(a.W / DX, (a.H + DY - 1) / DY, a.conv_out ? 1 : channels);
+    Launch L(p, s, a.conv_out ? "direct_bf16_conv" : "direct_bf16_update");
+    if (DY == 128) {
+        allow_lds(k_direct<4>, lds);
+        hipLaunchKernelGGL(k_direct<4>, grid, dim3(DTHREADS), lds, s, a);
+    } else if (DY == 64) {
+        allow_lds(k_direct<2>, lds);
+        hipLaunchKernelGGL(k_direct<2>, grid, dim3(DTHREADS), lds, s, a);
+    } else {
+        allow_lds(k_direct<1>, lds);
+        hipLaunchKernelGGL(k_direct<1>, grid, dim3(DTHREADS), lds, s, a);
+    }
+    AH_HIP(hipGetLastError());
+}
+
+}  // namespace ah

@@ -59,6 +59,74 @@ Launch::~Launch() {
     }
 }
 
+
+// ------------------------------------------------------------ shared helpers
+// SplitMix64, crates/auto-life-gpu/src/world.rs:153-159.
+uint64_t splitmix64(uint64_t& s) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+double bf16_to_f64(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// Row schedule and bf16 fragments of one kernel for the direct path.  For
+// kernel row dy with chord half-width dxm the window starts at
+// e = -8*ceil(dxm/8) (16-byte aligned B reads) and needs
+// floor((dxm + 15 - e)/32) + 1 chunks of 32 input columns.  Fragment lane l,
+// element t holds bf16(K[dy][e + 32c + 8q + t - i]) with i = l%16, q = l/16.
+// Appends (e, chunks, first fragment, 0) per dy in [0, R] to `sched` and the
+// fragments to `frags`; returns the FP64 sum of all bf16 taps.
+double build_fragments(const std::vector<double>& t, int R, std::vector<int>& sched, std::vector<uint16_t>& frags,
+                       int& nfrag) {
+    const int side = 2 * R + 1;
+    std::vector<uint16_t> tb(t.size());
+    double sum = 0.0;
+    for (size_t i = 0; i < t.size(); ++i) {
+        tb[i] = f32_to_bf16((float)t[i]);
+        sum += bf16_to_f64(tb[i]);
+    }
+    auto tap = [&](int dy, int dx) -> uint16_t {
+        if (dx < -R || dx > R) return 0;
+        return tb[(size_t)(dy + R) * side + (dx + R)];
+    };
+    int nf = 0;
+    for (int dy = 0; dy <= R; ++dy) {
+        int dxm = -1;
+        for (int dx = 0; dx <= R; ++dx)
+            if (t[(size_t)(dy + R) * side + (dx + R)] != 0.0) dxm = dx;
+        int e = 0, chunks = 0;
+        if (dxm >= 0) {
+            e = -8 * ((dxm + 7) / 8);
+            chunks = (dxm + 15 - e) / 32 + 1;
+        }
+        sched.insert(sched.end(), {e, chunks, nf, 0});
+        for (int c = 0; c < chunks; ++c) {
+            for (int lane = 0; lane < 64; ++lane) {
+                const int i = lane & 15, q = lane >> 4;
+                for (int el = 0; el < 8; ++el) frags.push_back(tap(dy, e + 32 * c + 8 * q + el - i));
+            }
+            ++nf;
+        }
+    }
+    nfrag = nf;
+    return sum;
+}
+
 }  // namespace ah
 
 using namespace ah;
@@ -93,8 +161,7 @@ struct ah_sim {
     float2* twH = nullptr;
 
     // direct path
-    uint16_t* taps_bf16[AH_MAX_KERNELS]{};
-    float tap_scale[AH_MAX_KERNELS]{};
+    DirectArgs direct{};
 
     // stats
     double* pmass = nullptr;
@@ -151,6 +218,13 @@ void validate(const ah_lenia_desc& d) {
     if (d.path != AH_PATH_FFT_FP32 && d.path != AH_PATH_DIRECT_BF16) bad("unknown path");
     if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
         throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
+    if (d.path == AH_PATH_DIRECT_BF16) {
+        if (d.width < 64 || d.height < 32)
+            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path needs width >= 64 and height >= 32"};
+        for (uint32_t k = 0; k < d.n_kernels; ++k)
+            if (std::ceil((double)d.kernels[k].radius) > 31)
+                throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path supports kernel radius <= 31"};
+    }
     if (d.init > AH_INIT_ZERO) bad("unknown init");
     if (d.init == AH_INIT_ORBIUM && d.channels != 1) bad("Orbium init is single-channel");
     if (d.init == AH_INIT_PATCH && (d.init_size == 0 || d.init_size > side)) bad("patch size must be 1..side");
@@ -159,13 +233,58 @@ void validate(const ah_lenia_desc& d) {
     (void)wsum;
 }
 
-// SplitMix64, crates/auto-life-gpu/src/world.rs:153-159.
-uint64_t splitmix64(uint64_t& s) {
-    s += 0x9E3779B97F4A7C15ull;
-    uint64_t z = s;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
+
+// Fragments and row schedule of every kernel for the direct path, from the
+// FP64 taps on the device.  For kernel row dy with chord half-width dxm the
+// window starts at e = -8*ceil(dxm/8) (16-byte aligned B reads) and needs
+// floor((dxm + 15 - e)/32) + 1 chunks of 32 input columns.  Fragment lane l,
+// element t holds bf16(K[dy][e + 32c + 8q + t - i]) with i = l%16, q = l/16.
+void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
+    const ah_lenia_desc& d = s->d;
+    const int K = (int)d.n_kernels;
+    AH_HIP(hipStreamSynchronize(s->stream));  // taps are written on the sim's stream
+    DirectArgs& a = s->direct;
+    a = DirectArgs{};
+    a.W = (int)d.width;
+    a.H = (int)d.height;
+    int rmax = 0;
+    for (int k = 0; k < K; ++k) rmax = std::max(rmax, s->reach[k]);
+    a.halo_y = rmax;
+    a.halo_x = 8 * ((rmax + 7) / 8);
+    a.tr = direct_rows_per_tile(a.H) + 2 * a.halo_y;
+    a.tc = 64 + 2 * a.halo_x + 32;
+    a.tstride = a.tc + 8;
+    a.n_kernels = K;
+    std::vector<int> sched;
+    std::vector<uint16_t> frags;  // [fragment][64][8]
+    int nfrag_total = 0;
+    for (int k = 0; k < K; ++k) {
+        const int R = s->reach[k], side = 2 * R + 1;
+        std::vector<double> t(side * side);
+        AH_HIP(hipMemcpy(t.data(), taps[k], sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        DirectKernel& dk = a.k[k];
+        dk.reach = R;
+        dk.frag0 = nfrag_total;
+        a.sched_base[k] = (int)(sched.size() / 4);
+        dk.scale = (float)(1.0 / build_fragments(t, R, sched, frags, dk.nfrag));
+        dk.src = (int)d.kernels[k].src;
+        dk.dst = (int)d.kernels[k].dst;
+        dk.mu = s->kp[k].mu;
+        dk.inv_sigma = s->kp[k].inv_sigma;
+        dk.weight = s->kp[k].weight;
+        nfrag_total += dk.nfrag;
+        a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
+    }
+    if (direct_lds_bytes(a) > 160 * 1024)
+        throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
+    int4* dsched = (int4*)s->alloc(sizeof(int) * sched.size());
+    uint4* dfrags = (uint4*)s->alloc(sizeof(uint16_t) * frags.size());
+    AH_HIP(hipMemcpy(dsched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
+    AH_HIP(hipMemcpy(dfrags, frags.data(), sizeof(uint16_t) * frags.size(), hipMemcpyHostToDevice));
+    a.sched = dsched;
+    a.frags = dfrags;
+    a.dt = d.dt;
+    s->state_alt = (float*)s->alloc(sizeof(float) * s->cells() * d.channels);
 }
 
 void setup(ah_sim* s) {
@@ -234,7 +353,7 @@ void setup(ah_sim* s) {
                               s->khat + (size_t)k * (W / 8) * np * BC, s->kpack + (size_t)k * np);
             }
         } else {
-            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path not built yet"};
+            setup_direct(s, taps);
         }
 
         // Seed.
@@ -276,25 +395,21 @@ void step_fft(ah_sim* s, float* conv_out, int conv_k) {
                    conv_k);
 }
 
-template <typename F>
-ah_status guard(F&& f) {
-    try {
-        f();
-        return AH_OK;
-    } catch (const Error& e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_error("host allocation failed");
-        return AH_ERR_ALLOC;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return AH_ERR_DEVICE;
-    } catch (...) {
-        set_error("unknown failure");
-        return AH_ERR_DEVICE;
-    }
+void step_direct(ah_sim* s, float* conv_out, int conv_k) {
+    DirectArgs a = s->direct;
+    a.state_in = s->state;
+    a.state_out = s->state_alt;
+    a.conv_out = conv_out;
+    a.conv_k = conv_k;
+    launch_direct(s->stream, &s->prof, a, (int)s->d.channels);
+    if (!conv_out) std::swap(s->state, s->state_alt);
 }
+
+void step_once(ah_sim* s, float* conv_out, int conv_k) {
+    if (s->d.path == AH_PATH_FFT_FP32) step_fft(s, conv_out, conv_k);
+    else step_direct(s, conv_out, conv_k);
+}
+
 
 }  // namespace
 
@@ -343,7 +458,7 @@ ah_status ah_sim_step(ah_sim* s, uint32_t steps, uint64_t* generations) {
     return guard([&] {
         if (!s) throw Error{AH_ERR_INVALID, "null sim"};
         AH_HIP(hipSetDevice(s->device));
-        for (uint32_t i = 0; i < steps; ++i) step_fft(s, nullptr, 0);
+        for (uint32_t i = 0; i < steps; ++i) step_once(s, nullptr, 0);
         s->generation += steps;
         if (generations) *generations = steps;
     });
@@ -373,7 +488,7 @@ ah_status ah_sim_read_field(ah_sim* s, const char* name, void* dst, size_t bytes
             if (bytes != plane) throw Error{AH_ERR_SIZE, "conv fields are height*width*4 bytes"};
             float* u = nullptr;
             AH_HIP(hipMallocAsync((void**)&u, plane, s->stream));
-            step_fft(s, u, k);
+            step_once(s, u, k);
             hipError_t e = hipMemcpyAsync(dst, u, bytes, hipMemcpyDeviceToHost, s->stream);
             (void)hipFreeAsync(u, s->stream);
             AH_HIP(e);
@@ -469,6 +584,38 @@ ah_status ah_sim_profile_read(ah_sim* s, ah_kernel_time* out, uint32_t capacity,
         for (uint32_t i = 0; i < n; ++i) out[i] = agg[i];
         *count = (uint32_t)agg.size();
     });
+}
+
+ah_status ah_copy_probe(size_t bytes, uint32_t iters, double* ms_per_copy) {
+    void* a = nullptr;
+    void* b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ah_status r = guard([&] {
+        if (!ms_per_copy || bytes < 16 || bytes % 16 || iters == 0) throw Error{AH_ERR_INVALID, "bytes must be a multiple of 16, iters > 0"};
+        AH_HIP(hipMalloc(&a, bytes));
+        AH_HIP(hipMalloc(&b, bytes));
+        AH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        AH_HIP(hipEventCreate(&e0));
+        AH_HIP(hipEventCreate(&e1));
+        AH_HIP(hipMemsetAsync(a, 0, bytes, st));
+        const size_t n4 = bytes / 16;
+        launch_copy_probe(st, (const float4*)a, (float4*)b, n4);
+        AH_HIP(hipEventRecord(e0, st));
+        for (uint32_t i = 0; i < iters; ++i) launch_copy_probe(st, (const float4*)a, (float4*)b, n4);
+        AH_HIP(hipEventRecord(e1, st));
+        AH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = (double)ms / iters;
+    });
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (st) (void)hipStreamDestroy(st);
+    return r;
 }
 
 }  // extern "C"

@@ -24,6 +24,8 @@
 // Half-spectrum layout (complex64): spec[kb][y][c], kx = 4*kb + c.  Bin kx=0
 // packs the two real bins X[0] and X[W/2] as (re, im); the column pass treats
 // that column as two real sequences sharing one complex transform.
+#include <algorithm>
+
 #include "ah_internal.h"
 
 namespace ah {
@@ -410,11 +412,16 @@ __global__ void __launch_bounds__(1024) k_row_c2r(RowArgs a) {
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel; done once per kernel.
+// Dynamic LDS above 64 KiB must be allowed per kernel.  Done once per kernel
+// symbol (up to the 160 KiB a gfx950 workgroup can hold), never inside a
+// stream capture.
 template <typename K>
 void set_lds(K kernel, size_t bytes) {
-    if (bytes <= 64 * 1024) return;
-    AH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes));
+    static std::vector<const void*> done;
+    const void* key = reinterpret_cast<const void*>(kernel);
+    if (bytes <= 64 * 1024 || std::find(done.begin(), done.end(), key) != done.end()) return;
+    AH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done.push_back(key);
 }
 
 // Rows per row-pass workgroup: at least RW, enough to give >= 256 threads.

@@ -369,3 +369,16 @@ void launch_stats(hipStream_t s, Profiler* p, const float* state, size_t cells, 
 }
 
 }  // namespace ah
+
+namespace ah {
+namespace {
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace
+
+void launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n4) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, s, src, dst, n4);
+    AH_HIP(hipGetLastError());
+}
+}  // namespace ah

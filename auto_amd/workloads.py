@@ -75,6 +75,7 @@ def lenia_reference(width: int, height: int, path: str = "fft_fp32") -> LeniaDes
 
 
 def single_ring_256(path: str = "fft_fp32") -> LeniaDesc:
+    """256^2, R=13, mu=.15, sigma=.015, dt=.1, centred 64^2 patch."""
     return desc(256, 256, [dict(radius=13.0, mu=0.15, sigma=0.015)], 0.1, path,
                 init=INIT_PATCH, init_size=64)
 

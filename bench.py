@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Lenia world-step benchmark on one MI355X.
+
+Workload: the 4096x4096 three-ring world (R=26, beta=[1,2/3,1/3], mu=0.26,
+sigma=0.036, dt=0.1, 64 random 48x48 blobs), FP32 state.  One step is one
+Lenia generation of the whole torus; cell-updates/s = width*height*steps/s as
+the reference computes it (crates/auto-cli/src/bench.rs:161).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fft_fp32]
+
+Prints one JSON line.  `value` is the FFT path's cell-updates/s over exactly K
+timed steps with the state resident in HBM, bracketed by stream
+synchronisation.  Per-kernel device times come from HIP events recorded on the
+engine's own stream over the same timed region; `roofline` prices the
+dominant kernel's algorithmic bytes against the 8 TB/s HBM peak.  The CPU
+baseline times the oracle's FP32 restatement of the reference step on host
+cores over a bounded sample, and the same leg checks one GPU step against the
+oracle's FP64 step from the same input.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+# FFT-path kernels priced by algorithmic HBM bytes (single channel, one
+# kernel; the half spectrum is W/2 x H complex64 = 4 B/cell).
+FFT_KERNELS = ("row_r2c", "col_conv", "row_c2r_update")
+
+
+def khat_bytes(w: int, h: int) -> int:
+    """K-hat bytes one col_conv launch reads: [W/8][H/2+1][4] float + packed column."""
+    return (w // 8) * (h // 2 + 1) * 4 * 4 + (h // 2 + 1) * 8
+
+
+def kernel_bytes(name: str, w: int, h: int) -> float:
+    """row_r2c: read state + write spectrum; col_conv: read + write spectrum
+    + K-hat; row_c2r_update: read spectrum + read state + write state."""
+    cells = w * h
+    if name == "row_r2c":
+        return 8.0 * cells
+    if name == "col_conv":
+        return 8.0 * cells + khat_bytes(w, h)
+    if name == "row_c2r_update":
+        return 12.0 * cells
+    raise KeyError(name)
+
+
+def workload_name(d) -> str:
+    k = d.kernels[0]
+    return (f"rings3_{d.width}: {d.width}x{d.height} torus, R={k.radius:g}, "
+            f"beta={[round(b, 4) for b in list(k.beta)[:k.rings]]}, mu={k.mu:g}, sigma={k.sigma:g}, "
+            f"dt={d.dt:g}, {d.init_count} blobs of {d.init_size}^2")
+
+
+def time_path(path: str, side: int, steps: int, warmup: int, seed: int):
+    from auto_amd import Sim, workloads as W
+    d = W.rings3_4096(path=path, side=side)
+    sim = Sim(d, seed)
+    cells = d.width * d.height
+    # Warm up, including one profiled pass so the event pool is allocated.
+    sim.profile(True)
+    sim.step(warmup + steps)
+    sim.wait()
+    sim.profile(True)  # reset counters, keep the events
+    sim.wait()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.wait()
+    t1 = time.perf_counter()
+    prof = sim.profile_read()
+    sim.profile(False)
+    dt = t1 - t0
+    kernels = {n: {"launches": l, "avg_us": 1e3 * ms / l} for n, (l, ms) in prof.items()}
+    return d, sim, dict(cells=cells, seconds=dt, value=cells * steps / dt, ms_per_step=1e3 * dt / steps,
+                        kernels=kernels)
+
+
+def cpu_leg(d, sim, budget_s: float):
+    """Oracle leg: time the FP32 port of the reference step on host cores and
+    check one GPU step against the FP64 oracle step from the same input."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    O.set_threads(threads)
+    s = sim.read_field("state")
+    spectra = O.kernel_spectra32(d)
+    t0 = time.perf_counter()
+    n = 0
+    x = s
+    while True:
+        x = O.step_ref32(d, x, spectra)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s or n >= 50:
+            break
+    cpu_s = time.perf_counter() - t0
+    cells = d.width * d.height
+    base = {"value": cells * n / cpu_s, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} step(s) of the same {d.width}x{d.height} world on {threads} OpenMP threads "
+                      f"(C restatement of the reference FP32 Stockham step, {cpu_s:.1f} s)"}
+    # Error of one GPU step vs FP64 from the same input state.
+    sim.step(1)
+    got = sim.read_field("state")
+    want = O.step64(d, s, "fft")
+    diff = got.astype(np.float64) - want
+    err = {"max_abs": float(np.abs(diff).max()),
+           "rel_l2": float(np.linalg.norm(diff) / max(np.linalg.norm(want), 1e-300)),
+           "vs_cpu_fp32_port_max_abs": float(np.abs(got - O.step_ref32(d, s, spectra)).max())}
+    return base, err
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--side", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--path", default="fft_fp32", choices=["fft_fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}). "
+              "A batch of worlds shards by world and one large world by slab FFT, but neither is built.",
+              file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    from auto_amd import _abi
+    import ctypes
+    lib = _abi.load()
+    probe_bytes = 1 << 28
+    ms = ctypes.c_double()
+    _abi.check(lib.ah_copy_probe(probe_bytes, 20, ctypes.byref(ms)))
+    copy_gbps = 2 * probe_bytes / (ms.value * 1e-3) / 1e9
+
+    d, sim, r = time_path(args.path, args.side, args.steps, args.warmup, args.seed)
+    w, h = d.width, d.height
+    kern = r["kernels"]
+    for name, k in kern.items():
+        if name in FFT_KERNELS:
+            b = kernel_bytes(name, w, h)
+            k["bytes"] = b
+            k["GBps"] = b / (k["avg_us"] * 1e-6) / 1e9
+    dom = max((n for n in kern if "bytes" in kern[n]), key=lambda n: kern[n]["avg_us"])
+    step_bytes = sum(kern[n]["bytes"] for n in kern if "bytes" in kern[n])
+    step_us = sum(kern[n]["avg_us"] for n in kern if "bytes" in kern[n])
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBPS, "traffic": None,
+                "bytes_per_launch": kern[dom]["bytes"],
+                "step": {"bytes": step_bytes, "kernel_us": step_us,
+                         "achieved": step_bytes / (step_us * 1e-6) / 1e9,
+                         "frac": step_bytes / (step_us * 1e-6) / 1e9 / HBM_PEAK_GBPS},
+                "copy_probe_GBps": copy_gbps}
+
+    out = {
+        "metric": "Lenia cell-updates/s at 4096^2 (plain FP32 FFT vs BF16 MFMA) at <=1e-3 err",
+        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload_name(d), "path": args.path, "seed": args.seed},
+        "roofline": roofline,
+        "kernels": kern,
+    }
+    if not args.no_cpu and rank == 0:
+        base, err = cpu_leg(d, sim, args.cpu_seconds)
+        out["cpu_baseline"] = base
+        out["err_vs_fp64_one_step"] = err
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -158,6 +158,64 @@ typedef struct {
 ah_status ah_sim_profile(ah_sim* sim, int enable);
 ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacity, uint32_t* count);
 
+/*
+ * Batched uneven worlds.  No counterpart in the reference, which runs one
+ * world per Sim (sim.rs:202-309).  Every world is single-channel with one
+ * kernel (radius <= 31), width >= 64, height >= 32, path AH_PATH_DIRECT_BF16,
+ * and an ORBIUM, PATCH or ZERO init seeded with its own seed.  Worlds differ in
+ * size, rule and dt.  All worlds advance together in one persistent launch per
+ * step; ah_batch_step replays hipGraph-captured chunks of `graph_steps`
+ * launches and launches the remainder one by one.  At creation an autotuner
+ * times every admissible (tile rows, grid) pair on the batch and keeps the
+ * fastest; the candidates and their times are reported by ah_batch_info_get.
+ */
+typedef struct ah_batch ah_batch;
+
+enum { AH_BATCH_MAX_CANDIDATES = 8 };
+
+typedef struct {
+    uint32_t tile_rows;  /* output rows per tile (tiles are 64 columns wide) */
+    uint32_t grid;       /* persistent workgroups                             */
+    double ms_per_step;  /* measured by the autotuner                         */
+} ah_batch_tune;
+
+typedef struct {
+    uint32_t n_worlds;
+    uint32_t tile_rows, grid, ntiles, graph_steps;
+    uint64_t total_cells;  /* cell-updates per step */
+    uint64_t total_bytes;
+    uint64_t generation;
+    uint32_t n_tune;
+    ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES];
+} ah_batch_info;
+
+ah_status ah_batch_new(const ah_lenia_desc* descs, uint32_t n_worlds, const uint32_t* seeds, ah_batch** out);
+ah_status ah_batch_step(ah_batch* batch, uint32_t steps);
+ah_status ah_batch_wait(ah_batch* batch);
+/* One world's state, float32 [height][width]; `bytes` must match exactly. */
+ah_status ah_batch_read_world(ah_batch* batch, uint32_t world, float* dst, size_t bytes);
+ah_status ah_batch_write_world(ah_batch* batch, uint32_t world, const float* src, size_t bytes);
+ah_status ah_batch_info_get(const ah_batch* batch, ah_batch_info* out);
+/*
+ * Launch-overhead measurement: runs `steps` (a multiple of graph_steps) eager
+ * launches with per-launch HIP events (summed device time -> kernel_ms, host
+ * wall time -> eager_wall_ms), then the same number of steps as graph
+ * replays (host wall time -> graph_wall_ms).  Advances the batch 2*steps.
+ */
+ah_status ah_batch_measure(ah_batch* batch, uint32_t steps, double* kernel_ms, double* graph_wall_ms,
+                           double* eager_wall_ms);
+void ah_batch_destroy(ah_batch* batch);
+
+/*
+ * Bandwidth probe: a plain vectorised device-to-device copy of `bytes`
+ * (read `bytes`, write `bytes`), run `iters` times on a private stream after
+ * one warm-up.  Returns the mean device milliseconds per copy.  It gives the
+ * practical HBM ceiling beside the 8 TB/s spec figure and the known-size
+ * kernel ("copy_probe") that byte counters are calibrated against.  No
+ * counterpart in the reference.
+ */
+ah_status ah_copy_probe(size_t bytes, uint32_t iters, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
