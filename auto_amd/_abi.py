@@ -23,13 +23,16 @@ ERR_DIVERGED = 7
 
 PATH_FFT_FP32 = 0
 PATH_DIRECT_BF16 = 1
+PATH_DIRECT_BF16_X1 = 2
+PATH_DIRECT_BF16_X3 = 3
 
 INIT_ORBIUM = 0
 INIT_PATCH = 1
 INIT_BLOBS = 2
 INIT_ZERO = 3
 
-PATHS = {"fft_fp32": PATH_FFT_FP32, "direct_bf16": PATH_DIRECT_BF16}
+PATHS = {"fft_fp32": PATH_FFT_FP32, "direct_bf16": PATH_DIRECT_BF16,
+         "direct_bf16_x1": PATH_DIRECT_BF16_X1, "direct_bf16_x3": PATH_DIRECT_BF16_X3}
 
 
 class Kernel(ctypes.Structure):

@@ -45,8 +45,12 @@ ah_status guard(F&& f) {
 uint64_t splitmix64(uint64_t& state);
 uint16_t f32_to_bf16(float f);
 double bf16_to_f64(uint16_t h);
-double build_fragments(const std::vector<double>& taps, int R, std::vector<int>& sched, std::vector<uint16_t>& frags,
-                       int& nfrag);
+// Appends the row schedule and the hi fragments (then, for terms == 3, the lo
+// fragments) of one kernel; returns the FP64 sum of the bf16 taps used.
+double build_fragments(const std::vector<double>& taps, int R, int terms, std::vector<int>& sched,
+                       std::vector<uint16_t>& frags, int& nfrag);
+// 1 or 3 bf16 terms for a kernel with these taps and growth (see auto_hip.h).
+int direct_terms(const std::vector<double>& taps, float dt, float weight, float sigma, uint32_t path);
 
 #define AH_HIP(call)                                                                  \
     do {                                                                              \
@@ -124,7 +128,8 @@ size_t fft_lds_limit();
 // direct_bf16.hip
 struct DirectKernel {  // one kernel as the direct path reads it
     int reach;         // R: rows dy in [-R, R]
-    int frag0, nfrag;  // fragments of this kernel in the fragment table
+    int frag0, nfrag;  // hi fragments [frag0, frag0+nfrag); lo fragments follow them
+    int terms;         // 1: K_hi s_hi; 3: + K_hi s_lo + K_lo s_hi
     float scale;       // 1 / sum of the bf16-rounded taps
     int src, dst;
     float mu, inv_sigma, weight;
@@ -149,7 +154,7 @@ struct DirectArgs {
 struct BatchWorld {  // one world of a batch as the batch kernel reads it
     int W, H;
     size_t off;      // first cell in both state buffers
-    int reach, frag0, nfrag, sched0;
+    int reach, frag0, nfrag, sched0, terms;
     float scale, mu, inv_sigma, dt;
 };
 

@@ -103,7 +103,8 @@ bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 void validate_world(const ah_lenia_desc& d, uint32_t i) {
     auto bad = [&](const std::string& m) { throw Error{AH_ERR_INVALID, "world " + std::to_string(i) + ": " + m}; };
-    if (d.path != AH_PATH_DIRECT_BF16) bad("batched worlds run the direct BF16 path (path must be AH_PATH_DIRECT_BF16)");
+    if (d.path == AH_PATH_FFT_FP32 || d.path > AH_PATH_DIRECT_BF16_X3)
+        bad("batched worlds run the direct BF16 path (path must be one of the AH_PATH_DIRECT_BF16 values)");
     if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
     if (d.width < 64 || d.height < 32 || d.width > 8192 || d.height > 8192) bad("width 64..8192, height 32..8192");
     if (d.channels != 1 || d.n_kernels != 1) bad("batched worlds are single-channel with one kernel");
@@ -257,14 +258,15 @@ void setup(ah_batch* b) {
             bw.reach = R;
             bw.frag0 = nfrag_total;
             bw.sched0 = (int)(sched.size() / 4);
-            bw.scale = (float)(1.0 / build_fragments(t, R, sched, frags, bw.nfrag));
+            bw.terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, d.path);
+            bw.scale = (float)(1.0 / build_fragments(t, R, bw.terms, sched, frags, bw.nfrag));
             bw.mu = d.kernels[0].mu;
             bw.inv_sigma = 1.0f / d.kernels[0].sigma;
             bw.dt = d.dt;
             int chunks = 0;
             for (int dy = -R; dy <= R; ++dy) chunks += sched[(bw.sched0 + std::abs(dy)) * 4 + 1];
-            b->cost[w] = chunks;
-            nfrag_total += bw.nfrag;
+            b->cost[w] = chunks * bw.terms;
+            nfrag_total += bw.nfrag * (bw.terms == 3 ? 2 : 1);
             b->nfrag_max = std::max(b->nfrag_max, bw.nfrag);
             off += (size_t)d.width * d.height;
         }

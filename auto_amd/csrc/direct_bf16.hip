@@ -44,11 +44,15 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
     return u >> 16;
 }
 
+__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+
 // Stage rows [y0-hy, y0+DY+hy) x cols [x0-hx, x0-hx+tc) of one channel as
-// bf16, wrapped on the torus.  Columns outside [x0-hy, x0+DX+hy) are zero:
-// no nonzero tap reaches them, and zeros keep non-finite data out of range.
+// bf16, wrapped on the torus: the value's bf16 rounding (hi), or with lo set
+// the bf16 rounding of what hi leaves over, x - hi.  Columns outside
+// [x0-hy, x0+DX+hy) are zero: no nonzero tap reaches them, and zeros keep
+// non-finite data out of range.
 __device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int W, int H, int x0, int y0,
-                                           int halo_y, int halo_x, int tr, int tc, int tstride) {
+                                           int halo_y, int halo_x, int tr, int tc, int tstride, bool lo) {
     const int groups = tc >> 2;
     for (int i = threadIdx.x; i < tr * groups; i += DTHREADS) {
         int r = i / groups, j = (i - r * groups) * 4;
@@ -63,21 +67,22 @@ __device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             int c = cx + t;
-            h[t] = (c >= -halo_y && c < DX + halo_y) ? bf16_bits(e[t]) : 0u;
+            uint32_t hv = bf16_bits(e[t]);
+            if (lo) hv = bf16_bits(e[t] - bf16_value(hv));
+            h[t] = (c >= -halo_y && c < DX + halo_y) ? hv : 0u;
         }
         *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
     }
 }
 
-// All MFMAs of one kernel over this wave's 16 columns x 16*NB rows.
+// All MFMAs of one kernel term over this wave's 16 columns x 16*NB rows,
+// accumulated into acc.
 template <int NB>
 __device__ __forceinline__ void conv_tile(f32x4 (&acc)[NB], const uint16_t* tile, const uint4* frag_lds,
                                           const int4* sched, int reach, int halo_y, int halo_x, int tstride) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int strip = wave & 3, half = wave >> 2;
     const int q = lane >> 4, n = lane & 15;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int row0 = half * 16 * NB + n + halo_y;
     const int colS = 16 * strip + halo_x + 8 * q;
     for (int dy = -reach; dy <= reach; ++dy) {
@@ -91,6 +96,39 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[NB], const uint16_t* tile
                 acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bv), acc[b], 0, 0, 0);
             }
         }
+    }
+}
+
+// One kernel's potential over the tile: K_hi s_hi, plus K_lo s_hi and
+// K_hi s_lo for three terms.  `staged` tracks which (source, hi/lo) tile sits
+// in LDS (source*2 + lo) so that consecutive kernels on one source skip the
+// restaging; `loaded` tracks the fragment block in LDS the same way.
+template <int NB>
+__device__ __forceinline__ void kernel_potential(f32x4 (&acc)[NB], uint16_t* tile, uint4* frag_lds, int& staged,
+                                                 int& loaded,
+                                                 const float* src_plane, int src, int W, int H, int x0, int y0,
+                                                 int halo_y, int halo_x, int tr, int tc, int tstride,
+                                                 const uint4* __restrict__ frags, int frag0, int nfrag, int terms,
+                                                 const int4* sched, int reach) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // (tile lo?, fragment block) per term
+    const int nterm = terms == 3 ? 3 : 1;
+    for (int term = 0; term < nterm; ++term) {
+        const bool lo_tile = term == 2;
+        const int fbase = frag0 + (term == 1 ? nfrag : 0);
+        __syncthreads();  // earlier reads of tile / fragments are done
+        const int want = src * 2 + (lo_tile ? 1 : 0);
+        if (staged != want) {
+            stage_tile(tile, src_plane, W, H, x0, y0, halo_y, halo_x, tr, tc, tstride, lo_tile);
+            staged = want;
+        }
+        if (loaded != fbase) {
+            for (int i = threadIdx.x; i < nfrag * 64; i += DTHREADS) frag_lds[i] = frags[(size_t)fbase * 64 + i];
+            loaded = fbase;
+        }
+        __syncthreads();
+        conv_tile<NB>(acc, tile, frag_lds, sched, reach, halo_y, halo_x, tstride);
     }
 }
 
@@ -118,20 +156,14 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) g[b][r] = 0.0f;
 
-    int staged = -1;
+    int staged = -1, loaded = -1;
     for (int k = 0; k < a.n_kernels; ++k) {
         const DirectKernel kk = a.k[k];
         if (a.conv_out ? (k != a.conv_k) : (kk.dst != dst)) continue;
-        __syncthreads();  // previous kernel's reads of tile / fragments are done
-        if (kk.src != staged) {
-            stage_tile(tile, a.state_in + (size_t)kk.src * W * H, W, H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc,
-                       a.tstride);
-            staged = kk.src;
-        }
-        for (int i = tid; i < kk.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)kk.frag0 * 64 + i];
-        __syncthreads();
         f32x4 acc[NB];
-        conv_tile<NB>(acc, tile, frag_lds, a.sched + a.sched_base[k], kk.reach, a.halo_y, a.halo_x, a.tstride);
+        kernel_potential<NB>(acc, tile, frag_lds, staged, loaded, a.state_in + (size_t)kk.src * W * H, kk.src, W, H, x0, y0,
+                             a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, a.frags, kk.frag0, kk.nfrag, kk.terms,
+                             a.sched + a.sched_base[k], kk.reach);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -163,8 +195,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
 
 // Persistent batch kernel: workgroup b walks tiles [seg[b], seg[b+1]) of a
 // list grouped by world, cut into segments of near-equal MFMA cost; a tile
-// names its world, whose kernel fragments are reloaded only when the world
-// changes.  States ping-pong between buf[parity] and
+// names its world (size, offset, kernel fragments, growth, dt).  States ping-pong between buf[parity] and
 // buf[parity^1]; world w's cells live at the same offset in both.
 template <int NB>
 __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
@@ -183,15 +214,10 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
         const int4 tj = a.tiles[t];  // (world, x0, y0, -)
         const BatchWorld w = a.worlds[tj.x];
         const int x0 = tj.y, y0 = tj.z;
-        __syncthreads();
-        stage_tile(tile, in + w.off, w.W, w.H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride);
-        if (tj.x != loaded) {
-            for (int i = tid; i < w.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)w.frag0 * 64 + i];
-            loaded = tj.x;
-        }
-        __syncthreads();
+        int staged = -1;  // a new tile always restages
         f32x4 acc[NB];
-        conv_tile<NB>(acc, tile, frag_lds, a.sched + w.sched0, w.reach, a.halo_y, a.halo_x, a.tstride);
+        kernel_potential<NB>(acc, tile, frag_lds, staged, loaded, in + w.off, 0, w.W, w.H, x0, y0, a.halo_y, a.halo_x, a.tr,
+                             a.tc, a.tstride, a.frags, w.frag0, w.nfrag, w.terms, a.sched + w.sched0, w.reach);
         const int xo = x0 + 16 * strip + 4 * q;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {

@@ -89,21 +89,21 @@ double bf16_to_f64(uint16_t h) {
 // e = -8*ceil(dxm/8) (16-byte aligned B reads) and needs
 // floor((dxm + 15 - e)/32) + 1 chunks of 32 input columns.  Fragment lane l,
 // element t holds bf16(K[dy][e + 32c + 8q + t - i]) with i = l%16, q = l/16.
-// Appends (e, chunks, first fragment, 0) per dy in [0, R] to `sched` and the
-// fragments to `frags`; returns the FP64 sum of all bf16 taps.
-double build_fragments(const std::vector<double>& t, int R, std::vector<int>& sched, std::vector<uint16_t>& frags,
-                       int& nfrag) {
+// Appends (e, chunks, first fragment, 0) per dy in [0, R] to `sched`, the hi
+// fragments to `frags` and, for three terms, the lo fragments
+// bf16(K - bf16(K)) right after them (same schedule, offset nfrag).
+double build_fragments(const std::vector<double>& t, int R, int terms, std::vector<int>& sched,
+                       std::vector<uint16_t>& frags, int& nfrag) {
     const int side = 2 * R + 1;
-    std::vector<uint16_t> tb(t.size());
+    std::vector<uint16_t> hi(t.size()), lo(t.size());
     double sum = 0.0;
     for (size_t i = 0; i < t.size(); ++i) {
-        tb[i] = f32_to_bf16((float)t[i]);
-        sum += bf16_to_f64(tb[i]);
+        hi[i] = f32_to_bf16((float)t[i]);
+        lo[i] = f32_to_bf16((float)(t[i] - bf16_to_f64(hi[i])));
+        sum += bf16_to_f64(hi[i]);
+        if (terms == 3) sum += bf16_to_f64(lo[i]);
     }
-    auto tap = [&](int dy, int dx) -> uint16_t {
-        if (dx < -R || dx > R) return 0;
-        return tb[(size_t)(dy + R) * side + (dx + R)];
-    };
+    std::vector<int> rows;  // (e, chunks) per dy
     int nf = 0;
     for (int dy = 0; dy <= R; ++dy) {
         int dxm = -1;
@@ -115,16 +115,35 @@ double build_fragments(const std::vector<double>& t, int R, std::vector<int>& sc
             chunks = (dxm + 15 - e) / 32 + 1;
         }
         sched.insert(sched.end(), {e, chunks, nf, 0});
-        for (int c = 0; c < chunks; ++c) {
-            for (int lane = 0; lane < 64; ++lane) {
-                const int i = lane & 15, q = lane >> 4;
-                for (int el = 0; el < 8; ++el) frags.push_back(tap(dy, e + 32 * c + 8 * q + el - i));
-            }
-            ++nf;
+        rows.insert(rows.end(), {e, chunks});
+        nf += chunks;
+    }
+    for (int part = 0; part < (terms == 3 ? 2 : 1); ++part) {
+        const std::vector<uint16_t>& tb = part ? lo : hi;
+        for (int dy = 0; dy <= R; ++dy) {
+            const int e = rows[2 * dy], chunks = rows[2 * dy + 1];
+            for (int c = 0; c < chunks; ++c)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int i = lane & 15, q = lane >> 4;
+                    for (int el = 0; el < 8; ++el) {
+                        const int dx = e + 32 * c + 8 * q + el - i;
+                        frags.push_back(dx < -R || dx > R ? 0 : tb[(size_t)(dy + R) * side + (dx + R)]);
+                    }
+                }
         }
     }
     nfrag = nf;
     return sum;
+}
+
+int direct_terms(const std::vector<double>& t, float dt, float weight, float sigma, uint32_t path) {
+    if (path == AH_PATH_DIRECT_BF16_X1) return 1;
+    if (path == AH_PATH_DIRECT_BF16_X3) return 3;
+    double sq = 0.0;
+    for (double v : t) sq += v * v;
+    const double neff = 1.0 / sq;
+    const double slope = 1.213 * (double)dt * (double)weight / (double)sigma;
+    return slope * 0.0078 / std::sqrt(neff) <= 1e-3 ? 1 : 3;
 }
 
 }  // namespace ah
@@ -215,10 +234,10 @@ void validate(const ah_lenia_desc& d) {
         if (kd.src >= d.channels || kd.dst >= d.channels) bad("kernel src/dst out of range");
         wsum[kd.dst] += kd.weight;
     }
-    if (d.path != AH_PATH_FFT_FP32 && d.path != AH_PATH_DIRECT_BF16) bad("unknown path");
+    if (d.path > AH_PATH_DIRECT_BF16_X3) bad("unknown path");
     if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
         throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
-    if (d.path == AH_PATH_DIRECT_BF16) {
+    if (d.path != AH_PATH_FFT_FP32) {
         if (d.width < 64 || d.height < 32)
             throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path needs width >= 64 and height >= 32"};
         for (uint32_t k = 0; k < d.n_kernels; ++k)
@@ -266,13 +285,14 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         dk.reach = R;
         dk.frag0 = nfrag_total;
         a.sched_base[k] = (int)(sched.size() / 4);
-        dk.scale = (float)(1.0 / build_fragments(t, R, sched, frags, dk.nfrag));
+        dk.terms = direct_terms(t, d.dt, s->kp[k].weight, d.kernels[k].sigma, d.path);
+        dk.scale = (float)(1.0 / build_fragments(t, R, dk.terms, sched, frags, dk.nfrag));
         dk.src = (int)d.kernels[k].src;
         dk.dst = (int)d.kernels[k].dst;
         dk.mu = s->kp[k].mu;
         dk.inv_sigma = s->kp[k].inv_sigma;
         dk.weight = s->kp[k].weight;
-        nfrag_total += dk.nfrag;
+        nfrag_total += dk.nfrag * (dk.terms == 3 ? 2 : 1);
         a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
     }
     if (direct_lds_bytes(a) > 160 * 1024)

@@ -93,6 +93,28 @@ __device__ __forceinline__ void dft(float2* v) {
     else dft8<SIGN>(v);
 }
 
+// Twiddles w^r, r = 1..R-1, with w = tw[k*tstep] (conjugated for SIGN > 0),
+// from at most three table loads (w, w^2, w^4) and their products.
+template <int R, int SIGN>
+__device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ tw, int k, int tstep) {
+    auto ld = [&](int m) {
+        float2 v = tw[m * k * tstep];
+        if (SIGN > 0) v.y = -v.y;
+        return v;
+    };
+    w[1] = ld(1);
+    if constexpr (R >= 4) {
+        w[2] = ld(2);
+        w[3] = cmul(w[1], w[2]);
+    }
+    if constexpr (R >= 8) {
+        w[4] = ld(4);
+        w[5] = cmul(w[1], w[4]);
+        w[6] = cmul(w[2], w[4]);
+        w[7] = cmul(w[3], w[4]);
+    }
+}
+
 // One radix-R Stockham stage over `nseq` sequences of length M stored back
 // to back in LDS.  Every thread owns E/R butterflies; all loads happen before
 // the first barrier, all stores after it.  tw[t] = exp(-2*pi*i*t/NT);
@@ -120,12 +142,10 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         if (Ls > 1) {
+            float2 w[R];
+            twiddles<R, SIGN>(w, tw, kk[i], tstep);
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * kk[i] * tstep];
-                if (SIGN > 0) w.y = -w.y;
-                v[i][r] = cmul(v[i][r], w);
-            }
+            for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], w[r]);
         }
         dft<R, SIGN>(v[i]);
 #pragma unroll
@@ -135,7 +155,7 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
 }
 
 // Complex FFT of length M (power of two, >= 8) on nseq sequences in LDS.
-// E = elements per thread = nseq*M/blockDim.x (8 or 16).
+// E = elements per thread = nseq*M/blockDim.x.
 template <int E, int SIGN>
 __device__ void fft_lds(float2* lds, int M, int nseq, const float2* __restrict__ tw, int NT) {
     int Ls = 1;
@@ -147,19 +167,30 @@ __device__ void fft_lds(float2* lds, int M, int nseq, const float2* __restrict__
     else if (Ls * 2 == M) stage<2, E, SIGN>(lds, M, nseq, Ls, tw, NT);
 }
 
+// gfx950 hands consecutive workgroup ids to its 8 XCDs in turn.  Remapping
+// makes consecutive logical ids run on one XCD at about the same time, so the
+// rows (or columns) that share a 128-byte spectrum line meet in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    if (G & 7) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+constexpr int FFT_E = 8;          // complex elements per thread in every pass
+constexpr int FFT_THREADS = 512;  // upper bound on the workgroup size
+
 // ------------------------------------------------------------ pass 1
-// Grid: H/rows workgroups, rows consecutive rows each; block = rows*M/E threads.
-template <int E>
-__global__ void __launch_bounds__(1024) k_row_r2c(int W, int H, int rows, const float* __restrict__ state,
-                                                  float2* __restrict__ spec, const float2* __restrict__ twW) {
+// Grid: (H/rows, channels); `rows` consecutive rows per workgroup (one row
+// when W >= 4096), rows*M/8 threads.
+__global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows, const float* __restrict__ state,
+                                                         float2* __restrict__ spec, const float2* __restrict__ twW) {
     extern __shared__ float2 lds[];
     const int M = W >> 1;
-    const int y0 = blockIdx.x * rows;
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * rows;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int ch = blockIdx.y;
     state += (size_t)ch * W * H;
     spec += (size_t)ch * (W >> 1) * H;
-    // rows consecutive rows are one contiguous run of rows*M float2.
+    // `rows` consecutive rows are one contiguous run of rows*M float2.
     const float4* src = reinterpret_cast<const float4*>(state + (size_t)y0 * W);
     const int n4 = rows * M / 2;
     for (int i = tid; i < n4; i += nt) {
@@ -168,7 +199,7 @@ __global__ void __launch_bounds__(1024) k_row_r2c(int W, int H, int rows, const 
         lds[pad(2 * i + 1)] = make_float2(v.z, v.w);
     }
     __syncthreads();
-    fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
+    fft_lds<FFT_E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
     // Even/odd split: X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[M-k])/2,
     // O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output float4 q
     // holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
@@ -199,12 +230,13 @@ __global__ void __launch_bounds__(1024) k_row_r2c(int W, int H, int rows, const 
 }
 
 // ------------------------------------------------------------ pass 2
-// Grid: (W/8 blocks, source channels); block = 4*H/E threads.
+// Grid: (W/2/cw, source channels); each workgroup owns `cw` (1, 2 or 4)
+// adjacent columns of one 4-column block, all H rows; cw*H/8 threads.
 // spec_in: [channels][W/8][H][4]; spec_out: [n_kernels][W/8][H][4] (may alias
 // spec_in when one kernel reads one channel).  khat: [n_kernels][W/8][H/2+1][4],
 // kpack: [n_kernels][H/2+1].
 struct ColArgs {
-    int W, H;
+    int W, H, cw;
     const float2* spec_in;
     float2* spec_out;
     const float* khat;
@@ -214,24 +246,41 @@ struct ColArgs {
     int ksrc[AH_MAX_KERNELS];
 };
 
-// Multiply the transformed block in LDS by kernel k's spectrum, in place.
+// Element (y, column c of this workgroup) of a block in global memory.
+__device__ __forceinline__ size_t col_index(int kb, int c, int y, int H) { return ((size_t)kb * H + y) * BC + c; }
+
+__device__ __forceinline__ void load_cols(float2* lds, const float2* __restrict__ src, int kb, int c0, int cw, int H) {
+    for (int t = threadIdx.x; t < cw * H; t += blockDim.x) {
+        int y = t / cw, c = t - y * cw;
+        lds[pad(c * H + y)] = src[col_index(kb, c0 + c, y, H)];
+    }
+}
+
+__device__ __forceinline__ void store_cols(const float2* lds, float2* __restrict__ dst, int kb, int c0, int cw, int H) {
+    for (int t = threadIdx.x; t < cw * H; t += blockDim.x) {
+        int y = t / cw, c = t - y * cw;
+        dst[col_index(kb, c0 + c, y, H)] = lds[pad(c * H + y)];
+    }
+}
+
+// Multiply the transformed columns in LDS by kernel k's spectrum, in place.
 // Pairs (ky, H-ky) are owned by one thread, which the packed kx=0 column
 // needs: D[ky] = A C[ky] + B conj C[-ky] with A=(K0+KN)/2, B=(K0-KN)/2.
-__device__ __forceinline__ void multiply_block(float2* lds, const ColArgs& a, int kb, int k) {
-    const int H = a.H, half = H >> 1;
+__device__ __forceinline__ void multiply_cols(float2* lds, const ColArgs& a, int kb, int c0, int k) {
+    const int H = a.H, half = H >> 1, cw = a.cw;
     const int npairs = half + 1;
     const float* kh = a.khat + ((size_t)k * (a.W >> 3) + kb) * (size_t)npairs * BC;
-    for (int t = threadIdx.x; t < npairs * BC; t += blockDim.x) {
-        int p = t / BC, c = t - p * BC;
+    for (int t = threadIdx.x; t < npairs * cw; t += blockDim.x) {
+        int p = t / cw, c = t - p * cw;
         int y1 = p, y2 = (H - p) & (H - 1);
         float2 c1 = lds[pad(c * H + y1)], c2 = lds[pad(c * H + y2)];
         float2 d1, d2;
-        if (kb == 0 && c == 0) {
+        if (kb == 0 && c0 + c == 0) {
             float2 ab = a.kpack[(size_t)k * npairs + p];
             d1 = cadd(make_float2(ab.x * c1.x, ab.x * c1.y), make_float2(ab.y * c2.x, -ab.y * c2.y));
             d2 = cadd(make_float2(ab.x * c2.x, ab.x * c2.y), make_float2(ab.y * c1.x, -ab.y * c1.y));
         } else {
-            float kv = kh[t];
+            float kv = kh[(size_t)p * BC + c0 + c];
             d1 = make_float2(kv * c1.x, kv * c1.y);
             d2 = make_float2(kv * c2.x, kv * c2.y);
         }
@@ -241,60 +290,46 @@ __device__ __forceinline__ void multiply_block(float2* lds, const ColArgs& a, in
     __syncthreads();
 }
 
-__device__ __forceinline__ void store_block(const float2* lds, float2* dst, int H) {
-    float4* d4 = reinterpret_cast<float4*>(dst);
-    for (int q = threadIdx.x; q < 2 * H; q += blockDim.x) {
-        int y = q >> 1, cp = (q & 1) * 2;
-        float2 u = lds[pad(cp * H + y)], v = lds[pad((cp + 1) * H + y)];
-        d4[q] = make_float4(u.x, u.y, v.x, v.y);
-    }
-}
-
-template <int E>
-__global__ void __launch_bounds__(1024) k_col_conv(ColArgs a) {
+__global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     extern __shared__ float2 lds[];
-    const int H = a.H;
-    const int kb = blockIdx.x, ch = blockIdx.y;
-    const size_t blk = (size_t)H * BC;  // float2 per block
-    const float2* src = a.spec_in + ((size_t)ch * (a.W >> 3) + kb) * blk;
-    {
-        const float4* s4 = reinterpret_cast<const float4*>(src);
-        for (int q = threadIdx.x; q < 2 * H; q += blockDim.x) {
-            float4 v = s4[q];
-            int y = q >> 1, cp = (q & 1) * 2;
-            lds[pad(cp * H + y)] = make_float2(v.x, v.y);
-            lds[pad((cp + 1) * H + y)] = make_float2(v.z, v.w);
-        }
-    }
+    const int H = a.H, cw = a.cw;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int kb = (L * cw) >> 2, c0 = (L * cw) & 3;
+    const int ch = blockIdx.y;
+    const size_t chan = (size_t)(a.W >> 1) * H;  // float2 per channel / kernel spectrum
+    load_cols(lds, a.spec_in + ch * chan, kb, c0, cw, H);
     __syncthreads();
-    fft_lds<E, -1>(lds, H, BC, a.twH, H);
+    fft_lds<FFT_E, -1>(lds, H, cw, a.twH, H);
     int users = 0, last = -1;
     for (int k = 0; k < a.n_kernels; ++k)
-        if (a.ksrc[k] == ch) { ++users; last = k; }
+        if (a.ksrc[k] == ch) {
+            ++users;
+            last = k;
+        }
     if (users == 1) {
-        multiply_block(lds, a, kb, last);
-        fft_lds<E, 1>(lds, H, BC, a.twH, H);
-        store_block(lds, a.spec_out + ((size_t)last * (a.W >> 3) + kb) * blk, H);
+        multiply_cols(lds, a, kb, c0, last);
+        fft_lds<FFT_E, 1>(lds, H, cw, a.twH, H);
+        store_cols(lds, a.spec_out + last * chan, kb, c0, cw, H);
         return;
     }
     // Several kernels read this channel: keep its spectrum in registers and
     // rebuild the LDS copy before each kernel's multiply.
-    float2 keep[E];
+    float2 keep[FFT_E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) keep[i] = lds[pad(threadIdx.x + i * blockDim.x)];
+    for (int i = 0; i < FFT_E; ++i) keep[i] = lds[pad(threadIdx.x + i * blockDim.x)];
     bool first = true;
     for (int k = 0; k < a.n_kernels; ++k) {
         if (a.ksrc[k] != ch) continue;
         if (!first) {
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
+            for (int i = 0; i < FFT_E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
             __syncthreads();
         }
         first = false;
-        multiply_block(lds, a, kb, k);
-        fft_lds<E, 1>(lds, H, BC, a.twH, H);
-        store_block(lds, a.spec_out + ((size_t)k * (a.W >> 3) + kb) * blk, H);
+        multiply_cols(lds, a, kb, c0, k);
+        fft_lds<FFT_E, 1>(lds, H, cw, a.twH, H);
+        store_cols(lds, a.spec_out + k * chan, kb, c0, cw, H);
     }
 }
 
@@ -315,7 +350,6 @@ struct RowArgs {
 // Load kernel k's spectrum rows into LDS and turn them into the length-M
 // complex sequence whose inverse FFT interleaves the real output:
 // Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) conj(w^k).
-template <int E>
 __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
     const float4* s4 = reinterpret_cast<const float4*>(a.spec + (size_t)k * M * H);
@@ -354,14 +388,13 @@ __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
     __syncthreads();
 }
 
-template <int E>
-__global__ void __launch_bounds__(1024) k_row_c2r(RowArgs a) {
+__global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     extern __shared__ float2 lds[];
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
-    const int y0 = blockIdx.x * rows;
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * rows;
     const int ch = blockIdx.y;  // destination channel
     const int nt = blockDim.x, tid = threadIdx.x;
-    constexpr int NQ = E / 2;  // float4 (4 output cells) per thread
+    constexpr int NQ = FFT_E / 2;  // float4 (4 output cells) per thread
     float acc[NQ][4];
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
@@ -370,8 +403,8 @@ __global__ void __launch_bounds__(1024) k_row_c2r(RowArgs a) {
     for (int k = 0; k < a.n_kernels; ++k) {
         const KernelParams kp = a.kp[k];
         if (a.conv_out ? (k != a.conv_k) : ((int)kp.dst != ch)) continue;
-        load_c2r<E>(lds, a, k, y0);
-        fft_lds<E, 1>(lds, M, rows, a.twW, W);
+        load_c2r(lds, a, k, y0);
+        fft_lds<FFT_E, 1>(lds, M, rows, a.twW, W);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             int q = tid + i * nt;  // float4 index within the rows*W/4 cells
@@ -411,7 +444,6 @@ __global__ void __launch_bounds__(1024) k_row_c2r(RowArgs a) {
     }
 }
 
-// Dynamic LDS above 64 KiB must be allowed per kernel; done once per kernel.
 // Dynamic LDS above 64 KiB must be allowed per kernel.  Done once per kernel
 // symbol (up to the 160 KiB a gfx950 workgroup can hold), never inside a
 // stream capture.
@@ -424,12 +456,20 @@ void set_lds(K kernel, size_t bytes) {
     done.push_back(key);
 }
 
-// Rows per row-pass workgroup: at least RW, enough to give >= 256 threads.
+// Rows per row-pass workgroup: one row of a wide world, more rows of a
+// narrow one so a workgroup has at least 256 threads (8 elements each).
 int row_rows(int W, int H) {
     int M = W / 2;
-    int rows = RW;
+    int rows = 1;
     while (rows * M < 2048 && rows < H) rows *= 2;
     return rows;
+}
+
+// Columns per column-pass workgroup (1, 2 or 4 of a 4-column block).
+int col_cols(int H) {
+    int cw = 1;
+    while (cw * H < 2048 && cw < BC) cw *= 2;
+    return cw;
 }
 
 }  // namespace
@@ -440,19 +480,10 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
                     const float2* twW) {
     const int M = W / 2, rows = row_rows(W, H);
     const int elems = rows * M;
-    const int E = elems > 8 * 1024 ? 16 : 8;
-    const int threads = elems / E;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "row_r2c");
-    if (E == 8) {
-        set_lds(k_row_r2c<8>, lds);
-        hipLaunchKernelGGL(k_row_r2c<8>, dim3(H / rows, channels), dim3(threads), lds, s, W, H, rows, state, spec,
-                           twW);
-    } else {
-        set_lds(k_row_r2c<16>, lds);
-        hipLaunchKernelGGL(k_row_r2c<16>, dim3(H / rows, channels), dim3(threads), lds, s, W, H, rows, state, spec,
-                           twW);
-    }
+    set_lds(k_row_r2c, lds);
+    hipLaunchKernelGGL(k_row_r2c, dim3(H / rows, channels), dim3(elems / FFT_E), lds, s, W, H, rows, state, spec, twW);
     AH_HIP(hipGetLastError());
 }
 
@@ -462,6 +493,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     ColArgs a;
     a.W = W;
     a.H = H;
+    a.cw = col_cols(H);
     a.spec_in = spec_in;
     a.spec_out = spec_out;
     a.khat = khat;
@@ -469,18 +501,11 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.twH = twH;
     a.n_kernels = n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
-    const int elems = BC * H;
-    const int E = elems > 8 * 1024 ? 16 : 8;
-    const int threads = elems / E;
+    const int elems = a.cw * H;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
-    if (E == 8) {
-        set_lds(k_col_conv<8>, lds);
-        hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 8, channels), dim3(threads), lds, s, a);
-    } else {
-        set_lds(k_col_conv<16>, lds);
-        hipLaunchKernelGGL(k_col_conv<16>, dim3(W / 8, channels), dim3(threads), lds, s, a);
-    }
+    set_lds(k_col_conv, lds);
+    hipLaunchKernelGGL(k_col_conv, dim3(W / 2 / a.cw, channels), dim3(elems / FFT_E), lds, s, a);
     AH_HIP(hipGetLastError());
 }
 
@@ -500,20 +525,12 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
     a.conv_out = conv_out;
     a.conv_k = conv_k;
-    const int M = W / 2;
-    const int elems = a.rows * M;
-    const int E = elems > 8 * 1024 ? 16 : 8;
-    const int threads = elems / E;
+    const int elems = a.rows * (W / 2);
     const size_t lds = padded(elems) * sizeof(float2);
     const int gy = conv_out ? 1 : channels;
     Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
-    if (E == 8) {
-        set_lds(k_row_c2r<8>, lds);
-        hipLaunchKernelGGL(k_row_c2r<8>, dim3(H / a.rows, gy), dim3(threads), lds, s, a);
-    } else {
-        set_lds(k_row_c2r<16>, lds);
-        hipLaunchKernelGGL(k_row_c2r<16>, dim3(H / a.rows, gy), dim3(threads), lds, s, a);
-    }
+    set_lds(k_row_c2r, lds);
+    hipLaunchKernelGGL(k_row_c2r, dim3(H / a.rows, gy), dim3(elems / FFT_E), lds, s, a);
     AH_HIP(hipGetLastError());
 }
 

@@ -116,4 +116,62 @@ class Sim:
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
 
 
-__all__ = ["Sim", "build", "available", "workloads", "_abi"]
+class Batch:
+    """Hundreds of uneven single-channel worlds advanced together (ah_batch_*).
+
+    `worlds` is a list of (LeniaDesc, seed); every desc uses path
+    "direct_bf16".  No counterpart in the reference, which runs one world per
+    Sim; `read_world(i)` returns what `Sim.read_field("state")[0]` would.
+    """
+
+    def __init__(self, worlds):
+        lib = load()
+        self._lib = lib
+        n = len(worlds)
+        descs = (LeniaDesc * n)(*[d for d, _ in worlds])
+        seeds = (ctypes.c_uint32 * n)(*[s & 0xFFFFFFFF for _, s in worlds])
+        h = ctypes.c_void_p()
+        check(lib.ah_batch_new(descs, n, seeds, ctypes.byref(h)))
+        self._h = h
+        self.descs = [d for d, _ in worlds]
+        self.seeds = [s for _, s in worlds]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ah_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, steps: int = 1) -> None:
+        check(self._lib.ah_batch_step(self._h, steps))
+
+    def wait(self) -> None:
+        check(self._lib.ah_batch_wait(self._h))
+
+    def read_world(self, i: int) -> np.ndarray:
+        d = self.descs[i]
+        out = np.empty((d.height, d.width), np.float32)
+        check(self._lib.ah_batch_read_world(self._h, i, out.ctypes.data, out.nbytes))
+        return out
+
+    def write_world(self, i: int, data: np.ndarray) -> None:
+        a = np.ascontiguousarray(data, np.float32)
+        check(self._lib.ah_batch_write_world(self._h, i, a.ctypes.data, a.nbytes))
+
+    def info(self) -> _abi.BatchInfo:
+        i = _abi.BatchInfo()
+        check(self._lib.ah_batch_info_get(self._h, ctypes.byref(i)))
+        return i
+
+    def measure(self, steps: int) -> dict:
+        k, g, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self._lib.ah_batch_measure(self._h, steps, ctypes.byref(k), ctypes.byref(g), ctypes.byref(e)))
+        return {"kernel_ms": k.value, "graph_wall_ms": g.value, "eager_wall_ms": e.value}
+
+
+__all__ = ["Sim", "Batch", "build", "available", "workloads", "_abi"]

@@ -61,6 +61,30 @@ def workload_name(d) -> str:
             f"dt={d.dt:g}, {d.init_count} blobs of {d.init_size}^2")
 
 
+def direct_flops_per_cell(d) -> float:
+    """MFMA flops per cell of the direct path: 16*16*32*2 per MFMA per 256
+    outputs, times the 32-column chunks of every kernel row (|dx| <= chord)."""
+    import numpy as np
+    k = d.kernels[0]
+    R = int(np.ceil(k.radius))
+    chunks = 0
+    for dy in range(-R, R + 1):
+        xs = [dx for dx in range(0, R + 1) if np.hypot(dx, dy) < k.radius]
+        if not xs:
+            continue
+        dxm = max(xs)
+        e = -8 * ((dxm + 7) // 8)
+        chunks += (dxm + 15 - e) // 32 + 1
+    return chunks * 16 * 16 * 32 * 2 / 256.0
+
+
+def useful_flops_per_cell(d) -> float:
+    """2 flops per nonzero tap of the kernel (0 < r < R)."""
+    R = d.kernels[0].radius
+    n = int(R) + 1
+    return 2.0 * sum(1 for y in range(-n, n + 1) for x in range(-n, n + 1) if 0 < (x * x + y * y) ** 0.5 < R)
+
+
 def time_path(path: str, side: int, steps: int, warmup: int, seed: int):
     from auto_amd import Sim, workloads as W
     d = W.rings3_4096(path=path, side=side)
@@ -84,7 +108,33 @@ def time_path(path: str, side: int, steps: int, warmup: int, seed: int):
                         kernels=kernels)
 
 
-def cpu_leg(d, sim, budget_s: float):
+def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
+    """The 384 uneven worlds: aggregate cell-updates/s over graph-replayed
+    steps, autotuner choices, and the launch-overhead share."""
+    from auto_amd import Batch, workloads as W
+    worlds = W.uneven_worlds(seed=seed, n=n, path="direct_bf16")
+    b = Batch(worlds)
+    info = b.info()
+    b.step(graph_steps)  # warm-up: captures the graph
+    b.wait()
+    t0 = time.perf_counter()
+    b.step(2 * graph_steps)
+    b.wait()
+    dt = time.perf_counter() - t0
+    m = b.measure(graph_steps)
+    cells = info.total_cells
+    return {"worlds": n, "cells_per_step": cells, "value": cells * 2 * graph_steps / dt,
+            "unit": "cell-updates/s", "ms_per_step": 1e3 * dt / (2 * graph_steps),
+            "tile_rows": info.tile_rows, "grid": info.grid, "tiles": info.ntiles,
+            "autotune": [{"tile_rows": info.tune[i].tile_rows, "grid": info.tune[i].grid,
+                          "ms_per_step": info.tune[i].ms_per_step} for i in range(info.n_tune)],
+            "graph_steps": graph_steps,
+            "launch_overhead_share_graph": 1.0 - m["kernel_ms"] / m["graph_wall_ms"],
+            "launch_overhead_share_eager": 1.0 - m["kernel_ms"] / m["eager_wall_ms"],
+            "measure": m}
+
+
+def cpu_leg(d, sim, budget_s: float, time_cpu: bool = True):
     """Oracle leg: time the FP32 port of the reference step on host cores and
     check one GPU step against the FP64 oracle step from the same input."""
     import numpy as np
@@ -93,6 +143,12 @@ def cpu_leg(d, sim, budget_s: float):
     O.set_threads(threads)
     s = sim.read_field("state")
     spectra = O.kernel_spectra32(d)
+    if not time_cpu:
+        sim.step(1)
+        got = sim.read_field("state")
+        diff = got.astype(np.float64) - O.step64(d, s, "fft")
+        return None, {"max_abs": float(np.abs(diff).max()),
+                      "rel_l2": float(np.linalg.norm(diff) / max(np.linalg.norm(O.step64(d, s, "fft")), 1e-300))}
     t0 = time.perf_counter()
     n = 0
     x = s
@@ -124,7 +180,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--path", default="fft_fp32", choices=["fft_fp32"])
+    ap.add_argument("--path", default="fft_fp32", choices=["fft_fp32", "direct_bf16"],
+                    help="path whose rate is `value` (both are timed)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 384-world batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     args = ap.parse_args()
@@ -143,38 +201,62 @@ def main() -> int:
     _abi.check(lib.ah_copy_probe(probe_bytes, 20, ctypes.byref(ms)))
     copy_gbps = 2 * probe_bytes / (ms.value * 1e-3) / 1e9
 
-    d, sim, r = time_path(args.path, args.side, args.steps, args.warmup, args.seed)
-    w, h = d.width, d.height
-    kern = r["kernels"]
-    for name, k in kern.items():
-        if name in FFT_KERNELS:
-            b = kernel_bytes(name, w, h)
-            k["bytes"] = b
-            k["GBps"] = b / (k["avg_us"] * 1e-6) / 1e9
-    dom = max((n for n in kern if "bytes" in kern[n]), key=lambda n: kern[n]["avg_us"])
-    step_bytes = sum(kern[n]["bytes"] for n in kern if "bytes" in kern[n])
-    step_us = sum(kern[n]["avg_us"] for n in kern if "bytes" in kern[n])
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBPS, "traffic": None,
-                "bytes_per_launch": kern[dom]["bytes"],
-                "step": {"bytes": step_bytes, "kernel_us": step_us,
-                         "achieved": step_bytes / (step_us * 1e-6) / 1e9,
-                         "frac": step_bytes / (step_us * 1e-6) / 1e9 / HBM_PEAK_GBPS},
-                "copy_probe_GBps": copy_gbps}
+    results = {}
+    sims = {}
+    for path in ("fft_fp32", "direct_bf16"):
+        d, sim, r = time_path(path, args.side, args.steps, args.warmup, args.seed)
+        w, h = d.width, d.height
+        kern = r["kernels"]
+        if path == "fft_fp32":
+            for name, k in kern.items():
+                if name in FFT_KERNELS:
+                    k["bytes"] = kernel_bytes(name, w, h)
+                    k["GBps"] = k["bytes"] / (k["avg_us"] * 1e-6) / 1e9
+            dom = max((n for n in kern if "bytes" in kern[n]), key=lambda n: kern[n]["avg_us"])
+            step_bytes = sum(kern[n]["bytes"] for n in kern if "bytes" in kern[n])
+            step_us = sum(kern[n]["avg_us"] for n in kern if "bytes" in kern[n])
+            roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBPS, "traffic": None,
+                    "bytes_per_launch": kern[dom]["bytes"],
+                    "step": {"bytes": step_bytes, "kernel_us": step_us,
+                             "achieved": step_bytes / (step_us * 1e-6) / 1e9,
+                             "frac": step_bytes / (step_us * 1e-6) / 1e9 / HBM_PEAK_GBPS},
+                    "copy_probe_GBps": copy_gbps}
+        else:
+            name = "direct_bf16_update"
+            fpc = direct_flops_per_cell(d)
+            tflops = fpc * w * h / (kern[name]["avg_us"] * 1e-6) / 1e12
+            kern[name]["flops"] = fpc * w * h
+            kern[name]["TFLOPs"] = tflops
+            roof = {"bound": "mfma", "kernel": name, "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "traffic": None,
+                    "flops_per_cell": fpc, "useful_flops_per_cell": useful_flops_per_cell(d)}
+        results[path] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "kernels": kern, "roofline": roof}
+        sims[path] = (d, sim)
 
+    head = results[args.path]
+    d = sims[args.path][0]
     out = {
         "metric": "Lenia cell-updates/s at 4096^2 (plain FP32 FFT vs BF16 MFMA) at <=1e-3 err",
-        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "value": head["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.path == "fft_fp32" else "bf16",
+        "data": "synthetic",
         "config": {"workload": workload_name(d), "path": args.path, "seed": args.seed},
-        "roofline": roofline,
-        "kernels": kern,
+        "roofline": head["roofline"],
+        "kernels": head["kernels"],
+        "paths": {p: {"value": v["value"], "ms_per_step": v["ms_per_step"], "roofline": v["roofline"]}
+                  for p, v in results.items()},
+        "direct_over_fft": results["direct_bf16"]["value"] / results["fft_fp32"]["value"],
     }
+    if not args.no_batch:
+        out["batch"] = time_batch(args.seed)
     if not args.no_cpu and rank == 0:
-        base, err = cpu_leg(d, sim, args.cpu_seconds)
+        base, err = cpu_leg(*sims["fft_fp32"], args.cpu_seconds)
         out["cpu_baseline"] = base
-        out["err_vs_fp64_one_step"] = err
+        out["err_vs_fp64_one_step"] = {"fft_fp32": err}
+        _, err_d = cpu_leg(*sims["direct_bf16"], 0.0, time_cpu=False)
+        out["err_vs_fp64_one_step"]["direct_bf16"] = err_d
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

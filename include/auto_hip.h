@@ -46,9 +46,19 @@ enum { AH_MAX_CHANNELS = 3, AH_MAX_KERNELS = 9, AH_MAX_RINGS = 3 };
 
 /* How the periodic convolution is computed. */
 typedef enum {
-    AH_PATH_FFT_FP32 = 0,    /* fused three-pass FP32 real FFT                  */
-    AH_PATH_DIRECT_BF16 = 1  /* direct box convolution on BF16 MFMA, FP32 acc.  */
+    AH_PATH_FFT_FP32 = 0,       /* fused three-pass FP32 real FFT                        */
+    AH_PATH_DIRECT_BF16 = 1,    /* direct convolution on BF16 MFMA, FP32 accumulate; the
+                                   engine picks 1 or 3 bf16 terms per kernel (below)      */
+    AH_PATH_DIRECT_BF16_X1 = 2, /* force one term: bf16(K) * bf16(s)                      */
+    AH_PATH_DIRECT_BF16_X3 = 3  /* force three terms: K_hi s_hi + K_hi s_lo + K_lo s_hi,
+                                   with x = x_hi + x_lo split into two bf16 values        */
 } ah_path;
+/*
+ * AH_PATH_DIRECT_BF16 chooses per kernel: one term when the growth slope
+ * times the bf16 error bound stays within 1e-3, i.e.
+ *   1.213 * dt * h / sigma * 0.0078 / sqrt(N_eff) <= 1e-3,  N_eff = 1 / sum(K^2),
+ * else three terms (about FP32 accuracy at three times the MFMA work).
+ */
 
 /* How the state is seeded at creation. */
 typedef enum {

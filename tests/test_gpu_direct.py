@@ -111,3 +111,33 @@ def test_direct_rejects_large_radius():
     with pytest.raises(EngineError) as e:
         Sim(d, 1)
     assert e.value.code == 6
+
+
+@pytest.mark.parametrize("w,h,radius", [(128, 64, 9.0), (512, 256, 26.0)])
+def test_direct_three_term_conv_is_fp32_accurate(O, w, h, radius):
+    """K_hi s_hi + K_hi s_lo + K_lo s_hi on BF16 MFMA reaches ~FP32 accuracy."""
+    d = W.desc(w, h, [dict(radius=radius, beta=[1.0, 0.5], mu=0.2, sigma=0.03)], 0.1, path="direct_bf16_x3",
+               init=INIT_ZERO)
+    sim = Sim(d, 1)
+    s = np.random.default_rng(w).random((1, h, w), dtype=np.float32)
+    sim.write_field("state", s)
+    err = np.abs(sim.read_field("conv0") - O.conv_fft64(d, s, 0)).max()
+    print(f"{w}x{h} R={radius} three-term conv max-abs {err:.3e}")
+    assert err < 5e-6
+
+
+def test_direct_auto_precision_meets_gate_on_high_gain_rule(O):
+    """sigma=0.01, dt=0.2 (growth slope x24): the auto mode must pick three
+    terms and stay within 1e-3 per step; one forced term does not."""
+    kern = [dict(radius=9.0, mu=0.2, sigma=0.01)]
+    errs = {}
+    for path in ("direct_bf16", "direct_bf16_x1"):
+        d = W.desc(256, 256, kern, 0.2, path=path, init=INIT_ZERO)
+        sim = Sim(d, 1)
+        s = np.random.default_rng(5).random((1, 256, 256), dtype=np.float32) * 0.4
+        sim.write_field("state", s)
+        sim.step(1)
+        errs[path] = float(np.abs(sim.read_field("state") - O.step64(d, s, "fft")).max())
+    print(errs)
+    assert errs["direct_bf16"] <= 1e-4
+    assert errs["direct_bf16"] < errs["direct_bf16_x1"]
