@@ -99,12 +99,14 @@ BATCH_MAX_CANDIDATES = 8
 
 
 class BatchTune(ctypes.Structure):
-    _fields_ = [("tile_rows", ctypes.c_uint32), ("grid", ctypes.c_uint32), ("ms_per_step", ctypes.c_double)]
+    _fields_ = [("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("ms_per_step", ctypes.c_double)]
 
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [
         ("n_worlds", ctypes.c_uint32),
+        ("tile_cols", ctypes.c_uint32),
         ("tile_rows", ctypes.c_uint32),
         ("grid", ctypes.c_uint32),
         ("ntiles", ctypes.c_uint32),
@@ -161,14 +163,16 @@ class EngineError(RuntimeError):
 
 
 def load() -> ctypes.CDLL:
-    """Load libauto_hip.so from the package directory; raise if it is absent."""
+    """Load libauto_hip.so from the package directory (or the path in the
+    AUTO_HIP_LIB environment variable); raise if it is absent."""
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    path = os.environ.get("AUTO_HIP_LIB", LIB_PATH)
+    if not os.path.exists(path):
         raise ImportError(
-            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
-    lib = ctypes.CDLL(LIB_PATH)
+            f"{path} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

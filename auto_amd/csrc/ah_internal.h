@@ -139,6 +139,7 @@ struct DirectArgs {
     int W, H;
     int halo_y, halo_x;  // max R over kernels; max R rounded up to a multiple of 8
     int tr, tc, tstride; // LDS tile rows, columns, row stride (bf16 elements)
+    int cols, rows;      // output tile shape of the launch (64 or 128 each)
     int n_kernels;
     DirectKernel k[AH_MAX_KERNELS];
     const int4* sched;   // per kernel, per |dy| in [0, R]: (e, chunks, first fragment, 0)
@@ -160,7 +161,7 @@ struct BatchWorld {  // one world of a batch as the batch kernel reads it
 
 struct BatchArgs {
     const BatchWorld* worlds;
-    const int4* tiles;  // (world, x0, y0, 0); tile = 64 columns x 32*NB rows
+    const int4* tiles;  // (world, x0, y0, 0); tile = cols x rows of the launch shape
     const int* seg;     // workgroup b runs tiles [seg[b], seg[b+1])
     int ntiles;
     float* buf0;
@@ -171,8 +172,7 @@ struct BatchArgs {
     int halo_y, halo_x, tr, tc, tstride, nfrag_max;
 };
 size_t batch_lds_bytes(const BatchArgs& a);
-void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int nb, int grid);
-int direct_rows_per_tile(int H);
+void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int cols, int rows, int grid);
 size_t direct_lds_bytes(const DirectArgs& a);
 void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels);
 

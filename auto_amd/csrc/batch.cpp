@@ -40,7 +40,7 @@ struct ah_batch {
     uint64_t total_cells = 0;
 
     // chosen tiling
-    int nb = 0, grid = 0, ntiles = 0;
+    int cols = 0, rows = 0, grid = 0, ntiles = 0;
     int4* tiles = nullptr;
     int* seg = nullptr;
     ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES]{};
@@ -89,8 +89,8 @@ struct ah_batch {
         a.frags = frags;
         a.halo_y = halo_y;
         a.halo_x = halo_x;
-        a.tr = 32 * nb + 2 * halo_y;
-        a.tc = 64 + 2 * halo_x + 32;
+        a.tr = rows + 2 * halo_y;
+        a.tc = cols + 2 * halo_x + 32;
         a.tstride = a.tc + 8;
         a.nfrag_max = nfrag_max;
         return a;
@@ -106,7 +106,7 @@ void validate_world(const ah_lenia_desc& d, uint32_t i) {
     if (d.path == AH_PATH_FFT_FP32 || d.path > AH_PATH_DIRECT_BF16_X3)
         bad("batched worlds run the direct BF16 path (path must be one of the AH_PATH_DIRECT_BF16 values)");
     if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
-    if (d.width < 64 || d.height < 32 || d.width > 8192 || d.height > 8192) bad("width 64..8192, height 32..8192");
+    if (d.width < 64 || d.height < 64 || d.width > 8192 || d.height > 8192) bad("width and height 64..8192");
     if (d.channels != 1 || d.n_kernels != 1) bad("batched worlds are single-channel with one kernel");
     const ah_kernel& k = d.kernels[0];
     if (!(k.radius > 0.0f) || !std::isfinite(k.radius)) bad("kernel radius must be positive");
@@ -121,15 +121,15 @@ void validate_world(const ah_lenia_desc& d, uint32_t i) {
         bad("patch size must be 1..side");
 }
 
-// Tiles of every world for row-block count nb, and contiguous segments of
-// near-equal cost for `grid` workgroups.
-void build_tiling(ah_batch* b, int nb, int grid, std::vector<int4>& tiles, std::vector<int>& seg) {
+// Tiles (cols x rows) of every world, and contiguous segments of near-equal
+// cost for `grid` workgroups.
+void build_tiling(ah_batch* b, int cols, int rows, int grid, std::vector<int4>& tiles, std::vector<int>& seg) {
     tiles.clear();
     std::vector<double> tcost;
     for (size_t w = 0; w < b->d.size(); ++w) {
         const int W = (int)b->d[w].width, H = (int)b->d[w].height;
-        for (int y0 = 0; y0 < H; y0 += 32 * nb)
-            for (int x0 = 0; x0 < W; x0 += 64) {
+        for (int y0 = 0; y0 < H; y0 += rows)
+            for (int x0 = 0; x0 < W; x0 += cols) {
                 tiles.push_back(make_int4((int)w, x0, y0, 0));
                 tcost.push_back((double)b->cost[w]);
             }
@@ -146,17 +146,18 @@ void build_tiling(ah_batch* b, int nb, int grid, std::vector<int4>& tiles, std::
     }
 }
 
-void install_tiling(ah_batch* b, int nb, int grid) {
+void install_tiling(ah_batch* b, int cols, int rows, int grid) {
     std::vector<int4> tiles;
     std::vector<int> seg;
-    build_tiling(b, nb, grid, tiles, seg);
+    build_tiling(b, cols, rows, grid, tiles, seg);
     if (b->tiles) b->release(b->tiles);
     if (b->seg) b->release(b->seg);
     b->tiles = (int4*)b->alloc(sizeof(int4) * tiles.size());
     b->seg = (int*)b->alloc(sizeof(int) * seg.size());
     AH_HIP(hipMemcpy(b->tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
     AH_HIP(hipMemcpy(b->seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
-    b->nb = nb;
+    b->cols = cols;
+    b->rows = rows;
     b->grid = grid;
     b->ntiles = (int)tiles.size();
     for (auto& g : b->graph)
@@ -167,14 +168,17 @@ void install_tiling(ah_batch* b, int nb, int grid) {
 }
 
 void launch_step(ah_batch* b, Profiler* p) {
-    launch_direct_batch(b->stream, p, b->args(b->parity), b->nb, b->grid);
+    launch_direct_batch(b->stream, p, b->args(b->parity), b->cols, b->rows, b->grid);
     b->parity ^= 1;
 }
 
-// Times each admissible (rows per tile, grid) on the batch and keeps the fastest.
+// Times each admissible (tile shape, grid) on the batch and keeps the fastest.
 void autotune(ah_batch* b) {
-    int min_h = 1 << 30;
-    for (auto& d : b->d) min_h = std::min(min_h, (int)d.height);
+    int min_h = 1 << 30, min_w = 1 << 30;
+    for (auto& d : b->d) {
+        min_h = std::min(min_h, (int)d.height);
+        min_w = std::min(min_w, (int)d.width);
+    }
     hipDeviceProp_t prop;
     AH_HIP(hipGetDeviceProperties(&prop, b->device));
     const int cus = prop.multiProcessorCount;
@@ -182,36 +186,39 @@ void autotune(ah_batch* b) {
     AH_HIP(hipEventCreate(&e0));
     AH_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    int best_nb = 0, best_grid = 0;
+    int best_c = 0, best_r = 0, best_grid = 0;
     b->n_tune = 0;
     try {
-        for (int nb : {1, 2, 4}) {
-            if (32 * nb > min_h) continue;
-            b->nb = nb;
-            const size_t lds = batch_lds_bytes(b->args(0));
-            if (lds > 160 * 1024) continue;
-            const int per_cu = std::max(1, (int)((160 * 1024) / lds));
-            for (int k = 1; k <= std::min(per_cu, 2); ++k) {
-                const int grid = cus * k;
-                install_tiling(b, nb, grid);
-                launch_step(b, nullptr);  // warm-up
-                const int reps = 4;
-                AH_HIP(hipEventRecord(e0, b->stream));
-                for (int r = 0; r < reps; ++r) launch_step(b, nullptr);
-                AH_HIP(hipEventRecord(e1, b->stream));
-                AH_HIP(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                AH_HIP(hipEventElapsedTime(&ms, e0, e1));
-                const double per = ms / reps;
-                if (b->n_tune < AH_BATCH_MAX_CANDIDATES)
-                    b->tune[b->n_tune++] = ah_batch_tune{(uint32_t)(32 * nb), (uint32_t)grid, per};
-                if (per < best) {
-                    best = per;
-                    best_nb = nb;
-                    best_grid = grid;
+        for (int cols : {128, 64})
+            for (int rows : {128, 64}) {
+                if (cols > min_w || rows > min_h) continue;
+                b->cols = cols;
+                b->rows = rows;
+                const size_t lds = batch_lds_bytes(b->args(0));
+                if (lds > 160 * 1024) continue;
+                const int per_cu = std::max(1, (int)((160 * 1024) / lds));
+                for (int k = 1; k <= std::min(per_cu, 2); ++k) {
+                    const int grid = cus * k;
+                    install_tiling(b, cols, rows, grid);
+                    launch_step(b, nullptr);  // warm-up
+                    const int reps = 4;
+                    AH_HIP(hipEventRecord(e0, b->stream));
+                    for (int r = 0; r < reps; ++r) launch_step(b, nullptr);
+                    AH_HIP(hipEventRecord(e1, b->stream));
+                    AH_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    AH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    const double per = ms / reps;
+                    if (b->n_tune < AH_BATCH_MAX_CANDIDATES)
+                        b->tune[b->n_tune++] = ah_batch_tune{(uint32_t)cols, (uint32_t)rows, (uint32_t)grid, per};
+                    if (per < best) {
+                        best = per;
+                        best_c = cols;
+                        best_r = rows;
+                        best_grid = grid;
+                    }
                 }
             }
-        }
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -219,8 +226,8 @@ void autotune(ah_batch* b) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!best_nb) throw Error{AH_ERR_UNSUPPORTED, "no tile shape fits the batch in LDS"};
-    install_tiling(b, best_nb, best_grid);
+    if (!best_c) throw Error{AH_ERR_UNSUPPORTED, "no tile shape fits the batch in LDS"};
+    install_tiling(b, best_c, best_r, best_grid);
     b->parity = 0;
 }
 
@@ -398,7 +405,8 @@ ah_status ah_batch_info_get(const ah_batch* b, ah_batch_info* out) {
         if (!b || !out) throw Error{AH_ERR_INVALID, "null argument"};
         ah_batch_info i{};
         i.n_worlds = (uint32_t)b->d.size();
-        i.tile_rows = (uint32_t)(32 * b->nb);
+        i.tile_cols = (uint32_t)b->cols;
+        i.tile_rows = (uint32_t)b->rows;
         i.grid = (uint32_t)b->grid;
         i.ntiles = (uint32_t)b->ntiles;
         i.graph_steps = b->graph_steps;

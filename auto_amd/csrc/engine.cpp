@@ -238,8 +238,8 @@ void validate(const ah_lenia_desc& d) {
     if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
         throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
     if (d.path != AH_PATH_FFT_FP32) {
-        if (d.width < 64 || d.height < 32)
-            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path needs width >= 64 and height >= 32"};
+        if (d.width < 64 || d.height < 64)
+            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path needs width >= 64 and height >= 64"};
         for (uint32_t k = 0; k < d.n_kernels; ++k)
             if (std::ceil((double)d.kernels[k].radius) > 31)
                 throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path supports kernel radius <= 31"};
@@ -270,9 +270,14 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     for (int k = 0; k < K; ++k) rmax = std::max(rmax, s->reach[k]);
     a.halo_y = rmax;
     a.halo_x = 8 * ((rmax + 7) / 8);
-    a.tr = direct_rows_per_tile(a.H) + 2 * a.halo_y;
-    a.tc = 64 + 2 * a.halo_x + 32;
-    a.tstride = a.tc + 8;
+    // Output tile: the largest of 128x128, 128x64, 64x64 that the world and LDS allow.
+    auto shape = [&](int cols, int rows) {
+        a.cols = cols;
+        a.rows = rows;
+        a.tr = rows + 2 * a.halo_y;
+        a.tc = cols + 2 * a.halo_x + 32;
+        a.tstride = a.tc + 8;
+    };
     a.n_kernels = K;
     std::vector<int> sched;
     std::vector<uint16_t> frags;  // [fragment][64][8]
@@ -295,8 +300,16 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         nfrag_total += dk.nfrag * (dk.terms == 3 ? 2 : 1);
         a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
     }
-    if (direct_lds_bytes(a) > 160 * 1024)
-        throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
+    bool fits = false;
+    for (auto cr : {std::make_pair(128, 128), std::make_pair(128, 64), std::make_pair(64, 64)}) {
+        if (cr.first > (int)d.width || cr.second > (int)d.height) continue;
+        shape(cr.first, cr.second);
+        if (direct_lds_bytes(a) <= 160 * 1024) {
+            fits = true;
+            break;
+        }
+    }
+    if (!fits) throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
     int4* dsched = (int4*)s->alloc(sizeof(int) * sched.size());
     uint4* dfrags = (uint4*)s->alloc(sizeof(uint16_t) * frags.size());
     AH_HIP(hipMemcpy(dsched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));

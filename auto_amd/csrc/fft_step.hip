@@ -175,12 +175,12 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-constexpr int FFT_E = 8;          // complex elements per thread in every pass
 constexpr int FFT_THREADS = 512;  // upper bound on the workgroup size
 
 // ------------------------------------------------------------ pass 1
 // Grid: (H/rows, channels); `rows` consecutive rows per workgroup (one row
 // when W >= 4096), rows*M/8 threads.
+template <int E>
 __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows, const float* __restrict__ state,
                                                          float2* __restrict__ spec, const float2* __restrict__ twW) {
     extern __shared__ float2 lds[];
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
         lds[pad(2 * i + 1)] = make_float2(v.z, v.w);
     }
     __syncthreads();
-    fft_lds<FFT_E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
+    fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
     // Even/odd split: X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[M-k])/2,
     // O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output float4 q
     // holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
@@ -290,6 +290,7 @@ __device__ __forceinline__ void multiply_cols(float2* lds, const ColArgs& a, int
     __syncthreads();
 }
 
+template <int E>
 __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     extern __shared__ float2 lds[];
     const int H = a.H, cw = a.cw;
@@ -299,7 +300,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     const size_t chan = (size_t)(a.W >> 1) * H;  // float2 per channel / kernel spectrum
     load_cols(lds, a.spec_in + ch * chan, kb, c0, cw, H);
     __syncthreads();
-    fft_lds<FFT_E, -1>(lds, H, cw, a.twH, H);
+    fft_lds<E, -1>(lds, H, cw, a.twH, H);
     int users = 0, last = -1;
     for (int k = 0; k < a.n_kernels; ++k)
         if (a.ksrc[k] == ch) {
@@ -308,27 +309,27 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
         }
     if (users == 1) {
         multiply_cols(lds, a, kb, c0, last);
-        fft_lds<FFT_E, 1>(lds, H, cw, a.twH, H);
+        fft_lds<E, 1>(lds, H, cw, a.twH, H);
         store_cols(lds, a.spec_out + last * chan, kb, c0, cw, H);
         return;
     }
     // Several kernels read this channel: keep its spectrum in registers and
     // rebuild the LDS copy before each kernel's multiply.
-    float2 keep[FFT_E];
+    float2 keep[E];
 #pragma unroll
-    for (int i = 0; i < FFT_E; ++i) keep[i] = lds[pad(threadIdx.x + i * blockDim.x)];
+    for (int i = 0; i < E; ++i) keep[i] = lds[pad(threadIdx.x + i * blockDim.x)];
     bool first = true;
     for (int k = 0; k < a.n_kernels; ++k) {
         if (a.ksrc[k] != ch) continue;
         if (!first) {
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < FFT_E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
+            for (int i = 0; i < E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
             __syncthreads();
         }
         first = false;
         multiply_cols(lds, a, kb, c0, k);
-        fft_lds<FFT_E, 1>(lds, H, cw, a.twH, H);
+        fft_lds<E, 1>(lds, H, cw, a.twH, H);
         store_cols(lds, a.spec_out + k * chan, kb, c0, cw, H);
     }
 }
@@ -388,13 +389,14 @@ __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
     __syncthreads();
 }
 
+template <int E>
 __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     extern __shared__ float2 lds[];
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * rows;
     const int ch = blockIdx.y;  // destination channel
     const int nt = blockDim.x, tid = threadIdx.x;
-    constexpr int NQ = FFT_E / 2;  // float4 (4 output cells) per thread
+    constexpr int NQ = E / 2;  // float4 (4 output cells) per thread
     float acc[NQ][4];
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
@@ -404,7 +406,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
         const KernelParams kp = a.kp[k];
         if (a.conv_out ? (k != a.conv_k) : ((int)kp.dst != ch)) continue;
         load_c2r(lds, a, k, y0);
-        fft_lds<FFT_E, 1>(lds, M, rows, a.twW, W);
+        fft_lds<E, 1>(lds, M, rows, a.twW, W);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             int q = tid + i * nt;  // float4 index within the rows*W/4 cells
@@ -476,14 +478,23 @@ int col_cols(int H) {
 
 size_t fft_lds_limit() { return 160 * 1024; }
 
+// Complex elements per thread of each pass (8 or 16).
+int row_E(int W, int H) { return 8; }
+int col_E(int H) { return 8; }
+
 void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, const float* state, float2* spec,
                     const float2* twW) {
-    const int M = W / 2, rows = row_rows(W, H);
+    const int M = W / 2, rows = row_rows(W, H), E = row_E(W, H);
     const int elems = rows * M;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "row_r2c");
-    set_lds(k_row_r2c, lds);
-    hipLaunchKernelGGL(k_row_r2c, dim3(H / rows, channels), dim3(elems / FFT_E), lds, s, W, H, rows, state, spec, twW);
+    if (E == 16) {
+        set_lds(k_row_r2c<16>, lds);
+        hipLaunchKernelGGL(k_row_r2c<16>, dim3(H / rows, channels), dim3(elems / E), lds, s, W, H, rows, state, spec, twW);
+    } else {
+        set_lds(k_row_r2c<8>, lds);
+        hipLaunchKernelGGL(k_row_r2c<8>, dim3(H / rows, channels), dim3(elems / E), lds, s, W, H, rows, state, spec, twW);
+    }
     AH_HIP(hipGetLastError());
 }
 
@@ -501,11 +512,17 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.twH = twH;
     a.n_kernels = n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
+    const int E = col_E(H);
     const int elems = a.cw * H;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
-    set_lds(k_col_conv, lds);
-    hipLaunchKernelGGL(k_col_conv, dim3(W / 2 / a.cw, channels), dim3(elems / FFT_E), lds, s, a);
+    if (E == 16) {
+        set_lds(k_col_conv<16>, lds);
+        hipLaunchKernelGGL(k_col_conv<16>, dim3(W / 2 / a.cw, channels), dim3(elems / E), lds, s, a);
+    } else {
+        set_lds(k_col_conv<8>, lds);
+        hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / E), lds, s, a);
+    }
     AH_HIP(hipGetLastError());
 }
 
@@ -525,12 +542,18 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
     a.conv_out = conv_out;
     a.conv_k = conv_k;
+    const int E = row_E(W, H);
     const int elems = a.rows * (W / 2);
     const size_t lds = padded(elems) * sizeof(float2);
     const int gy = conv_out ? 1 : channels;
     Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
-    set_lds(k_row_c2r, lds);
-    hipLaunchKernelGGL(k_row_c2r, dim3(H / a.rows, gy), dim3(elems / FFT_E), lds, s, a);
+    if (E == 16) {
+        set_lds(k_row_c2r<16>, lds);
+        hipLaunchKernelGGL(k_row_c2r<16>, dim3(H / a.rows, gy), dim3(elems / E), lds, s, a);
+    } else {
+        set_lds(k_row_c2r<8>, lds);
+        hipLaunchKernelGGL(k_row_c2r<8>, dim3(H / a.rows, gy), dim3(elems / E), lds, s, a);
+    }
     AH_HIP(hipGetLastError());
 }
 

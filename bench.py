@@ -125,8 +125,8 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
     cells = info.total_cells
     return {"worlds": n, "cells_per_step": cells, "value": cells * 2 * graph_steps / dt,
             "unit": "cell-updates/s", "ms_per_step": 1e3 * dt / (2 * graph_steps),
-            "tile_rows": info.tile_rows, "grid": info.grid, "tiles": info.ntiles,
-            "autotune": [{"tile_rows": info.tune[i].tile_rows, "grid": info.tune[i].grid,
+            "tile": [info.tile_cols, info.tile_rows], "grid": info.grid, "tiles": info.ntiles,
+            "autotune": [{"tile": [info.tune[i].tile_cols, info.tune[i].tile_rows], "grid": info.tune[i].grid,
                           "ms_per_step": info.tune[i].ms_per_step} for i in range(info.n_tune)],
             "graph_steps": graph_steps,
             "launch_overhead_share_graph": 1.0 - m["kernel_ms"] / m["graph_wall_ms"],

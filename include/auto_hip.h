@@ -44,7 +44,8 @@ typedef enum {
 
 enum { AH_MAX_CHANNELS = 3, AH_MAX_KERNELS = 9, AH_MAX_RINGS = 3 };
 
-/* How the periodic convolution is computed. */
+/* How the periodic convolution is computed.  The direct paths need width
+ * and height >= 64 and kernel radius <= 31. */
 typedef enum {
     AH_PATH_FFT_FP32 = 0,       /* fused three-pass FP32 real FFT                        */
     AH_PATH_DIRECT_BF16 = 1,    /* direct convolution on BF16 MFMA, FP32 accumulate; the
@@ -171,7 +172,7 @@ ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacit
 /*
  * Batched uneven worlds.  No counterpart in the reference, which runs one
  * world per Sim (sim.rs:202-309).  Every world is single-channel with one
- * kernel (radius <= 31), width >= 64, height >= 32, path AH_PATH_DIRECT_BF16,
+ * kernel (radius <= 31), width and height >= 64, a direct BF16 path,
  * and an ORBIUM, PATCH or ZERO init seeded with its own seed.  Worlds differ in
  * size, rule and dt.  All worlds advance together in one persistent launch per
  * step; ah_batch_step replays hipGraph-captured chunks of `graph_steps`
@@ -184,14 +185,15 @@ typedef struct ah_batch ah_batch;
 enum { AH_BATCH_MAX_CANDIDATES = 8 };
 
 typedef struct {
-    uint32_t tile_rows;  /* output rows per tile (tiles are 64 columns wide) */
+    uint32_t tile_cols;  /* output columns per tile (64 or 128)               */
+    uint32_t tile_rows;  /* output rows per tile (64 or 128)                  */
     uint32_t grid;       /* persistent workgroups                             */
     double ms_per_step;  /* measured by the autotuner                         */
 } ah_batch_tune;
 
 typedef struct {
     uint32_t n_worlds;
-    uint32_t tile_rows, grid, ntiles, graph_steps;
+    uint32_t tile_cols, tile_rows, grid, ntiles, graph_steps;
     uint64_t total_cells;  /* cell-updates per step */
     uint64_t total_bytes;
     uint64_t generation;

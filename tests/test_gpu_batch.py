@@ -35,7 +35,7 @@ def test_batch_worlds_match_single_world_step(O):
         worst_plain = max(worst_plain, float(np.abs(got - ref).max()))
         worst64 = max(worst64, float(np.abs(got - O.step64(d, before[i][None], "fft")[0]).max()))
     print(f"batch of {len(worlds)}: worst max-abs vs plain FFT step {worst_plain:.3e}, vs FP64 {worst64:.3e}; "
-          f"tiles {info.ntiles} rows {info.tile_rows} grid {info.grid}")
+          f"tiles {info.ntiles} shape {info.tile_cols}x{info.tile_rows} grid {info.grid}")
     assert worst_plain <= 1e-3 and worst64 <= 1e-3
 
 

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("w,h,radius,beta", [
-    (64, 32, 9.0, [1.0]),
+    (64, 64, 9.0, [1.0]),
     (128, 64, 13.0, [1.0]),
     (256, 256, 26.0, [1.0, 2 / 3, 1 / 3]),
     (512, 128, 31.0, [0.5, 1.0]),
