@@ -11,6 +11,6 @@ rule/step interface on top of it:
 There is no CPU fallback: creating a Sim without the built library raises.
 """
 from . import workloads
-from .sim import Sim, Batch, build, available
+from .sim import Sim, Batch, Surrogate, build, available
 
-__all__ = ["workloads", "Sim", "Batch", "build", "available"]
+__all__ = ["workloads", "Sim", "Batch", "Surrogate", "build", "available"]

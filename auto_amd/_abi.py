@@ -119,6 +119,22 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+SURROGATE_PARAMS = 104129
+
+
+class SurrogateReport(ctypes.Structure):
+    _fields_ = [
+        ("lenia_steps", ctypes.c_uint64),
+        ("surrogate_calls", ctypes.c_uint64),
+        ("plain_steps", ctypes.c_uint64),
+        ("judges", ctypes.c_uint64),
+        ("fallbacks", ctypes.c_uint64),
+        ("max_judge_rel_l2", ctypes.c_double),
+        ("last_judge_rel_l2", ctypes.c_double),
+        ("ms", ctypes.c_double),
+    ]
+
+
 LIB_NAME = "libauto_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
@@ -149,6 +165,21 @@ _SIGNATURES = {
     "ah_batch_measure": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "ah_batch_destroy": (None, [_P]),
+    "ah_surrogate_new": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_P)]),
+    "ah_surrogate_destroy": (None, [_P]),
+    "ah_surrogate_get_params": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "ah_surrogate_set_params": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "ah_surrogate_forward": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P]),
+    "ah_surrogate_loss_grad": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.POINTER(ctypes.c_double), _P]),
+    "ah_surrogate_collect": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32]),
+    "ah_surrogate_dataset": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, _P, _P]),
+    "ah_surrogate_train": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                          ctypes.c_uint32, _P]),
+    "ah_surrogate_eval": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "ah_surrogate_advance": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                            ctypes.POINTER(SurrogateReport)]),
 }
 
 _lib = None

@@ -178,4 +178,11 @@ void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels
 
 void launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n4);
 
+// Access to a sim for the surrogate (engine.cpp).
+float* sim_state(ah_sim* s);
+hipStream_t sim_stream(ah_sim* s);
+const ah_lenia_desc& sim_desc(const ah_sim* s);
+void sim_advance(ah_sim* s, uint32_t steps);  // plain steps on the sim's own path
+void sim_add_generations(ah_sim* s, uint64_t n);
+
 }  // namespace ah

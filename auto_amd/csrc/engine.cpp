@@ -446,6 +446,17 @@ void step_once(ah_sim* s, float* conv_out, int conv_k) {
 
 }  // namespace
 
+namespace ah {
+float* sim_state(ah_sim* s) { return s->state; }
+hipStream_t sim_stream(ah_sim* s) { return s->stream; }
+const ah_lenia_desc& sim_desc(const ah_sim* s) { return s->d; }
+void sim_advance(ah_sim* s, uint32_t steps) {
+    for (uint32_t i = 0; i < steps; ++i) step_once(s, nullptr, 0);
+    s->generation += steps;
+}
+void sim_add_generations(ah_sim* s, uint64_t n) { s->generation += n; }
+}  // namespace ah
+
 extern "C" {
 
 const char* ah_last_error(void) { return ah::g_error.c_str(); }

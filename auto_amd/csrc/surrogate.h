@@ -1,0 +1,79 @@
+// Surrogate network: shapes, parameter layout and kernel launchers.
+#pragma once
+
+#include "ah_internal.h"
+
+namespace ah {
+namespace surr {
+
+constexpr int C1 = 32, C2 = 64;
+// Flat FP32 parameter vector: W1 [32][25][1], b1 [32], W2 [64][25][32], b2 [64],
+// W3 [32][25][64], b3 [32], W4 [1][25][32], b4 [1].
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + C1 * 25;
+constexpr int OFF_W2 = OFF_B1 + C1;
+constexpr int OFF_B2 = OFF_W2 + C2 * 25 * C1;
+constexpr int OFF_W3 = OFF_B2 + C2;
+constexpr int OFF_B3 = OFF_W3 + C1 * 25 * C2;
+constexpr int OFF_W4 = OFF_B3 + C1;
+constexpr int OFF_B4 = OFF_W4 + 25 * C1;
+constexpr int N_PARAMS = OFF_B4 + 1;  // 104129
+
+enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2 };
+
+struct ConvArgs {
+    const uint16_t* X;   // bf16 NHWC input, Cin channels (half resolution when up)
+    int N, H, W;         // output size (full size of the input view)
+    int Cin, up, ks;     // Cin multiple of 32; ks 1 or 5
+    const uint16_t* Wt;  // bf16 [Cout][ks*ks][Cin]
+    int Cout;            // 32 or 64
+    const float* bias;   // [Cout] or null
+    int mode;            // ConvMode
+    float* z;            // fp32 [N][H][W][Cout] or null
+    uint16_t* a;         // bf16 [N][H][W][Cout]: gelu(z) (+ add)
+    const uint16_t* add; // bf16 [N][H][W][Cout] for CONV_GELU_ADD
+};
+
+struct WGradArgs {
+    const float* dz_f;     // fp32 [pix][Cout] (or [pix] when Cout == 1)
+    const uint16_t* dz_h;  // bf16 [pix][Cout] when dz_f is null
+    const uint16_t* X;     // bf16 NHWC layer input, channel stride xstride
+    int xstride, xoff;     // channel ci of pixel p at X[p*xstride + xoff + ci]
+    int N, H, W, Cin, Cout, up;
+};
+
+struct PackedWeights {
+    uint16_t* W1x;  // [32][1][32]
+    uint16_t* W2;   // [64][25][32]
+    uint16_t* W2T;  // [32][25][64]
+    uint16_t* W3;   // [32][25][64]
+    uint16_t* W3T;  // [64][25][32]
+    float* W4;      // [25][32]
+};
+
+void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, int H, int W);
+void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name);
+void pool2(hipStream_t s, Profiler* p, const uint16_t* a, uint16_t* out, int N, int H, int W, int C);
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const float* W4, const float* b4, const float* x,
+               float* r, float* y, int N, int H, int W);
+int loss_blocks(size_t total);
+void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const float* x, const float* r, float* dr,
+          double* part, size_t total);
+void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W);
+void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total);
+void dz2(hipStream_t s, Profiler* p, const float* du, const float* z2, uint16_t* out, int N, int H, int W, int C);
+void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
+         int C);
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out);
+void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
+           int chunks, float* out);
+void adam(hipStream_t s, Profiler* p, float* theta, float* m, float* v, const float* g, int n, float lr, float b1,
+          float b2, float eps, float c1, float c2);
+void pack_weights(hipStream_t s, Profiler* p, const float* theta, const PackedWeights& w);
+void crop(hipStream_t s, const float* state, int W, int H, const int2* corners, int count, int side, float* dst);
+void gather(hipStream_t s, const float* src, const int* index, int count, size_t n, float* dst);
+int rel_l2_blocks(size_t n);
+void rel_l2(hipStream_t s, const float* a, const float* b, size_t n, double* part);
+
+}  // namespace surr
+}  // namespace ah

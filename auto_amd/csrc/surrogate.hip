@@ -1,0 +1,543 @@
+// Learned k-step surrogate: kernels.
+//
+// The network (no counterpart in the reference) maps a 1-channel state to
+// the state k steps ahead:
+//   z1 = conv5(x; 1->32)+b1, a1 = gelu(z1);  p = avgpool2(a1)
+//   z2 = conv5(p; 32->64)+b2, a2 = gelu(z2);  u = upsample2(a2)
+//   z3 = conv5(u; 64->32)+b3, s = gelu(z3) + a1
+//   r  = conv5(s; 32->1)+b4,  y = clip(x + r, 0, 1)
+// conv5 = 5x5 cross-correlation with circular padding, gelu = z*Phi(z).
+//
+// Layout: activations are NHWC, bf16 where a convolution reads them (a1, p,
+// a2, s, dz3, dz2 and the first layer's im2col), FP32 where only elementwise
+// code reads them (pre-activations z and the backward input gradients).
+//
+// Hot kernels run on MFMA as an implicit GEMM, D[co][pixel] = sum over
+// (tap, ci) of W[co][tap][ci] * X[pixel+tap][ci], with
+// v_mfma_f32_16x16x32_bf16: A = 16 output channels x 32 input channels of one
+// tap (one 16-byte load per lane from [co][tap][ci] weights), B = 32 input
+// channels x 16 consecutive pixels (one 16-byte load per lane from NHWC).
+// The upsampled input of layer 3 is never stored: the kernel reads the
+// half-resolution tensor at (y/2, x/2).  The backward input gradients of
+// layers 2 and 3 are the same kernel over tap-flipped, channel-swapped
+// weights.  The 1-channel output layer and the weight gradients (reductions
+// over pixels) are VALU kernels with fixed-order partial sums.
+#include <algorithm>
+#include <cmath>
+
+#include "ah_internal.h"
+#include "surrogate.h"
+
+namespace ah {
+namespace surr {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.3989422804014327f;
+}
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// ---------------------------------------------------------------- im2col
+// x fp32 [N][H][W] -> X1 bf16 [N][H][W][32]: channel t < 25 holds x at tap t.
+__global__ void k_im2col1(const float* __restrict__ x, uint16_t* __restrict__ X1, int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * 32;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i & 31);
+    const size_t pix = i >> 5;
+    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
+    uint16_t v = 0;
+    if (c < 25) {
+        const int dy = c / 5 - 2, dx = c % 5 - 2;
+        v = (uint16_t)bf16_bits(x[((size_t)n * H + wrap(yy + dy, H)) * W + wrap(xx + dx, W)]);
+    }
+    X1[i] = v;
+}
+
+// ---------------------------------------------------------------- MFMA conv
+// Wave: one output row, 64 consecutive pixels (4 blocks of 16), MT blocks of
+// 16 output channels.  Workgroup: 4 waves = 4 rows.  Grid (W/64, H/4, N).
+template <int MT>
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int n = blockIdx.z, y = blockIdx.y * 4 + wave, x0 = blockIdx.x * 64;
+    const int H = a.H, W = a.W, Cin = a.Cin, taps = a.ks * a.ks, half = a.ks / 2;
+    const int Hx = a.up ? H / 2 : H, Wx = a.up ? W / 2 : W;
+    const uint16_t* X = a.X + (size_t)n * Hx * Wx * Cin;
+    f32x4 acc[MT][4];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[m][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < taps; ++t) {
+        const int dy = t / a.ks - half, dx = t % a.ks - half;
+        int yy = wrap(y + dy, H);
+        if (a.up) yy >>= 1;
+        int px[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int xx = wrap(x0 + 16 * b + l16 + dx, W);
+            if (a.up) xx >>= 1;
+            px[b] = (yy * Wx + xx) * Cin;
+        }
+        for (int c0 = 0; c0 < Cin; c0 += 32) {
+            bf16x8 A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                A[m] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(a.Wt + ((size_t)(16 * m + l16) * taps + t) * Cin + c0 + 8 * q));
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(X + px[b] + c0 + 8 * q));
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][b], 0, 0, 0);
+            }
+        }
+    }
+    // Lane holds channels 16m + 4q .. +3 of pixel x0 + 16b + l16.
+    const int Cout = a.Cout;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int co = 16 * m + 4 * q;
+            const size_t o = (((size_t)n * H + y) * W + x0 + 16 * b + l16) * Cout + co;
+            float z[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z[r] = acc[m][b][r] + (a.bias ? a.bias[co + r] : 0.0f);
+            if (a.z) *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
+            if (a.mode == CONV_GELU || a.mode == CONV_GELU_ADD) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = gelu(z[r]);
+                if (a.mode == CONV_GELU_ADD) {
+                    const uint2 ad = *reinterpret_cast<const uint2*>(a.add + o);
+                    v[0] += __uint_as_float(ad.x << 16);
+                    v[1] += __uint_as_float(ad.x & 0xffff0000u);
+                    v[2] += __uint_as_float(ad.y << 16);
+                    v[3] += __uint_as_float(ad.y & 0xffff0000u);
+                }
+                *reinterpret_cast<uint2*>(a.a + o) =
+                    make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+            }
+        }
+}
+
+// ---------------------------------------------------------------- pooling
+// a1 bf16 [N][H][W][32] -> p bf16 [N][H/2][W/2][32], mean of the 2x2 block.
+__global__ void k_pool2(const uint16_t* __restrict__ a, uint16_t* __restrict__ p, int N, int H, int W, int C) {
+    const int Hh = H / 2, Wh = W / 2;
+    const size_t total = (size_t)N * Hh * Wh * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t pix = i / C;
+    const int xh = (int)(pix % Wh), yh = (int)((pix / Wh) % Hh), n = (int)(pix / ((size_t)Wh * Hh));
+    const size_t b = (((size_t)n * H + 2 * yh) * W + 2 * xh) * C + c;
+    const float s = bf16f(a[b]) + bf16f(a[b + C]) + bf16f(a[b + (size_t)W * C]) + bf16f(a[b + (size_t)W * C + C]);
+    p[i] = (uint16_t)bf16_bits(0.25f * s);
+}
+
+// ---------------------------------------------------------------- layer 4
+// r = conv5(s; W4 32->1) + b4, y = clip(x + r, 0, 1).  W4 fp32 [25][32]
+// (already bf16-rounded values).  One thread per pixel.
+__global__ void k_out_layer(const uint16_t* __restrict__ s, const float* __restrict__ W4, const float* __restrict__ b4p,
+                            const float* __restrict__ x, float* __restrict__ r, float* __restrict__ y, int N, int H,
+                            int W) {
+    __shared__ float w[25 * 32];
+    for (int i = threadIdx.x; i < 25 * 32; i += blockDim.x) w[i] = W4[i];
+    __syncthreads();
+    const size_t total = (size_t)N * H * W;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int xx = (int)(i % W), yy = (int)((i / W) % H), n = (int)(i / ((size_t)W * H));
+    float acc = *b4p;
+    for (int t = 0; t < 25; ++t) {
+        const int dy = t / 5 - 2, dx = t % 5 - 2;
+        const uint4* sp = reinterpret_cast<const uint4*>(
+            s + (((size_t)n * H + wrap(yy + dy, H)) * W + wrap(xx + dx, W)) * 32);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const uint4 v = sp[g];
+            const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc += w[t * 32 + 8 * g + 2 * j] * __uint_as_float(e[j] << 16);
+                acc += w[t * 32 + 8 * g + 2 * j + 1] * __uint_as_float(e[j] & 0xffff0000u);
+            }
+        }
+    }
+    r[i] = acc;
+    y[i] = fminf(fmaxf(x[i] + acc, 0.0f), 1.0f);
+}
+
+// MSE loss: dr = 2 (y - t) / count where 0 < x + r < 1, else 0; per-block
+// FP64 partial sums of (y - t)^2.
+__global__ void __launch_bounds__(256) k_loss(const float* __restrict__ y, const float* __restrict__ tgt,
+                                              const float* __restrict__ x, const float* __restrict__ r,
+                                              float* __restrict__ dr, double* __restrict__ part, size_t total) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const float scale = 2.0f / (float)total;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const float d = y[i] - tgt[i];
+        acc += (double)d * (double)d;
+        if (dr) {
+            const float pre = x[i] + r[i];
+            dr[i] = (pre > 0.0f && pre < 1.0f) ? scale * d : 0.0f;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// ds = d(loss)/ds for layer 4: ds[p][ci] = sum_t W4[t][ci] dr[p - tap_t].
+__global__ void k_out_input_grad(const float* __restrict__ dr, const float* __restrict__ W4, float* __restrict__ ds,
+                                 int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * 32;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int ci = (int)(i & 31);
+    const size_t pix = i >> 5;
+    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
+    float acc = 0.0f;
+    for (int t = 0; t < 25; ++t) {
+        const int dy = t / 5 - 2, dx = t % 5 - 2;
+        acc += W4[t * 32 + ci] * dr[((size_t)n * H + wrap(yy - dy, H)) * W + wrap(xx - dx, W)];
+    }
+    ds[i] = acc;
+}
+
+// dz3 = bf16(ds * gelu'(z3))
+__global__ void k_dz3(const float* __restrict__ ds, const float* __restrict__ z3, uint16_t* __restrict__ dz3,
+                      size_t total) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) dz3[i] = (uint16_t)bf16_bits(ds[i] * gelu_grad(z3[i]));
+}
+
+// dz2 = bf16((sum of du over the 2x2 children) * gelu'(z2)), half resolution, C = 64.
+__global__ void k_dz2(const float* __restrict__ du, const float* __restrict__ z2, uint16_t* __restrict__ dz2, int N,
+                      int H, int W, int C) {
+    const int Hh = H / 2, Wh = W / 2;
+    const size_t total = (size_t)N * Hh * Wh * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t pix = i / C;
+    const int xh = (int)(pix % Wh), yh = (int)((pix / Wh) % Hh), n = (int)(pix / ((size_t)Wh * Hh));
+    const size_t b = (((size_t)n * H + 2 * yh) * W + 2 * xh) * C + c;
+    const float s = du[b] + du[b + C] + du[b + (size_t)W * C] + du[b + (size_t)W * C + C];
+    dz2[i] = (uint16_t)bf16_bits(s * gelu_grad(z2[i]));
+}
+
+// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, C = 32.
+__global__ void k_dz1(const float* __restrict__ ds, const float* __restrict__ dp, const float* __restrict__ z1,
+                      float* __restrict__ dz1, int N, int H, int W, int C) {
+    const size_t total = (size_t)N * H * W * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t pix = i / C;
+    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
+    const size_t ph = (((size_t)n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
+    dz1[i] = (ds[i] + 0.25f * dp[ph]) * gelu_grad(z1[i]);
+}
+
+// ---------------------------------------------------------------- weight grads
+// dW[co][t][ci] = sum over pixels of dZ[pix][co] * X[pix + tap_t][ci].
+// Thread = 4 output channels x one (t, ci); blockIdx.y = pixel chunk; the
+// chunk partials go to part[chunk][co][t][ci] for a fixed-order reduction.
+// dZ is fp32 (dz_f) or bf16 (dz_h); X is bf16 NHWC with channel stride
+// xstride (the first layer reads channel 12, tap (0,0), of its im2col), at
+// half resolution when `up` (the layer-3 input u = upsample2(a2)).
+__global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ part) {
+    const int Cq = (a.Cout + 3) / 4;
+    const int nout = Cq * 25 * a.Cin;
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= nout) return;
+    const int ci = o % a.Cin, t = (o / a.Cin) % 25, cq = o / (a.Cin * 25);
+    const int dy = t / 5 - 2, dx = t % 5 - 2;
+    const size_t npix = (size_t)a.N * a.H * a.W;
+    const size_t per = (npix + gridDim.y - 1) / gridDim.y;
+    const size_t p0 = per * blockIdx.y, p1 = p0 + per < npix ? p0 + per : npix;
+    const int Hx = a.up ? a.H / 2 : a.H, Wx = a.up ? a.W / 2 : a.W;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (size_t pix = p0; pix < p1; ++pix) {
+        const int xx = (int)(pix % a.W), yy = (int)((pix / a.W) % a.H), n = (int)(pix / ((size_t)a.W * a.H));
+        int sy = wrap(yy + dy, a.H), sx = wrap(xx + dx, a.W);
+        if (a.up) {
+            sy >>= 1;
+            sx >>= 1;
+        }
+        const float xv = bf16f(a.X[(((size_t)n * Hx + sy) * Wx + sx) * a.xstride + a.xoff + ci]);
+        if (a.Cout == 1) {
+            acc[0] += a.dz_f[pix] * xv;
+        } else if (a.dz_f) {
+            const float4 g = *reinterpret_cast<const float4*>(a.dz_f + pix * a.Cout + 4 * cq);
+            acc[0] += g.x * xv;
+            acc[1] += g.y * xv;
+            acc[2] += g.z * xv;
+            acc[3] += g.w * xv;
+        } else {
+            const uint2 g = *reinterpret_cast<const uint2*>(a.dz_h + pix * a.Cout + 4 * cq);
+            acc[0] += __uint_as_float(g.x << 16) * xv;
+            acc[1] += __uint_as_float(g.x & 0xffff0000u) * xv;
+            acc[2] += __uint_as_float(g.y << 16) * xv;
+            acc[3] += __uint_as_float(g.y & 0xffff0000u) * xv;
+        }
+    }
+    float* dst = part + (size_t)blockIdx.y * a.Cout * 25 * a.Cin;
+    const int nco = a.Cout == 1 ? 1 : 4;
+    for (int j = 0; j < nco; ++j) dst[((size_t)(4 * cq + j) * 25 + t) * a.Cin + ci] = acc[j];
+}
+
+// Bias gradient partials: part[chunk][c] = sum over the chunk's pixels of dZ[pix][c].
+__global__ void __launch_bounds__(256) k_bgrad(const float* __restrict__ dz_f, const uint16_t* __restrict__ dz_h,
+                                               int C, size_t npix, float* __restrict__ part) {
+    const size_t per = (npix + gridDim.x - 1) / gridDim.x;
+    const size_t p0 = per * blockIdx.x, p1 = p0 + per < npix ? p0 + per : npix;
+    for (int c = threadIdx.x; c < C; c += 256) {
+        float acc = 0.0f;
+        for (size_t pix = p0; pix < p1; ++pix) acc += dz_f ? dz_f[pix * C + c] : bf16f(dz_h[pix * C + c]);
+        part[(size_t)blockIdx.x * C + c] = acc;
+    }
+}
+
+// out[i] = sum over chunks of part[chunk][i], in chunk order.
+__global__ void k_reduce_parts(const float* __restrict__ part, int chunks, int n, float* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += part[(size_t)c * n + i];
+    out[i] = acc;
+}
+
+// ---------------------------------------------------------------- Adam
+// theta -= lr * mhat / (sqrt(vhat) + eps); bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t.
+__global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    theta[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Device copies of the master weights in the layouts the kernels read:
+// W1x [32][1][32] (im2col channel t), W2 [64][25][32], W3 [32][25][64] as
+// bf16; W2T [32][25][64] and W3T [64][25][32] tap-flipped and
+// channel-swapped for the backward input gradients; W4 [25][32] as fp32 of
+// the bf16-rounded values.
+__global__ void k_pack_weights(const float* __restrict__ theta, PackedWeights w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 32 * 32) {  // W1x[co][0][t]
+        const int co = i / 32, t = i % 32;
+        w.W1x[i] = t < 25 ? (uint16_t)bf16_bits(theta[OFF_W1 + co * 25 + t]) : (uint16_t)0;
+    }
+    if (i < 64 * 25 * 32) {  // W2[co][t][ci] and W2T[ci][24-t][co]
+        const int ci = i % 32, t = (i / 32) % 25, co = i / (32 * 25);
+        const uint16_t h = (uint16_t)bf16_bits(theta[OFF_W2 + i]);
+        w.W2[i] = h;
+        w.W2T[((size_t)ci * 25 + (24 - t)) * 64 + co] = h;
+    }
+    if (i < 32 * 25 * 64) {  // W3[co][t][ci] and W3T[ci][24-t][co]
+        const int ci = i % 64, t = (i / 64) % 25, co = i / (64 * 25);
+        const uint16_t h = (uint16_t)bf16_bits(theta[OFF_W3 + i]);
+        w.W3[i] = h;
+        w.W3T[((size_t)ci * 25 + (24 - t)) * 32 + co] = h;
+    }
+    if (i < 25 * 32) w.W4[i] = __uint_as_float(bf16_bits(theta[OFF_W4 + i]) << 16);
+}
+
+// Crop a (side x side) window at (cx, cy) of a torus state into dst.
+__global__ void k_crop(const float* __restrict__ state, int W, int H, const int2* __restrict__ corners, int side,
+                       float* __restrict__ dst) {
+    const int c = blockIdx.y;
+    const int2 cc = corners[c];
+    const size_t n = (size_t)side * side;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int xx = (int)(i % side), yy = (int)(i / side);
+        dst[(size_t)c * n + i] = state[(size_t)((cc.y + yy) % H) * W + (cc.x + xx) % W];
+    }
+}
+
+// Gather batch items: dst[b] = src[index[b]], each `n` floats.
+__global__ void k_gather(const float* __restrict__ src, const int* __restrict__ index, size_t n,
+                         float* __restrict__ dst) {
+    const int b = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[(size_t)b * n + i] = src[(size_t)index[b] * n + i];
+}
+
+// FP64 partials of sum (a-b)^2 and sum b^2 (relative L2 of a against b).
+__global__ void __launch_bounds__(256) k_rel_l2(const float* __restrict__ a, const float* __restrict__ b, size_t n,
+                                                double* __restrict__ part) {
+    __shared__ double s0[256], s1[256];
+    double d2 = 0.0, b2 = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double d = (double)a[i] - (double)b[i];
+        d2 += d * d;
+        b2 += (double)b[i] * (double)b[i];
+    }
+    s0[threadIdx.x] = d2;
+    s1[threadIdx.x] = b2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s0[threadIdx.x] += s0[threadIdx.x + s];
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+inline unsigned blocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, int H, int W) {
+    Launch L(p, s, "surr_im2col1");
+    hipLaunchKernelGGL(k_im2col1, dim3(blocks((size_t)N * H * W * 32)), dim3(256), 0, s, x, X1, N, H, W);
+    AH_HIP(hipGetLastError());
+}
+
+void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
+    if (a.W % 64 || a.H % 4 || a.Cin % 32 || (a.Cout != 32 && a.Cout != 64))
+        throw Error{AH_ERR_INVALID, "surrogate conv: unsupported shape"};
+    Launch L(p, s, name);
+    dim3 grid(a.W / 64, a.H / 4, a.N);
+    if (a.Cout == 64) hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, a);
+    AH_HIP(hipGetLastError());
+}
+
+void pool2(hipStream_t s, Profiler* p, const uint16_t* a, uint16_t* out, int N, int H, int W, int C) {
+    Launch L(p, s, "surr_pool2");
+    hipLaunchKernelGGL(k_pool2, dim3(blocks((size_t)N * H * W * C / 4)), dim3(256), 0, s, a, out, N, H, W, C);
+    AH_HIP(hipGetLastError());
+}
+
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const float* W4, const float* b4, const float* x,
+               float* r, float* y, int N, int H, int W) {
+    Launch L(p, s, "surr_out_layer");
+    hipLaunchKernelGGL(k_out_layer, dim3(blocks((size_t)N * H * W)), dim3(256), 0, s, sa, W4, b4, x, r, y, N, H, W);
+    AH_HIP(hipGetLastError());
+}
+
+int loss_blocks(size_t total) { return (int)std::min<size_t>(1024, (total + 255) / 256); }
+
+void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const float* x, const float* r, float* dr,
+          double* part, size_t total) {
+    Launch L(p, s, "surr_loss");
+    hipLaunchKernelGGL(k_loss, dim3(loss_blocks(total)), dim3(256), 0, s, y, t, x, r, dr, part, total);
+    AH_HIP(hipGetLastError());
+}
+
+void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W) {
+    Launch L(p, s, "surr_out_input_grad");
+    hipLaunchKernelGGL(k_out_input_grad, dim3(blocks((size_t)N * H * W * 32)), dim3(256), 0, s, dr, W4, ds, N, H, W);
+    AH_HIP(hipGetLastError());
+}
+
+void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total) {
+    Launch L(p, s, "surr_dz3");
+    hipLaunchKernelGGL(k_dz3, dim3(blocks(total)), dim3(256), 0, s, ds, z3, out, total);
+    AH_HIP(hipGetLastError());
+}
+
+void dz2(hipStream_t s, Profiler* p, const float* du, const float* z2, uint16_t* out, int N, int H, int W, int C) {
+    Launch L(p, s, "surr_dz2");
+    hipLaunchKernelGGL(k_dz2, dim3(blocks((size_t)N * H * W * C / 4)), dim3(256), 0, s, du, z2, out, N, H, W, C);
+    AH_HIP(hipGetLastError());
+}
+
+void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
+         int C) {
+    Launch L(p, s, "surr_dz1");
+    hipLaunchKernelGGL(k_dz1, dim3(blocks((size_t)N * H * W * C)), dim3(256), 0, s, ds, dp, z1, out, N, H, W, C);
+    AH_HIP(hipGetLastError());
+}
+
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out) {
+    const int Cq = (a.Cout + 3) / 4;
+    const int nout = Cq * 25 * a.Cin, nw = a.Cout * 25 * a.Cin;
+    {
+        Launch L(p, s, "surr_wgrad");
+        hipLaunchKernelGGL(k_wgrad, dim3(blocks(nout), chunks), dim3(256), 0, s, a, part);
+        AH_HIP(hipGetLastError());
+    }
+    Launch L(p, s, "surr_reduce");
+    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
+    AH_HIP(hipGetLastError());
+}
+
+void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
+           int chunks, float* out) {
+    {
+        Launch L(p, s, "surr_bgrad");
+        hipLaunchKernelGGL(k_bgrad, dim3(chunks), dim3(256), 0, s, dz_f, dz_h, C, npix, part);
+        AH_HIP(hipGetLastError());
+    }
+    Launch L(p, s, "surr_reduce");
+    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(C)), dim3(256), 0, s, part, chunks, C, out);
+    AH_HIP(hipGetLastError());
+}
+
+void adam(hipStream_t s, Profiler* p, float* theta, float* m, float* v, const float* g, int n, float lr, float b1,
+          float b2, float eps, float c1, float c2) {
+    Launch L(p, s, "surr_adam");
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, s, theta, m, v, g, n, lr, b1, b2, eps, c1, c2);
+    AH_HIP(hipGetLastError());
+}
+
+void pack_weights(hipStream_t s, Profiler* p, const float* theta, const PackedWeights& w) {
+    Launch L(p, s, "surr_pack_weights");
+    hipLaunchKernelGGL(k_pack_weights, dim3(blocks(64 * 25 * 32)), dim3(256), 0, s, theta, w);
+    AH_HIP(hipGetLastError());
+}
+
+void crop(hipStream_t s, const float* state, int W, int H, const int2* corners, int count, int side, float* dst) {
+    hipLaunchKernelGGL(k_crop, dim3(64, count), dim3(256), 0, s, state, W, H, corners, side, dst);
+    AH_HIP(hipGetLastError());
+}
+
+void gather(hipStream_t s, const float* src, const int* index, int count, size_t n, float* dst) {
+    hipLaunchKernelGGL(k_gather, dim3(64, count), dim3(256), 0, s, src, index, n, dst);
+    AH_HIP(hipGetLastError());
+}
+
+int rel_l2_blocks(size_t n) { return (int)std::min<size_t>(1024, (n + 255) / 256); }
+
+void rel_l2(hipStream_t s, const float* a, const float* b, size_t n, double* part) {
+    hipLaunchKernelGGL(k_rel_l2, dim3(rel_l2_blocks(n)), dim3(256), 0, s, a, b, n, part);
+    AH_HIP(hipGetLastError());
+}
+
+}  // namespace surr
+}  // namespace ah

@@ -174,4 +174,79 @@ class Batch:
         return {"kernel_ms": k.value, "graph_wall_ms": g.value, "eager_wall_ms": e.value}
 
 
-__all__ = ["Sim", "Batch", "build", "available", "workloads", "_abi"]
+class Surrogate:
+    """The learned k-step surrogate (ah_surrogate_*); no reference counterpart."""
+
+    def __init__(self, seed: int = 1):
+        lib = load()
+        self._lib = lib
+        h = ctypes.c_void_p()
+        check(lib.ah_surrogate_new(seed & 0xFFFFFFFF, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ah_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(_abi.SURROGATE_PARAMS, np.float32)
+        check(self._lib.ah_surrogate_get_params(self._h, out.ctypes.data, out.size))
+        return out
+
+    def set_params(self, theta: np.ndarray) -> None:
+        a = np.ascontiguousarray(theta, np.float32)
+        check(self._lib.ah_surrogate_set_params(self._h, a.ctypes.data, a.size))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        n, h, w = x.shape
+        y = np.empty_like(x)
+        check(self._lib.ah_surrogate_forward(self._h, x.ctypes.data, n, h, w, y.ctypes.data))
+        return y
+
+    def loss_grad(self, x: np.ndarray, target: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        n, h, w = x.shape
+        loss = ctypes.c_double()
+        g = np.empty(_abi.SURROGATE_PARAMS, np.float32)
+        check(self._lib.ah_surrogate_loss_grad(self._h, x.ctypes.data, t.ctypes.data, n, h, w, ctypes.byref(loss),
+                                               g.ctypes.data))
+        return loss.value, g
+
+    def collect(self, sim: Sim, pairs: int, crop: int = 256, horizon: int = 4, seed: int = 1) -> None:
+        check(self._lib.ah_surrogate_collect(self._h, sim._h, pairs, crop, horizon, seed))
+        self.crop, self.horizon, self.n_pairs = crop, horizon, pairs
+
+    def dataset(self, first: int, count: int):
+        x = np.empty((count, self.crop, self.crop), np.float32)
+        y = np.empty_like(x)
+        check(self._lib.ah_surrogate_dataset(self._h, first, count, x.ctypes.data, y.ctypes.data))
+        return x, y
+
+    def train(self, steps: int, batch: int = 64, n_train: int | None = None, lr: float = 1e-3,
+              seed: int = 1) -> np.ndarray:
+        losses = np.empty(steps, np.float32)
+        n_train = self.n_pairs if n_train is None else n_train
+        check(self._lib.ah_surrogate_train(self._h, steps, batch, n_train, lr, seed, losses.ctypes.data))
+        return losses
+
+    def eval(self, first: int, count: int) -> float:
+        mse = ctypes.c_double()
+        check(self._lib.ah_surrogate_eval(self._h, first, count, ctypes.byref(mse)))
+        return mse.value
+
+    def advance(self, sim: Sim, steps: int, judge_every: int = 16, tol: float = 1e-2) -> dict:
+        r = _abi.SurrogateReport()
+        check(self._lib.ah_surrogate_advance(self._h, sim._h, steps, judge_every, tol, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in r._fields_}
+
+
+__all__ = ["Sim", "Batch", "Surrogate", "build", "available", "workloads", "_abi"]

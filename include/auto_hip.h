@@ -219,6 +219,61 @@ ah_status ah_batch_measure(ah_batch* batch, uint32_t steps, double* kernel_ms, d
 void ah_batch_destroy(ah_batch* batch);
 
 /*
+ * Learned k-step surrogate.  No counterpart in the reference.  A small
+ * convolutional network maps a single-channel state to the state k steps
+ * ahead: a 2-level encoder-decoder of circular 5x5 convolutions, widths
+ * 32 -> 64 -> 32, GELU, an additive skip, residual output clipped to [0, 1]
+ * (the full definition is in auto_amd/csrc/surrogate.hip).  Weights and
+ * activations are bf16 with FP32 accumulation; FP32 master weights and Adam
+ * state live on the device.  Sizes fed to the network need height % 8 == 0
+ * and width % 128 == 0.
+ */
+typedef struct ah_surrogate ah_surrogate;
+
+enum { AH_SURROGATE_PARAMS = 104129 };
+
+typedef struct {
+    uint64_t lenia_steps;      /* steps the call advanced the sim             */
+    uint64_t surrogate_calls;  /* network calls whose output was kept         */
+    uint64_t plain_steps;      /* plain steps (judges, fallback, remainder)   */
+    uint64_t judges, fallbacks;
+    double max_judge_rel_l2, last_judge_rel_l2;
+    double ms;                 /* host wall time of the call                  */
+} ah_surrogate_report;
+
+/* Parameters drawn from SplitMix64(seed): uniform +-sqrt(3/fan_in), the
+ * output layer scaled by 0.1, biases zero. */
+ah_status ah_surrogate_new(uint32_t seed, ah_surrogate** out);
+void ah_surrogate_destroy(ah_surrogate* s);
+/* The flat FP32 parameter vector (W1, b1, W2, b2, W3, b3, W4, b4). */
+ah_status ah_surrogate_get_params(const ah_surrogate* s, float* dst, size_t n);
+ah_status ah_surrogate_set_params(ah_surrogate* s, const float* src, size_t n);
+/* Network output for host states x [n][h][w] -> y [n][h][w]. */
+ah_status ah_surrogate_forward(ah_surrogate* s, const float* x, uint32_t n, uint32_t h, uint32_t w, float* y);
+/* MSE loss against target and, if grad is set, its gradient (n_params floats). */
+ah_status ah_surrogate_loss_grad(ah_surrogate* s, const float* x, const float* target, uint32_t n, uint32_t h,
+                                 uint32_t w, double* loss, float* grad);
+/* Training pairs (x_t, x_{t+horizon}): `pairs` crops of side `crop` at
+ * SplitMix64(seed) positions, from the sim's own trajectory (it advances). */
+ah_status ah_surrogate_collect(ah_surrogate* s, ah_sim* sim, uint32_t pairs, uint32_t crop, uint32_t horizon,
+                               uint32_t seed);
+ah_status ah_surrogate_dataset(ah_surrogate* s, uint32_t first, uint32_t count, float* x, float* y);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches drawn from
+ * pairs [0, n_train); losses[i] is step i's minibatch MSE (may be NULL). */
+ah_status ah_surrogate_train(ah_surrogate* s, uint32_t steps, uint32_t batch, uint32_t n_train, float lr,
+                             uint32_t seed, float* losses);
+/* MSE of the network on pairs [first, first + count). */
+ah_status ah_surrogate_eval(ah_surrogate* s, uint32_t first, uint32_t count, double* mse);
+/*
+ * Advance `sim` by `steps` steps with the network (horizon steps per call).
+ * Every judge_every-th call is judged: `horizon` plain steps from the same
+ * input, relative L2 of the network's output against them; the sim keeps the
+ * plain result, and past `tol` the rest of the call falls back to plain steps.
+ */
+ah_status ah_surrogate_advance(ah_surrogate* s, ah_sim* sim, uint32_t steps, uint32_t judge_every, float tol,
+                               ah_surrogate_report* report);
+
+/*
  * Bandwidth probe: a plain vectorised device-to-device copy of `bytes`
  * (read `bytes`, write `bytes`), run `iters` times on a private stream after
  * one warm-up.  Returns the mean device milliseconds per copy.  It gives the
