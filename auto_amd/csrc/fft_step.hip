@@ -192,11 +192,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
     spec += (size_t)ch * (W >> 1) * H;
     // `rows` consecutive rows are one contiguous run of rows*M float2.
     const float4* src = reinterpret_cast<const float4*>(state + (size_t)y0 * W);
-    const int n4 = rows * M / 2;
-    for (int i = tid; i < n4; i += nt) {
-        float4 v = src[i];
-        lds[pad(2 * i)] = make_float2(v.x, v.y);
-        lds[pad(2 * i + 1)] = make_float2(v.z, v.w);
+    const int n4 = rows * M / 2;  // = nt * E / 2
+    {
+        // Every load is issued before the first LDS store: E/2 in flight per thread.
+        float4 v[E / 2];
+#pragma unroll
+        for (int j = 0; j < E / 2; ++j) v[j] = src[tid + j * nt];
+#pragma unroll
+        for (int j = 0; j < E / 2; ++j) {
+            const int i = tid + j * nt;
+            lds[pad(2 * i)] = make_float2(v[j].x, v[j].y);
+            lds[pad(2 * i + 1)] = make_float2(v[j].z, v[j].w);
+        }
     }
     __syncthreads();
     fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W

@@ -134,6 +134,60 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
             "measure": m}
 
 
+def time_multichannel(path: str, steps: int, seed: int):
+    """1024^2 x 3 channels, 9 kernels (k = 3*src + dst) drawn from SplitMix64."""
+    from auto_amd import Sim, workloads as W
+    d = W.multichannel_1024(seed=seed, path=path)
+    sim = Sim(d, seed)
+    sim.step(10)
+    sim.wait()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.wait()
+    dt = time.perf_counter() - t0
+    cells = d.width * d.height
+    return d, sim, {"value": cells * steps / dt, "unit": "cell-updates/s (3 channels per cell)",
+                    "channel_cell_updates_per_s": 3 * cells * steps / dt, "ms_per_step": 1e3 * dt / steps}
+
+
+def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 256, steps: int = 1000):
+    """Learned 4-step surrogate: pairs from the 4096^2 three-ring trajectory,
+    Adam training, then `steps` judged steps at 4096^2 beside plain stepping."""
+    import numpy as np
+    from auto_amd import Sim, Surrogate, workloads as W
+    d = W.rings3_4096()
+    gen = Sim(d, seed + 100)
+    gen.step(50)
+    net = Surrogate(seed=seed)
+    t0 = time.perf_counter()
+    net.collect(gen, pairs=pairs, crop=crop, horizon=4, seed=seed)
+    t1 = time.perf_counter()
+    held = min(512, pairs // 10)
+    losses = net.train(steps=train_steps, batch=64, n_train=pairs - held, lr=1e-3, seed=seed)
+    t2 = time.perf_counter()
+    held_mse = net.eval(pairs - held, held)
+    xh, yh = net.dataset(pairs - held, held)
+    a, b = Sim(d, seed), Sim(d, seed)
+    a.step(50)
+    b.step(50)
+    rep = net.advance(a, steps, judge_every=16, tol=1e-2)
+    t3 = time.perf_counter()
+    b.step(steps)
+    b.wait()
+    t4 = time.perf_counter()
+    sa, sb = a.read_field("state"), b.read_field("state")
+    cells = d.width * d.height
+    return {"pairs": pairs, "crop": crop, "collect_s": t1 - t0, "train_steps": train_steps, "batch": 64,
+            "train_s": t2 - t1, "train_ms_per_step": 1e3 * (t2 - t1) / max(train_steps, 1),
+            "loss_first10": float(np.mean(losses[:10])), "loss_last10": float(np.mean(losses[-10:])),
+            "held_out_mse": held_mse, "held_out_identity_mse": float(np.mean((xh - yh) ** 2)),
+            "advance": rep, "value": cells * steps / (rep["ms"] * 1e-3), "unit": "cell-updates/s",
+            "plain_value": cells * steps / (t4 - t3),
+            "final_rel_l2_vs_plain": float(np.linalg.norm(sa - sb) / np.linalg.norm(sb)),
+            "mass_drift_vs_plain": float(abs(sa.sum(dtype=np.float64) - sb.sum(dtype=np.float64))
+                                         / sb.sum(dtype=np.float64))}
+
+
 def cpu_leg(d, sim, budget_s: float, time_cpu: bool = True):
     """Oracle leg: time the FP32 port of the reference step on host cores and
     check one GPU step against the FP64 oracle step from the same input."""
@@ -142,13 +196,14 @@ def cpu_leg(d, sim, budget_s: float, time_cpu: bool = True):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     O.set_threads(threads)
     s = sim.read_field("state")
-    spectra = O.kernel_spectra32(d)
     if not time_cpu:
         sim.step(1)
         got = sim.read_field("state")
-        diff = got.astype(np.float64) - O.step64(d, s, "fft")
+        want = O.step64(d, s, "fft")
+        diff = got.astype(np.float64) - want
         return None, {"max_abs": float(np.abs(diff).max()),
-                      "rel_l2": float(np.linalg.norm(diff) / max(np.linalg.norm(O.step64(d, s, "fft")), 1e-300))}
+                      "rel_l2": float(np.linalg.norm(diff) / max(np.linalg.norm(want), 1e-300))}
+    spectra = O.kernel_spectra32(d)
     t0 = time.perf_counter()
     n = 0
     x = s
@@ -183,6 +238,8 @@ def main() -> int:
     ap.add_argument("--path", default="fft_fp32", choices=["fft_fp32", "direct_bf16"],
                     help="path whose rate is `value` (both are timed)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 384-world batch")
+    ap.add_argument("--surrogate", action="store_true", help="also train and run the learned surrogate (minutes)")
+    ap.add_argument("--surrogate-train-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     args = ap.parse_args()
@@ -251,12 +308,21 @@ def main() -> int:
     }
     if not args.no_batch:
         out["batch"] = time_batch(args.seed)
+    mc = {}
+    for path in ("fft_fp32", "direct_bf16"):
+        dm, sm, rm = time_multichannel(path, 200, args.seed)
+        mc[path] = rm
+        sims["mc_" + path] = (dm, sm)
+    out["multichannel"] = mc
+    if args.surrogate:
+        out["surrogate"] = time_surrogate(args.seed, args.surrogate_train_steps)
     if not args.no_cpu and rank == 0:
         base, err = cpu_leg(*sims["fft_fp32"], args.cpu_seconds)
         out["cpu_baseline"] = base
         out["err_vs_fp64_one_step"] = {"fft_fp32": err}
-        _, err_d = cpu_leg(*sims["direct_bf16"], 0.0, time_cpu=False)
-        out["err_vs_fp64_one_step"]["direct_bf16"] = err_d
+        for key in ("direct_bf16", "mc_fft_fp32", "mc_direct_bf16"):
+            _, e = cpu_leg(*sims[key], 0.0, time_cpu=False)
+            out["err_vs_fp64_one_step"][key] = e
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
