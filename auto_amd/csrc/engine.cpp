@@ -173,6 +173,7 @@ struct ah_sim {
 
     // FFT path
     float2* spec_src = nullptr;  // [C][W/8][H][4]
+    bool spec_ready = false;     // spec_src holds the half spectra of the current state
     float2* spec_k = nullptr;    // [K][W/8][H][4] (aliases spec_src when K == C == 1)
     float* khat = nullptr;       // [K][W/8][H/2+1][4]
     float2* kpack = nullptr;     // [K][H/2+1]
@@ -422,10 +423,15 @@ void setup(ah_sim* s) {
 void step_fft(ah_sim* s, float* conv_out, int conv_k) {
     const ah_lenia_desc& d = s->d;
     const int W = (int)d.width, H = (int)d.height, C = (int)d.channels, K = (int)d.n_kernels;
-    launch_row_r2c(s->stream, &s->prof, W, H, C, s->state, s->spec_src, s->twW);
+    // Pass 1 runs only when the state changed outside a step; otherwise the
+    // previous step's pass 3 already left the spectra of the current state.
+    if (!s->spec_ready) launch_row_r2c(s->stream, &s->prof, W, H, C, s->state, s->spec_src, s->twW);
+    // The column pass consumes spec_src (in place when it aliases spec_k).
+    s->spec_ready = false;
     launch_col_conv(s->stream, &s->prof, W, H, C, s->spec_src, s->spec_k, s->khat, s->kpack, s->twH, s->ksrc, K);
     launch_row_c2r(s->stream, &s->prof, W, H, C, K, s->spec_k, s->state, s->state, s->kp, d.dt, s->twW, conv_out,
-                   conv_k);
+                   conv_k, s->spec_src);
+    s->spec_ready = conv_out == nullptr;
 }
 
 void step_direct(ah_sim* s, float* conv_out, int conv_k) {
@@ -447,7 +453,12 @@ void step_once(ah_sim* s, float* conv_out, int conv_k) {
 }  // namespace
 
 namespace ah {
-float* sim_state(ah_sim* s) { return s->state; }
+// Callers may write through the returned pointer, so the spectra of the
+// previous state are dropped.
+float* sim_state(ah_sim* s) {
+    s->spec_ready = false;
+    return s->state;
+}
 hipStream_t sim_stream(ah_sim* s) { return s->stream; }
 const ah_lenia_desc& sim_desc(const ah_sim* s) { return s->d; }
 void sim_advance(ah_sim* s, uint32_t steps) {
@@ -550,6 +561,7 @@ ah_status ah_sim_write_field(ah_sim* s, const char* name, const void* src, size_
         if (std::strcmp(name, "state") != 0) throw Error{AH_ERR_UNKNOWN_FIELD, std::string("not writable: ") + name};
         if (bytes != sizeof(float) * s->cells() * s->d.channels)
             throw Error{AH_ERR_SIZE, "state is channels*height*width*4 bytes"};
+        s->spec_ready = false;
         AH_HIP(hipMemcpyAsync(s->state, src, bytes, hipMemcpyHostToDevice, s->stream));
         AH_HIP(hipStreamSynchronize(s->stream));
     });

@@ -44,6 +44,11 @@ __device__ __forceinline__ float2 mul_si(float2 a) {
     return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// log2 of a power of two.  Every size here (row and column lengths, rows and
+// columns per workgroup, stage spans) is one, so index math is shifts and
+// masks rather than integer division.
+__device__ __forceinline__ int ilog2(int v) { return __builtin_ctz(v); }
+
 // LDS index padding: one float2 every 32 breaks the power-of-two strides.
 __device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
 __host__ __device__ constexpr size_t padded(size_t n) { return n + (n >> 5) + 1; }
@@ -122,15 +127,15 @@ __device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ t
 template <int R, int E, int SIGN>
 __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, const float2* __restrict__ tw, int NT) {
     constexpr int NB = E / R;
-    const int per_seq = M / R;
+    const int per_seq = M / R, lg_seq = ilog2(per_seq);
     const int nthreads = blockDim.x, tid = threadIdx.x;
-    const int tstep = NT / (Ls * R);
+    const int tstep = NT >> ilog2(Ls * R);
     float2 v[NB][R];
     int base[NB], kk[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         int b = tid + i * nthreads;
-        int seq = b / per_seq, j = b - seq * per_seq;
+        int seq = b >> lg_seq, j = b & (per_seq - 1);
         int off = seq * M;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = lds[pad(off + j + r * per_seq)];
@@ -178,6 +183,39 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
 constexpr int FFT_THREADS = 512;  // upper bound on the workgroup size
 
 // ------------------------------------------------------------ pass 1
+// Even/odd split of `rows` forward-transformed rows in LDS (row r at r*M,
+// the length-M complex FFT of z[n] = x[2n] + i x[2n+1]) into the half
+// spectrum rows y0..y0+rows-1: X[k] = E[k] + w^k O[k], E = (Z[k] + conj
+// Z[M-k])/2, O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output
+// float4 q holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
+__device__ __forceinline__ void store_half_spectrum(const float2* lds, int M, int rows, int H, int y0,
+                                                    const float2* __restrict__ twW, float2* __restrict__ spec) {
+    float4* dst = reinterpret_cast<float4*>(spec);
+    const int n4 = rows * M / 2, lg2r = ilog2(2 * rows);
+    for (int q = threadIdx.x; q < n4; q += blockDim.x) {
+        int kb = q >> lg2r, rem = q & (2 * rows - 1);
+        int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
+        int off = row * M;
+        float2 out[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            int k = kx + t;
+            if (k == 0) {
+                float2 z0 = lds[pad(off)];
+                out[t] = make_float2(z0.x + z0.y, z0.x - z0.y);
+            } else {
+                float2 a = lds[pad(off + k)], b = conj2(lds[pad(off + M - k)]);
+                float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+                float2 d = csub(a, b);
+                float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // -i/2 * d
+                out[t] = cadd(e, cmul(twW[k], o));
+            }
+        }
+        size_t o4 = ((size_t)kb * H + y0 + row) * 2 + (rem & 1);
+        dst[o4] = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
+    }
+}
+
 // Grid: (H/rows, channels); `rows` consecutive rows per workgroup (one row
 // when W >= 4096), rows*M/8 threads.
 template <int E>
@@ -192,7 +230,6 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
     spec += (size_t)ch * (W >> 1) * H;
     // `rows` consecutive rows are one contiguous run of rows*M float2.
     const float4* src = reinterpret_cast<const float4*>(state + (size_t)y0 * W);
-    const int n4 = rows * M / 2;  // = nt * E / 2
     {
         // Every load is issued before the first LDS store: E/2 in flight per thread.
         float4 v[E / 2];
@@ -207,33 +244,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
     }
     __syncthreads();
     fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
-    // Even/odd split: X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[M-k])/2,
-    // O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output float4 q
-    // holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
-    float4* dst = reinterpret_cast<float4*>(spec);
-    for (int q = tid; q < n4; q += nt) {
-        int kb = q / (2 * rows), rem = q - kb * 2 * rows;
-        int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
-        const float2* z = lds;  // padded access below
-        int off = row * M;
-        float2 out[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            int k = kx + t;
-            if (k == 0) {
-                float2 z0 = z[pad(off)];
-                out[t] = make_float2(z0.x + z0.y, z0.x - z0.y);
-            } else {
-                float2 a = z[pad(off + k)], b = conj2(z[pad(off + M - k)]);
-                float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
-                float2 d = csub(a, b);
-                float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // -i/2 * d
-                out[t] = cadd(e, cmul(twW[k], o));
-            }
-        }
-        size_t o4 = ((size_t)kb * H + y0 + row) * 2 + (rem & 1);
-        dst[o4] = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
-    }
+    store_half_spectrum(lds, M, rows, H, y0, twW, spec);
 }
 
 // ------------------------------------------------------------ pass 2
@@ -257,15 +268,17 @@ struct ColArgs {
 __device__ __forceinline__ size_t col_index(int kb, int c, int y, int H) { return ((size_t)kb * H + y) * BC + c; }
 
 __device__ __forceinline__ void load_cols(float2* lds, const float2* __restrict__ src, int kb, int c0, int cw, int H) {
+    const int lg = ilog2(cw);
     for (int t = threadIdx.x; t < cw * H; t += blockDim.x) {
-        int y = t / cw, c = t - y * cw;
+        int y = t >> lg, c = t & (cw - 1);
         lds[pad(c * H + y)] = src[col_index(kb, c0 + c, y, H)];
     }
 }
 
 __device__ __forceinline__ void store_cols(const float2* lds, float2* __restrict__ dst, int kb, int c0, int cw, int H) {
+    const int lg = ilog2(cw);
     for (int t = threadIdx.x; t < cw * H; t += blockDim.x) {
-        int y = t / cw, c = t - y * cw;
+        int y = t >> lg, c = t & (cw - 1);
         dst[col_index(kb, c0 + c, y, H)] = lds[pad(c * H + y)];
     }
 }
@@ -275,10 +288,10 @@ __device__ __forceinline__ void store_cols(const float2* lds, float2* __restrict
 // needs: D[ky] = A C[ky] + B conj C[-ky] with A=(K0+KN)/2, B=(K0-KN)/2.
 __device__ __forceinline__ void multiply_cols(float2* lds, const ColArgs& a, int kb, int c0, int k) {
     const int H = a.H, half = H >> 1, cw = a.cw;
-    const int npairs = half + 1;
+    const int npairs = half + 1, lg = ilog2(cw);
     const float* kh = a.khat + ((size_t)k * (a.W >> 3) + kb) * (size_t)npairs * BC;
     for (int t = threadIdx.x; t < npairs * cw; t += blockDim.x) {
-        int p = t / cw, c = t - p * cw;
+        int p = t >> lg, c = t & (cw - 1);
         int y1 = p, y2 = (H - p) & (H - 1);
         float2 c1 = lds[pad(c * H + y1)], c2 = lds[pad(c * H + y2)];
         float2 d1, d2;
@@ -353,6 +366,7 @@ struct RowArgs {
     KernelParams kp[AH_MAX_KERNELS];
     float* conv_out;  // when set: write kernel conv_k's potential instead of updating
     int conv_k;
+    float2* spec_next;  // NEXT: half spectra [channels][W/8][H][4] of the updated state
 };
 
 // Load kernel k's spectrum rows into LDS and turn them into the length-M
@@ -361,42 +375,48 @@ struct RowArgs {
 __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
     const float4* s4 = reinterpret_cast<const float4*>(a.spec + (size_t)k * M * H);
-    const int n4 = rows * M / 2;
+    const int n4 = rows * M / 2, lg2r = ilog2(2 * rows);
     for (int q = threadIdx.x; q < n4; q += blockDim.x) {
-        int kb = q / (2 * rows), rem = q - kb * 2 * rows;
+        int kb = q >> lg2r, rem = q & (2 * rows - 1);
         int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
         float4 v = s4[((size_t)kb * H + y0 + row) * 2 + (rem & 1)];
         lds[pad(row * M + kx)] = make_float2(v.x, v.y);
         lds[pad(row * M + kx + 1)] = make_float2(v.z, v.w);
     }
     __syncthreads();
-    const int half = M >> 1;
-    for (int t = threadIdx.x; t < rows * (half + 1); t += blockDim.x) {
-        int row = t / (half + 1), kk = t - row * (half + 1);
+    // Thread t owns bins kk and M-kk of one row (kk in 1..M/2-1); kk = 0 also
+    // takes the DC/Nyquist pair and the self-paired middle bin M/2.
+    const int half = M >> 1, lgh = ilog2(half);
+    auto z_of = [&](float2 x1, float2 x2, float2 cw) {
+        float2 s1 = cadd(x1, conj2(x2)), d1 = csub(x1, conj2(x2));
+        return cadd(s1, cmul(make_float2(-d1.y, d1.x), cw));
+    };
+    for (int t = threadIdx.x; t < rows * half; t += blockDim.x) {
+        int row = t >> lgh, kk = t & (half - 1);
         int off = row * M;
         if (kk == 0) {
             float2 p = lds[pad(off)];
             lds[pad(off)] = make_float2(p.x + p.y, p.x - p.y);
+            float2 xm = lds[pad(off + half)];
+            lds[pad(off + half)] = z_of(xm, xm, conj2(a.twW[half]));
             continue;
         }
         int k2 = M - kk;
         float2 x1 = lds[pad(off + kk)], x2 = lds[pad(off + k2)];
-        // Z[kk] from (x1, x2); Z[k2] from (x2, x1) with w^k2 = -conj(w^kk).
+        // Z[kk] from (x1, x2); Z[k2] from (x2, x1) with conj(w^k2) = -w^kk.
         float2 w1 = a.twW[kk];
-        float2 s1 = cadd(x1, conj2(x2)), d1 = csub(x1, conj2(x2));
-        float2 z1 = cadd(s1, cmul(make_float2(-d1.y, d1.x), conj2(w1)));
-        lds[pad(off + kk)] = z1;
-        if (k2 != kk) {
-            float2 w2 = make_float2(-w1.x, -w1.y);  // conj(w^k2) = conj(-conj(w^kk)) = -w^kk
-            float2 s2 = cadd(x2, conj2(x1)), d2 = csub(x2, conj2(x1));
-            float2 z2 = cadd(s2, cmul(make_float2(-d2.y, d2.x), w2));
-            lds[pad(off + k2)] = z2;
-        }
+        lds[pad(off + kk)] = z_of(x1, x2, conj2(w1));
+        lds[pad(off + k2)] = z_of(x2, x1, make_float2(-w1.x, -w1.y));
     }
     __syncthreads();
 }
 
-template <int E>
+// NEXT: the updated rows are also the next step's pass-1 input, so they go
+// back into LDS and through the forward real FFT here, and the state is read
+// once per step instead of twice.  In place when spec_next aliases spec: a
+// workgroup reads and writes only its own rows, and every read of them is
+// behind the barrier at the end of the kernel loop.
+template <int E, bool NEXT>
 __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     extern __shared__ float2 lds[];
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
@@ -450,6 +470,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) r[j] = fminf(fmaxf(r[j] + a.dt * acc[i][j], 0.0f), 1.0f);
         o4[q] = make_float4(r[0], r[1], r[2], r[3]);
+        if constexpr (NEXT) {
+            lds[pad(2 * q)] = make_float2(r[0], r[1]);
+            lds[pad(2 * q + 1)] = make_float2(r[2], r[3]);
+        }
+    }
+    if constexpr (NEXT) {
+        __syncthreads();
+        fft_lds<E, -1>(lds, M, rows, a.twW, W);
+        store_half_spectrum(lds, M, rows, H, y0, a.twW, a.spec_next + (size_t)ch * M * H);
     }
 }
 
@@ -485,23 +514,14 @@ int col_cols(int H) {
 
 size_t fft_lds_limit() { return 160 * 1024; }
 
-// Complex elements per thread of each pass (8 or 16).
-int row_E(int W, int H) { return 8; }
-int col_E(int H) { return 8; }
-
 void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, const float* state, float2* spec,
                     const float2* twW) {
-    const int M = W / 2, rows = row_rows(W, H), E = row_E(W, H);
+    const int M = W / 2, rows = row_rows(W, H);
     const int elems = rows * M;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "row_r2c");
-    if (E == 16) {
-        set_lds(k_row_r2c<16>, lds);
-        hipLaunchKernelGGL(k_row_r2c<16>, dim3(H / rows, channels), dim3(elems / E), lds, s, W, H, rows, state, spec, twW);
-    } else {
-        set_lds(k_row_r2c<8>, lds);
-        hipLaunchKernelGGL(k_row_r2c<8>, dim3(H / rows, channels), dim3(elems / E), lds, s, W, H, rows, state, spec, twW);
-    }
+    set_lds(k_row_r2c<8>, lds);
+    hipLaunchKernelGGL(k_row_r2c<8>, dim3(H / rows, channels), dim3(elems / 8), lds, s, W, H, rows, state, spec, twW);
     AH_HIP(hipGetLastError());
 }
 
@@ -519,23 +539,17 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.twH = twH;
     a.n_kernels = n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
-    const int E = col_E(H);
     const int elems = a.cw * H;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
-    if (E == 16) {
-        set_lds(k_col_conv<16>, lds);
-        hipLaunchKernelGGL(k_col_conv<16>, dim3(W / 2 / a.cw, channels), dim3(elems / E), lds, s, a);
-    } else {
-        set_lds(k_col_conv<8>, lds);
-        hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / E), lds, s, a);
-    }
+    set_lds(k_col_conv<8>, lds);
+    hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / 8), lds, s, a);
     AH_HIP(hipGetLastError());
 }
 
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
-                    float* conv_out, int conv_k) {
+                    float* conv_out, int conv_k, float2* spec_next) {
     RowArgs a;
     a.W = W;
     a.H = H;
@@ -549,17 +563,18 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
     a.conv_out = conv_out;
     a.conv_k = conv_k;
-    const int E = row_E(W, H);
+    a.spec_next = conv_out ? nullptr : spec_next;
     const int elems = a.rows * (W / 2);
     const size_t lds = padded(elems) * sizeof(float2);
     const int gy = conv_out ? 1 : channels;
-    Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
-    if (E == 16) {
-        set_lds(k_row_c2r<16>, lds);
-        hipLaunchKernelGGL(k_row_c2r<16>, dim3(H / a.rows, gy), dim3(elems / E), lds, s, a);
+    if (a.spec_next) {
+        Launch L(p, s, "row_c2r_update_r2c");
+        set_lds(k_row_c2r<8, true>, lds);
+        hipLaunchKernelGGL((k_row_c2r<8, true>), dim3(H / a.rows, gy), dim3(elems / 8), lds, s, a);
     } else {
-        set_lds(k_row_c2r<8>, lds);
-        hipLaunchKernelGGL(k_row_c2r<8>, dim3(H / a.rows, gy), dim3(elems / E), lds, s, a);
+        Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
+        set_lds(k_row_c2r<8, false>, lds);
+        hipLaunchKernelGGL((k_row_c2r<8, false>), dim3(H / a.rows, gy), dim3(elems / 8), lds, s, a);
     }
     AH_HIP(hipGetLastError());
 }

@@ -33,7 +33,7 @@ BF16_PEAK_TFLOPS = 2500.0
 
 # FFT-path kernels priced by algorithmic HBM bytes (single channel, one
 # kernel; the half spectrum is W/2 x H complex64 = 4 B/cell).
-FFT_KERNELS = ("row_r2c", "col_conv", "row_c2r_update")
+FFT_KERNELS = ("row_r2c", "col_conv", "row_c2r_update", "row_c2r_update_r2c")
 
 
 def khat_bytes(w: int, h: int) -> int:
@@ -43,7 +43,8 @@ def khat_bytes(w: int, h: int) -> int:
 
 def kernel_bytes(name: str, w: int, h: int) -> float:
     """row_r2c: read state + write spectrum; col_conv: read + write spectrum
-    + K-hat; row_c2r_update: read spectrum + read state + write state."""
+    + K-hat; row_c2r_update: read spectrum + read state + write state;
+    row_c2r_update_r2c: that plus writing the next step's spectrum."""
     cells = w * h
     if name == "row_r2c":
         return 8.0 * cells
@@ -51,6 +52,8 @@ def kernel_bytes(name: str, w: int, h: int) -> float:
         return 8.0 * cells + khat_bytes(w, h)
     if name == "row_c2r_update":
         return 12.0 * cells
+    if name == "row_c2r_update_r2c":
+        return 16.0 * cells
     raise KeyError(name)
 
 
