@@ -44,6 +44,14 @@ __device__ __forceinline__ float2 mul_si(float2 a) {
     return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// Workgroup barrier for LDS traffic only: this wave's LDS operations finish,
+// then all waves meet.  Unlike __syncthreads() it does not wait for global
+// loads and stores in flight (its release fence would), so global stores and
+// global->LDS prefetches keep running across FFT stages.  Nothing in this
+// file exchanges global memory between the threads of a workgroup.  The
+// memory clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // log2 of a power of two.  Every size here (row and column lengths, rows and
 // columns per workgroup, stage spans) is one, so index math is shifts and
 // masks rather than integer division.
@@ -143,7 +151,7 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
         kk[i] = k;
         base[i] = off + (j - k) * R + k;
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         if (Ls > 1) {
@@ -156,7 +164,7 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[pad(base[i] + r * Ls)] = v[i][r];
     }
-    __syncthreads();
+    lds_sync();
 }
 
 // Complex FFT of length M (power of two, >= 8) on nseq sequences in LDS.
@@ -242,7 +250,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
             lds[pad(2 * i + 1)] = make_float2(v[j].z, v[j].w);
         }
     }
-    __syncthreads();
+    lds_sync();
     fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
     store_half_spectrum(lds, M, rows, H, y0, twW, spec);
 }
@@ -307,7 +315,7 @@ __device__ __forceinline__ void multiply_cols(float2* lds, const ColArgs& a, int
         lds[pad(c * H + y1)] = d1;
         if (y2 != y1) lds[pad(c * H + y2)] = d2;
     }
-    __syncthreads();
+    lds_sync();
 }
 
 template <int E>
@@ -319,7 +327,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     const int ch = blockIdx.y;
     const size_t chan = (size_t)(a.W >> 1) * H;  // float2 per channel / kernel spectrum
     load_cols(lds, a.spec_in + ch * chan, kb, c0, cw, H);
-    __syncthreads();
+    lds_sync();
     fft_lds<E, -1>(lds, H, cw, a.twH, H);
     int users = 0, last = -1;
     for (int k = 0; k < a.n_kernels; ++k)
@@ -342,15 +350,175 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     for (int k = 0; k < a.n_kernels; ++k) {
         if (a.ksrc[k] != ch) continue;
         if (!first) {
-            __syncthreads();
+            lds_sync();
 #pragma unroll
             for (int i = 0; i < E; ++i) lds[pad(threadIdx.x + i * blockDim.x)] = keep[i];
-            __syncthreads();
+            lds_sync();
         }
         first = false;
         multiply_cols(lds, a, kb, c0, k);
         fft_lds<E, 1>(lds, H, cw, a.twH, H);
         store_cols(lds, a.spec_out + k * chan, kb, c0, cw, H);
+    }
+}
+
+// Streamed column pass for H = 4096 with one channel read by one kernel.  A
+// persistent workgroup of 512 threads walks columns L, L+G, L+2G, ... of the
+// half spectrum with two LDS buffers: while it transforms, multiplies and
+// writes back the column in one buffer, the next column arrives in the other
+// via global_load_lds (each lane's dword goes to LDS without a register), so
+// the load of one column overlaps the arithmetic of the previous one.
+// Everything between the waits is LDS work or loads issued before the
+// prefetch, so the prefetch is waited for only at the top of the next column.
+constexpr int SH = 4096, ST = SH / 8;  // column length, threads
+
+// Rows 32r..32r+31 of column (kb, c) into the padded run at 33r: one wave
+// instruction, lane l fetches dword l&1 of row 32r + l/2.
+__device__ __forceinline__ void prefetch_column(float2* buf, const float2* __restrict__ src, int col) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const float* base = reinterpret_cast<const float*>(src + col_index(col >> 2, col & 3, 0, SH)) + (lane & 1);
+#pragma unroll
+    for (int i = 0; i < SH / 32 / (ST / 64); ++i) {
+        const int r = wave + i * (ST / 64);
+        const float* g = base + (size_t)(32 * r + (lane >> 1)) * (2 * BC);
+        __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(buf + 33 * r), 4, 0, 0);
+    }
+}
+
+// Wait for this wave's global memory operations, then for every wave.  The
+// vmcnt(0) wait is the builtin so the compiler's own wait tracking sees it
+// and adds no wait of its own for the prefetch into the other buffer.
+__device__ __forceinline__ void prefetch_sync() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt, lgkmcnt unconstrained
+    lds_sync();
+}
+
+// Length-4096 FFT of one column in LDS: four radix-8 Stockham stages (spans
+// 1, 8, 64, 512), thread t owning butterfly j = t of each.  tw[s] = w for
+// stage s+1, w = exp(-2*pi*i*k/(8*Ls)), k = t mod Ls, read once per kernel
+// from the FP64-built table; its powers are products (a few ulp).
+template <int SIGN>
+__device__ __forceinline__ void fft_column(float2* buf, const float2 (&tw)[3]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int Ls = 1 << (3 * s);
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = buf[pad(t + r * ST)];
+        const int k = t & (Ls - 1);
+        const int base = (t - k) * 8 + k;
+        lds_sync();
+        if (s > 0) {
+            float2 w[8];
+            w[1] = tw[s - 1];
+            // Opaque copy: keeps the compiler from hoisting the powers of all
+            // stages out of the column loop, which would hold ~70 registers.
+            asm volatile("" : "+v"(w[1].x), "+v"(w[1].y));
+            if (SIGN > 0) w[1].y = -w[1].y;
+            w[2] = cmul(w[1], w[1]);
+            w[4] = cmul(w[2], w[2]);
+            w[3] = cmul(w[1], w[2]);
+            w[5] = cmul(w[1], w[4]);
+            w[6] = cmul(w[2], w[4]);
+            w[7] = cmul(w[3], w[4]);
+#pragma unroll
+            for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w[r]);
+        }
+        dft8<SIGN>(v);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[pad(base + r * Ls)] = v[r];
+        lds_sync();
+    }
+}
+
+// K-hat factors of the pairs (p, H-p) this thread multiplies: p = t + 512i,
+// i < 4, and thread 0 also owns p = H/2 (the fifth load of every other thread
+// repeats that one and goes unused).  Always issued, also for the packed
+// kx=0 column, whose (A, B) factors multiply_column reads itself, so the
+// number of global operations in flight does not depend on the column.
+__device__ __forceinline__ void load_khat(float (&kf)[5], const ColArgs& a, int col) {
+    const int t = threadIdx.x;
+    constexpr int NP = SH / 2 + 1;
+    const float* kh = a.khat + (size_t)(col >> 2) * NP * BC + (col & 3);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) kf[i] = kh[(size_t)(i < 4 ? t + i * ST : SH / 2) * BC];
+}
+
+__device__ __forceinline__ void multiply_column(float2* buf, const float (&kf)[5], const float2* __restrict__ kpack,
+                                                bool packed) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        if (i == 4 && t != 0) break;
+        const int p = t + i * ST;
+        const int y1 = p, y2 = (SH - p) & (SH - 1);
+        float2 c1 = buf[pad(y1)], c2 = buf[pad(y2)];
+        float2 d1, d2;
+        if (packed) {
+            const float2 ab = kpack[p];
+            d1 = cadd(make_float2(ab.x * c1.x, ab.x * c1.y), make_float2(ab.y * c2.x, -ab.y * c2.y));
+            d2 = cadd(make_float2(ab.x * c2.x, ab.x * c2.y), make_float2(ab.y * c1.x, -ab.y * c1.y));
+        } else {
+            d1 = make_float2(kf[i] * c1.x, kf[i] * c1.y);
+            d2 = make_float2(kf[i] * c2.x, kf[i] * c2.y);
+        }
+        buf[pad(y1)] = d1;
+        if (y2 != y1) buf[pad(y2)] = d2;
+    }
+    lds_sync();
+}
+
+__device__ __forceinline__ void store_column(const float2* buf, float2* __restrict__ dst, int col) {
+    float2* d = dst + col_index(col >> 2, col & 3, 0, SH);
+#pragma unroll
+    for (int i = 0; i < SH / ST; ++i) {
+        const int y = threadIdx.x + i * ST;
+        d[(size_t)y * BC] = buf[pad(y)];
+    }
+}
+
+__device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const float2 (&tw)[3],
+                                            const float (&kf)[5], int col) {
+    fft_column<-1>(buf, tw);
+    multiply_column(buf, kf, a.kpack, col == 0);
+    fft_column<1>(buf, tw);
+    store_column(buf, a.spec_out, col);
+}
+
+// NC columns per workgroup, G = W/2/NC workgroups: column i of workgroup L
+// is L + i*G.  Fully unrolled, so each step names its buffers statically and
+// the compiler's wait tracking can tell the prefetch target from the buffer
+// being transformed.
+template <int NC>
+__global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4))) k_col_stream(ColArgs a) {
+    __shared__ float2 bufA[padded(SH)];
+    __shared__ float2 bufB[padded(SH)];
+    const int G = gridDim.x;
+    const int col0 = xcd_remap(blockIdx.x, G);
+    float2 tw[3];
+#pragma unroll
+    for (int s = 1; s < 4; ++s) {
+        const int Ls = 1 << (3 * s);
+        tw[s - 1] = a.twH[(threadIdx.x & (Ls - 1)) * (SH / (8 * Ls))];
+    }
+    // K-hat for column i is loaded with column i's prefetch, during column
+    // i-1, so both are complete at the wait that opens column i.
+    float kf[2][5];
+    load_khat(kf[0], a, col0);
+    prefetch_column(bufA, a.spec_in, col0);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        float2* cur = (i & 1) ? bufB : bufA;
+        float2* nxt = (i & 1) ? bufA : bufB;
+        const int col = col0 + i * G;
+        prefetch_sync();
+        if (i + 1 < NC) {
+            load_khat(kf[(i + 1) & 1], a, col + G);
+            prefetch_column(nxt, a.spec_in, col + G);
+        }
+        conv_column(cur, a, tw, kf[i & 1], col);
     }
 }
 
@@ -383,7 +551,7 @@ __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
         lds[pad(row * M + kx)] = make_float2(v.x, v.y);
         lds[pad(row * M + kx + 1)] = make_float2(v.z, v.w);
     }
-    __syncthreads();
+    lds_sync();
     // Thread t owns bins kk and M-kk of one row (kk in 1..M/2-1); kk = 0 also
     // takes the DC/Nyquist pair and the self-paired middle bin M/2.
     const int half = M >> 1, lgh = ilog2(half);
@@ -408,7 +576,7 @@ __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
         lds[pad(off + kk)] = z_of(x1, x2, conj2(w1));
         lds[pad(off + k2)] = z_of(x2, x1, make_float2(-w1.x, -w1.y));
     }
-    __syncthreads();
+    lds_sync();
 }
 
 // NEXT: the updated rows are also the next step's pass-1 input, so they go
@@ -450,7 +618,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
                 }
             }
         }
-        __syncthreads();
+        lds_sync();
     }
     const size_t cell0 = (size_t)y0 * W;
     if (a.conv_out) {
@@ -476,7 +644,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
         }
     }
     if constexpr (NEXT) {
-        __syncthreads();
+        lds_sync();
         fft_lds<E, -1>(lds, M, rows, a.twW, W);
         store_half_spectrum(lds, M, rows, H, y0, a.twW, a.spec_next + (size_t)ch * M * H);
     }
@@ -501,6 +669,16 @@ int row_rows(int W, int H) {
     int rows = 1;
     while (rows * M < 2048 && rows < H) rows *= 2;
     return rows;
+}
+
+// Compute units of the current device (cached per device).
+int cu_count() {
+    static int cached[64] = {};
+    int dev = 0;
+    AH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error{AH_ERR_DEVICE, "device index out of range"};
+    if (!cached[dev]) AH_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    return cached[dev];
 }
 
 // Columns per column-pass workgroup (1, 2 or 4 of a 4-column block).
@@ -539,6 +717,21 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.twH = twH;
     a.n_kernels = n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
+    if (channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == SH && W >= 64) {
+        // Up to four columns per workgroup, aiming at two workgroups per CU
+        // (each holds 2 x 33 KiB of LDS).
+        const int ncols = W / 2, slots = 2 * cu_count();
+        int nc = 1;
+        while (nc < 4 && ncols / (2 * nc) >= slots) nc *= 2;
+        const int G = ncols / nc;
+        a.cw = 1;
+        Launch L(p, s, "col_conv");
+        if (nc == 4) hipLaunchKernelGGL(k_col_stream<4>, dim3(G), dim3(ST), 0, s, a);
+        else if (nc == 2) hipLaunchKernelGGL(k_col_stream<2>, dim3(G), dim3(ST), 0, s, a);
+        else hipLaunchKernelGGL(k_col_stream<1>, dim3(G), dim3(ST), 0, s, a);
+        AH_HIP(hipGetLastError());
+        return;
+    }
     const int elems = a.cw * H;
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");

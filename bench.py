@@ -153,6 +153,78 @@ def time_multichannel(path: str, steps: int, seed: int):
                     "channel_cell_updates_per_s": 3 * cells * steps / dt, "ms_per_step": 1e3 * dt / steps}
 
 
+def log(msg: str) -> None:
+    """Progress on stderr; stdout carries only the JSON line."""
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_torch_fft(side: int, steps: int, seed: int):
+    """The same FP32 step as plain PyTorch eager code on the GPU, the way a
+    user without this engine would write it: torch.fft.rfft2 / irfft2 (rocFFT)
+    and elementwise growth.  The kernel comes from the engine itself (the
+    potential of a unit impulse at the origin is the kernel), so both sides
+    convolve with the same taps.  PyTorch runs in a child process: its own
+    HIP runtime sees no device in a process where this engine's runtime
+    already holds it."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    from auto_amd import Sim, workloads as W
+    d = W.rings3_4096(side=side)
+    k = d.kernels[0]
+    probe = Sim(d, seed)
+    impulse = np.zeros((1, d.height, d.width), np.float32)
+    impulse[0, 0, 0] = 1.0
+    probe.write_field("state", impulse)
+    kern = probe.read_field("conv0").reshape(d.height, d.width)
+    del probe
+    ours = Sim(d, seed)
+    ours.step(20)
+    s0 = ours.read_field("state").reshape(d.height, d.width)
+    ours.step(1)
+    one = ours.read_field("state").reshape(d.height, d.width)
+    del ours
+    with tempfile.TemporaryDirectory() as tmp:
+        np.savez(os.path.join(tmp, "in.npz"), kern=kern, s0=s0, one=one,
+                 params=np.array([k.mu, k.sigma, d.dt, k.weight], np.float64), steps=np.array(steps))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--torch-leg", tmp],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"torch leg failed ({r.returncode}): {r.stderr[-2000:]}")
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def torch_leg(tmp: str) -> int:
+    """Child process of time_torch_fft: PyTorch only, this engine not loaded."""
+    import numpy as np
+    import torch
+    z = np.load(os.path.join(tmp, "in.npz"))
+    mu, sigma, dt, weight = (float(v) for v in z["params"])
+    steps = int(z["steps"])
+    dev = torch.device("cuda:0")
+    khat = torch.fft.rfft2(torch.from_numpy(z["kern"]).to(dev))
+
+    def step(s):
+        u = torch.fft.irfft2(torch.fft.rfft2(s) * khat, s=s.shape)
+        g = 2.0 * torch.exp(-0.5 * ((u - mu) / sigma) ** 2) - 1.0
+        return torch.clamp(s + (dt * weight) * g, 0.0, 1.0)
+
+    s = torch.from_numpy(z["s0"]).to(dev)
+    err = float((step(s) - torch.from_numpy(z["one"]).to(dev)).abs().max())
+    for _ in range(10):
+        s = step(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = step(s)
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    cells = s.numel()
+    print(json.dumps({"value": cells * steps / dt_s, "unit": "cell-updates/s", "ms_per_step": 1e3 * dt_s / steps,
+                      "steps": steps, "torch": torch.__version__, "max_abs_vs_engine_one_step": err}), flush=True)
+    return 0
+
+
 def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 256, steps: int = 1000):
     """Learned 4-step surrogate: pairs from the 4096^2 three-ring trajectory,
     Adam training, then `steps` judged steps at 4096^2 beside plain stepping."""
@@ -165,14 +237,23 @@ def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 
     t0 = time.perf_counter()
     net.collect(gen, pairs=pairs, crop=crop, horizon=4, seed=seed)
     t1 = time.perf_counter()
+    log(f"surrogate: collected {pairs} pairs of {crop}^2 in {t1 - t0:.1f} s")
     held = min(512, pairs // 10)
-    losses = net.train(steps=train_steps, batch=64, n_train=pairs - held, lr=1e-3, seed=seed)
+    # Chunks of 100 steps (Adam state carries over between calls) so the
+    # run reports progress; each chunk samples batches from its own seed.
+    losses = []
+    for c, done in enumerate(range(0, train_steps, 100)):
+        n = min(100, train_steps - done)
+        losses += list(net.train(steps=n, batch=64, n_train=pairs - held, lr=1e-3, seed=seed * 1000 + c))
+        log(f"surrogate: {done + n}/{train_steps} training steps, last-{n} mean loss "
+            f"{float(np.mean(losses[-n:])):.3e}, {time.perf_counter() - t1:.1f} s")
     t2 = time.perf_counter()
     held_mse = net.eval(pairs - held, held)
     xh, yh = net.dataset(pairs - held, held)
     a, b = Sim(d, seed), Sim(d, seed)
     a.step(50)
     b.step(50)
+    log(f"surrogate: held-out MSE {held_mse:.3e}; advancing {steps} steps at {d.width}^2")
     rep = net.advance(a, steps, judge_every=16, tol=1e-2)
     t3 = time.perf_counter()
     b.step(steps)
@@ -245,7 +326,10 @@ def main() -> int:
     ap.add_argument("--surrogate-train-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
+    ap.add_argument("--torch-leg", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.torch_leg:
+        return torch_leg(args.torch_leg)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}). "
               "A batch of worlds shards by world and one large world by slab FFT, but neither is built.",
@@ -265,6 +349,7 @@ def main() -> int:
     sims = {}
     for path in ("fft_fp32", "direct_bf16"):
         d, sim, r = time_path(path, args.side, args.steps, args.warmup, args.seed)
+        log(f"{path}: {r['value']:.3e} cell-updates/s, {r['ms_per_step']:.4f} ms/step")
         w, h = d.width, d.height
         kern = r["kernels"]
         if path == "fft_fp32":
@@ -309,17 +394,24 @@ def main() -> int:
                   for p, v in results.items()},
         "direct_over_fft": results["direct_bf16"]["value"] / results["fft_fp32"]["value"],
     }
+    tf = time_torch_fft(args.side, 200, args.seed)
+    log(f"torch rfft2/irfft2 step: {tf['value']:.3e} cell-updates/s")
+    tf["engine_fft_over_torch"] = results["fft_fp32"]["value"] / tf["value"]
+    out["torch_rocfft"] = tf
     if not args.no_batch:
         out["batch"] = time_batch(args.seed)
+        log(f"batch: {out['batch']['value']:.3e} cell-updates/s")
     mc = {}
     for path in ("fft_fp32", "direct_bf16"):
         dm, sm, rm = time_multichannel(path, 200, args.seed)
+        log(f"multichannel {path}: {rm['value']:.3e} cell-updates/s")
         mc[path] = rm
         sims["mc_" + path] = (dm, sm)
     out["multichannel"] = mc
     if args.surrogate:
         out["surrogate"] = time_surrogate(args.seed, args.surrogate_train_steps)
     if not args.no_cpu and rank == 0:
+        log("cpu baseline and FP64 checks")
         base, err = cpu_leg(*sims["fft_fp32"], args.cpu_seconds)
         out["cpu_baseline"] = base
         out["err_vs_fp64_one_step"] = {"fft_fp32": err}

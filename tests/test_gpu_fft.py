@@ -38,7 +38,8 @@ def test_reference_known_answer(O):
     assert np.abs(got - O.step_ref32(d, s0)).max() < 2e-5
 
 
-@pytest.mark.parametrize("w,h", [(16, 16), (32, 64), (64, 32), (128, 128), (256, 64), (512, 1024), (2048, 256)])
+@pytest.mark.parametrize("w,h", [(16, 16), (32, 64), (64, 32), (128, 128), (256, 64), (512, 1024), (2048, 256),
+                                 (64, 4096), (2048, 4096)])
 def test_conv_matches_fp64(O, w, h):
     """The convolution alone (the reference's conv0 field) vs FP64, random state."""
     radius = min(w, h) / 2 - 1.5
