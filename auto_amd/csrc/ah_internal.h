@@ -7,6 +7,7 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/auto_hip.h"
@@ -62,6 +63,9 @@ int direct_terms(const std::vector<double>& taps, float dt, float weight, float 
 // Per-kernel launch timing on one stream (ah_sim_profile).
 struct Profiler {
     bool enabled = false;
+    int every = 1;  // time every `every`-th launch of each kernel name
+    std::vector<std::pair<const char*, uint64_t>> seen;  // launches per name
+    bool sample(const char* name);
     struct Rec {
         const char* name;
         hipEvent_t a, b;

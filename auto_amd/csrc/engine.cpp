@@ -38,6 +38,14 @@ void Profiler::reset() {
         pool.push_back(r.b);
     }
     recs.clear();
+    seen.clear();
+}
+
+bool Profiler::sample(const char* name) {
+    for (auto& e : seen)
+        if (std::strcmp(e.first, name) == 0) return e.second++ % (uint64_t)every == 0;
+    seen.push_back({name, 1});
+    return true;
 }
 
 Profiler::~Profiler() {
@@ -46,7 +54,7 @@ Profiler::~Profiler() {
 }
 
 Launch::Launch(Profiler* p_, hipStream_t s_, const char* n_) : p(p_), s(s_), name(n_) {
-    if (p && p->enabled) {
+    if (p && p->enabled && p->sample(name)) {
         b = p->get();
         AH_HIP(hipEventRecord(b, s));
     }
@@ -612,8 +620,10 @@ ah_status ah_sim_profile(ah_sim* s, int enable) {
     return guard([&] {
         if (!s) throw Error{AH_ERR_INVALID, "null sim"};
         AH_HIP(hipStreamSynchronize(s->stream));
+        if (enable < 0) throw Error{AH_ERR_INVALID, "enable must be >= 0"};
         s->prof.reset();
         s->prof.enabled = enable != 0;
+        s->prof.every = enable > 1 ? enable : 1;
     });
 }
 

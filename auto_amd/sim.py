@@ -105,8 +105,11 @@ class Sim:
         return s
 
     # -- kernel timing ----------------------------------------------------
-    def profile(self, enable: bool = True) -> None:
-        check(self._lib.ah_sim_profile(self._h, int(enable)))
+    def profile(self, enable: bool = True, every: int = 1) -> None:
+        """Time kernel launches with HIP events: every launch, or every
+        `every`-th launch of each kernel (keeps the events' cost out of the
+        step rate).  Enabling resets the counters."""
+        check(self._lib.ah_sim_profile(self._h, max(1, int(every)) if enable else 0))
 
     def profile_read(self) -> dict:
         cap = 32

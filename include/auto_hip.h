@@ -154,11 +154,15 @@ void ah_sim_destroy(ah_sim* sim);
 const char* ah_last_error(void);
 
 /*
- * Kernel timing.  While enabled, every kernel the sim launches is bracketed
- * by HIP events on the sim's stream; ah_sim_profile_read returns per-kernel
- * launch counts and summed device milliseconds (it synchronises).  Enabling
- * resets the counters.  This replaces the reference's unused TIMESTAMP_QUERY
- * (gpu.rs:83) and complements the wall-clock method of bench.rs:126-158.
+ * Kernel timing.  While enabled, kernels the sim launches are bracketed by
+ * HIP events on the sim's stream; ah_sim_profile_read returns per-kernel
+ * counts of timed launches and their summed device milliseconds (it
+ * synchronises).  enable = 1 times every launch; enable = N > 1 times every
+ * N-th launch of each kernel, which keeps the events' own cost (a few
+ * microseconds per recorded pair) out of the step rate.  0 turns timing off.
+ * Enabling resets the counters.  This replaces the reference's unused
+ * TIMESTAMP_QUERY (gpu.rs:83) and complements the wall-clock method of
+ * bench.rs:126-158.
  */
 typedef struct {
     char name[40];

@@ -193,3 +193,20 @@ def test_field_errors():
     buf = np.zeros(4, np.float32)
     assert sim._lib.ah_sim_read_field(sim._h, b"state", buf.ctypes.data, buf.nbytes) == 5  # AH_ERR_SIZE
     assert b"bytes" in sim._lib.ah_last_error()
+
+
+def test_profile_counts_sampled_launches():
+    """profile(every=N) times every N-th launch of each kernel; a negative
+    setting is rejected with the C error code."""
+    from auto_amd._abi import ERR_INVALID
+    sim = Sim(W.rings3_4096(side=512), 2)
+    sim.profile(True, every=4)
+    sim.step(40)
+    p = sim.profile_read()
+    assert p["col_conv"][0] == 10 and p["row_c2r_update_r2c"][0] == 10
+    assert p["row_r2c"][0] == 1  # only the first step transforms the state itself
+    assert all(ms > 0 for _, ms in p.values())
+    sim.profile(True)
+    sim.step(3)
+    assert sim.profile_read()["col_conv"][0] == 3
+    assert sim._lib.ah_sim_profile(sim._h, -1) == ERR_INVALID
