@@ -197,10 +197,11 @@ constexpr int FFT_THREADS = 512;  // upper bound on the workgroup size
 // Z[M-k])/2, O = -i/2 (Z[k] - conj Z[M-k]), w = exp(-2*pi*i/W).  Output
 // float4 q holds bins (kx, kx+1) of one row: kb = q/(2*rows), row, pair.
 __device__ __forceinline__ void store_half_spectrum(const float2* lds, int M, int rows, int H, int y0,
-                                                    const float2* __restrict__ twW, float2* __restrict__ spec) {
+                                                    const float2* __restrict__ twW, float2* __restrict__ spec,
+                                                    int nt) {
     float4* dst = reinterpret_cast<float4*>(spec);
     const int n4 = rows * M / 2, lg2r = ilog2(2 * rows);
-    for (int q = threadIdx.x; q < n4; q += blockDim.x) {
+    for (int q = threadIdx.x; q < n4; q += nt) {
         int kb = q >> lg2r, rem = q & (2 * rows - 1);
         int row = rem >> 1, kx = 4 * kb + 2 * (rem & 1);
         int off = row * M;
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_r2c(int W, int H, int rows,
     }
     lds_sync();
     fft_lds<E, -1>(lds, M, rows, twW, W);  // table stride W/M = 2 folds into NT = W
-    store_half_spectrum(lds, M, rows, H, y0, twW, spec);
+    store_half_spectrum(lds, M, rows, H, y0, twW, spec, blockDim.x);
 }
 
 // ------------------------------------------------------------ pass 2
@@ -646,7 +647,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     if constexpr (NEXT) {
         lds_sync();
         fft_lds<E, -1>(lds, M, rows, a.twW, W);
-        store_half_spectrum(lds, M, rows, H, y0, a.twW, a.spec_next + (size_t)ch * M * H);
+        store_half_spectrum(lds, M, rows, H, y0, a.twW, a.spec_next + (size_t)ch * M * H, blockDim.x);
     }
 }
 

@@ -210,3 +210,28 @@ def test_profile_counts_sampled_launches():
     sim.step(3)
     assert sim.profile_read()["col_conv"][0] == 3
     assert sim._lib.ah_sim_profile(sim._h, -1) == ERR_INVALID
+
+
+@pytest.mark.parametrize("h", [64, 256])
+def test_wide_world_steps_and_resume(O, h):
+    """4096-wide worlds run the streamed row pass: every step vs FP64 from the
+    same input, and a run resumed from a saved state (whose first step
+    transforms the state in pass 1) is bitwise equal to the uninterrupted run
+    (whose spectra come from the previous step's pass 3)."""
+    d = W.desc(4096, h, [dict(W.RINGS3)], 0.1, init=W.INIT_BLOBS, init_size=48, init_count=16)
+    a = Sim(d, 5)
+    khats = O.kernel_hats64(d)
+    s = a.read_field("state")
+    worst = 0.0
+    for _ in range(4):
+        a.step(1)
+        got = a.read_field("state")
+        worst = max(worst, np.abs(got - O.step64(d, s, "fft", khats)).max())
+        s = got
+    print(f"4096x{h} worst per-step max-abs {worst:.3e}")
+    assert worst < 1e-5
+    b = Sim(d, 77)
+    b.write_field("state", s)
+    a.step(6)
+    b.step(6)
+    assert np.array_equal(a.read_field("state"), b.read_field("state"))
