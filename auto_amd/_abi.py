@@ -30,6 +30,18 @@ INIT_ORBIUM = 0
 INIT_PATCH = 1
 INIT_BLOBS = 2
 INIT_ZERO = 3
+INIT_DISCS = 4
+INIT_NOISE = 5
+
+SHAPE_LENIA = 0
+SHAPE_SMOOTH_DISC = 1
+SHAPE_SMOOTH_RING = 2
+SHAPE_BAND = 3
+
+RULE_LENIA = 0
+RULE_SMOOTHLIFE = 1
+RULE_MNCA = 2
+RULES = {"lenia": RULE_LENIA, "smoothlife": RULE_SMOOTHLIFE, "mnca": RULE_MNCA}
 
 PATHS = {"fft_fp32": PATH_FFT_FP32, "direct_bf16": PATH_DIRECT_BF16,
          "direct_bf16_x1": PATH_DIRECT_BF16_X1, "direct_bf16_x3": PATH_DIRECT_BF16_X3}
@@ -45,6 +57,9 @@ class Kernel(ctypes.Structure):
         ("weight", ctypes.c_float),
         ("src", ctypes.c_uint32),
         ("dst", ctypes.c_uint32),
+        ("shape", ctypes.c_uint32),
+        ("inner", ctypes.c_float),
+        ("aa", ctypes.c_float),
     ]
 
 
@@ -60,6 +75,7 @@ class LeniaDesc(ctypes.Structure):
         ("init", ctypes.c_uint32),
         ("init_size", ctypes.c_uint32),
         ("init_count", ctypes.c_uint32),
+        ("rule", ctypes.c_uint32),
     ]
 
 

@@ -126,9 +126,10 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 // conv_out != nullptr: write kernel conv_k's potential there instead of updating the state.
 // spec_next != nullptr (update only): also write the half spectra of the updated
 // state there, which is what launch_row_r2c would produce from it next.
+// rule: ah_rule applied to the potentials.
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
-                    float* conv_out, int conv_k, float2* spec_next);
+                    float* conv_out, int conv_k, float2* spec_next, int rule);
 size_t fft_lds_limit();
 
 // direct_bf16.hip

@@ -109,6 +109,7 @@ void validate_world(const ah_lenia_desc& d, uint32_t i) {
     if (d.width < 64 || d.height < 64 || d.width > 8192 || d.height > 8192) bad("width and height 64..8192");
     if (d.channels != 1 || d.n_kernels != 1) bad("batched worlds are single-channel with one kernel");
     const ah_kernel& k = d.kernels[0];
+    if (d.rule != AH_RULE_LENIA || k.shape != AH_SHAPE_LENIA) bad("batched worlds run the Lenia rule and shape");
     if (!(k.radius > 0.0f) || !std::isfinite(k.radius)) bad("kernel radius must be positive");
     const int reach = (int)std::ceil((double)k.radius);
     if (2 * reach >= (int)std::min(d.width, d.height)) bad("world smaller than its kernel");

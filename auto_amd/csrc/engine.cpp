@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -222,6 +223,13 @@ namespace {
 bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 // Reject what the reference asserts on (spectral.rs:103-116) and what this build cannot run.
+// Largest |dx|, |dy| a kernel touches: its profile is 0 beyond.
+int kernel_reach(const ah_kernel& k) {
+    if (k.shape == AH_SHAPE_SMOOTH_DISC || k.shape == AH_SHAPE_SMOOTH_RING)
+        return (int)std::ceil((double)k.radius + (double)k.aa);
+    return (int)std::ceil((double)k.radius);
+}
+
 void validate(const ah_lenia_desc& d) {
     auto bad = [](const std::string& m) { throw Error{AH_ERR_INVALID, m}; };
     if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
@@ -230,19 +238,34 @@ void validate(const ah_lenia_desc& d) {
     if (d.n_kernels < 1 || d.n_kernels > AH_MAX_KERNELS) bad("n_kernels must be 1..9");
     if (!std::isfinite(d.dt) || d.dt <= 0.0f) bad("dt must be a positive finite number");
     const uint32_t side = std::min(d.width, d.height);
-    float wsum[AH_MAX_CHANNELS] = {0, 0, 0};
+    if (d.rule > AH_RULE_MNCA) bad("unknown rule");
     for (uint32_t k = 0; k < d.n_kernels; ++k) {
         const ah_kernel& kd = d.kernels[k];
+        if (kd.shape > AH_SHAPE_BAND) bad("unknown kernel shape");
         if (!(kd.radius > 0.0f) || !std::isfinite(kd.radius)) bad("kernel radius must be positive");
-        int reach = (int)std::ceil((double)kd.radius);
+        if (kd.shape == AH_SHAPE_SMOOTH_DISC || kd.shape == AH_SHAPE_SMOOTH_RING)
+            if (!(kd.aa > 0.0f) || !std::isfinite(kd.aa)) bad("antialiasing width must be positive");
+        if (kd.shape == AH_SHAPE_SMOOTH_RING || kd.shape == AH_SHAPE_BAND)
+            if (!(kd.inner >= 0.0f) || !(kd.inner < kd.radius)) bad("inner radius must be in [0, radius)");
+        const int reach = kernel_reach(kd);
         if (2 * reach >= (int)side) bad("world smaller than its kernel");
-        if (reach > 127) throw Error{AH_ERR_UNSUPPORTED, "kernel radius above 127 cells"};
-        if (kd.rings < 1 || kd.rings > AH_MAX_RINGS) bad("rings must be 1..3");
-        if (!(kd.sigma > 0.0f) || !std::isfinite(kd.mu)) bad("sigma must be positive, mu finite");
-        if (!(kd.weight > 0.0f)) bad("kernel weight must be positive");
+        if (reach > 127) throw Error{AH_ERR_UNSUPPORTED, "kernel reach above 127 cells"};
+        if (kd.shape == AH_SHAPE_LENIA && (kd.rings < 1 || kd.rings > AH_MAX_RINGS)) bad("rings must be 1..3");
+        if (d.rule == AH_RULE_LENIA) {
+            if (!(kd.sigma > 0.0f) || !std::isfinite(kd.mu)) bad("sigma must be positive, mu finite");
+            if (!(kd.weight > 0.0f)) bad("kernel weight must be positive");
+        }
         if (kd.src >= d.channels || kd.dst >= d.channels) bad("kernel src/dst out of range");
-        wsum[kd.dst] += kd.weight;
     }
+    if (d.rule != AH_RULE_LENIA) {
+        // The reference's SmoothLife and MNCA programs: one channel, two kernels (spectral.rs:354-480).
+        if (d.channels != 1 || d.n_kernels != 2) bad("SmoothLife and MNCA rules take one channel and two kernels");
+        if (d.path != AH_PATH_FFT_FP32)
+            throw Error{AH_ERR_UNSUPPORTED, "SmoothLife and MNCA rules run on the FFT path"};
+    }
+    for (uint32_t k = 0; k < d.n_kernels; ++k)
+        if (d.path != AH_PATH_FFT_FP32 && d.kernels[k].shape != AH_SHAPE_LENIA)
+            throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path takes Lenia-shaped kernels"};
     if (d.path > AH_PATH_DIRECT_BF16_X3) bad("unknown path");
     if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
         throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
@@ -253,12 +276,11 @@ void validate(const ah_lenia_desc& d) {
             if (std::ceil((double)d.kernels[k].radius) > 31)
                 throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path supports kernel radius <= 31"};
     }
-    if (d.init > AH_INIT_ZERO) bad("unknown init");
+    if (d.init > AH_INIT_NOISE) bad("unknown init");
     if (d.init == AH_INIT_ORBIUM && d.channels != 1) bad("Orbium init is single-channel");
     if (d.init == AH_INIT_PATCH && (d.init_size == 0 || d.init_size > side)) bad("patch size must be 1..side");
     if (d.init == AH_INIT_BLOBS && (d.init_size == 0 || d.init_size > side || d.init_count > 65536))
         bad("blob size must be 1..side and count <= 65536");
-    (void)wsum;
 }
 
 
@@ -344,9 +366,10 @@ void setup(ah_sim* s) {
     for (int k = 0; k < K; ++k) wsum[d.kernels[k].dst] += d.kernels[k].weight;
     for (int k = 0; k < K; ++k) {
         const ah_kernel& kd = d.kernels[k];
-        s->kp[k] = KernelParams{kd.mu, 1.0f / kd.sigma, kd.weight / wsum[kd.dst], kd.src, kd.dst};
+        if (d.rule == AH_RULE_LENIA) s->kp[k] = KernelParams{kd.mu, 1.0f / kd.sigma, kd.weight / wsum[kd.dst], kd.src, kd.dst};
+        else s->kp[k] = KernelParams{0.0f, 0.0f, 0.0f, kd.src, kd.dst};
         s->ksrc[k] = (int)kd.src;
-        s->reach[k] = (int)std::ceil((double)kd.radius);
+        s->reach[k] = kernel_reach(kd);
     }
 
     // FP64 taps of every kernel (scratch for set-up).
@@ -438,7 +461,7 @@ void step_fft(ah_sim* s, float* conv_out, int conv_k) {
     s->spec_ready = false;
     launch_col_conv(s->stream, &s->prof, W, H, C, s->spec_src, s->spec_k, s->khat, s->kpack, s->twH, s->ksrc, K);
     launch_row_c2r(s->stream, &s->prof, W, H, C, K, s->spec_k, s->state, s->state, s->kp, d.dt, s->twW, conv_out,
-                   conv_k, s->spec_src);
+                   conv_k, s->spec_src, (int)d.rule);
     s->spec_ready = conv_out == nullptr;
 }
 
@@ -483,7 +506,6 @@ const char* ah_last_error(void) { return ah::g_error.c_str(); }
 ah_status ah_lenia_build(const char* id, uint32_t width, uint32_t height, ah_lenia_desc* out) {
     return guard([&] {
         if (!id || !out) throw Error{AH_ERR_INVALID, "null argument"};
-        if (std::strcmp(id, "lenia") != 0) throw Error{AH_ERR_INVALID, std::string("unknown automaton id: ") + id};
         // automata::fit_size rounds to powers of two (mod.rs:51-61).
         auto p2 = [](uint32_t v) {
             uint32_t r = 1;
@@ -494,12 +516,63 @@ ah_status ah_lenia_build(const char* id, uint32_t width, uint32_t height, ah_len
         d.width = p2(width);
         d.height = p2(height);
         d.channels = 1;
-        d.n_kernels = 1;
-        // lenia(w, h, 13.0) (spectral.rs:286-352,489).
-        d.kernels[0] = ah_kernel{13.0f, 1, {1.0f, 0.0f, 0.0f}, 0.15f, 0.015f, 1.0f, 0, 0};
-        d.dt = 0.1f;
         d.path = AH_PATH_FFT_FP32;
-        d.init = AH_INIT_ORBIUM;
+        auto ring = [](float lo, float hi) {
+            ah_kernel k{};
+            k.shape = AH_SHAPE_BAND;
+            k.inner = lo;
+            k.radius = hi;
+            k.weight = 1.0f;
+            return k;
+        };
+        if (std::strcmp(id, "lenia") == 0) {
+            // lenia(w, h, 13.0) (spectral.rs:286-352,489).
+            d.n_kernels = 1;
+            d.kernels[0] = ah_kernel{13.0f, 1, {1.0f, 0.0f, 0.0f}, 0.15f, 0.015f, 1.0f, 0, 0, AH_SHAPE_LENIA, 0.0f, 0.0f};
+            d.dt = 0.1f;
+            d.init = AH_INIT_ORBIUM;
+            d.rule = AH_RULE_LENIA;
+        } else if (std::strcmp(id, "smoothlife") == 0) {
+            // smoothlife(w, h, 21.0) (spectral.rs:354-413,499): ri = ra/3, aa = 0.5.
+            const float ra = 21.0f, ri = ra / 3.0f, aa = 0.5f;
+            d.n_kernels = 2;
+            d.kernels[0] = ah_kernel{};
+            d.kernels[0].shape = AH_SHAPE_SMOOTH_DISC;
+            d.kernels[0].radius = ri;
+            d.kernels[0].aa = aa;
+            d.kernels[0].weight = 1.0f;
+            d.kernels[1] = ah_kernel{};
+            d.kernels[1].shape = AH_SHAPE_SMOOTH_RING;
+            d.kernels[1].radius = ra;
+            d.kernels[1].inner = ri;
+            d.kernels[1].aa = aa;
+            d.kernels[1].weight = 1.0f;
+            d.dt = 0.1f;
+            d.init = AH_INIT_DISCS;
+            d.rule = AH_RULE_SMOOTHLIFE;
+        } else if (std::strcmp(id, "mnca") == 0 || std::strncmp(id, "mnca.", 5) == 0) {
+            // mnca(w, h) = mnca_shaped(w, h, [12, 18, 3, 8]) (spectral.rs:415-428);
+            // "mnca.i0,o0,i1,o1" gives the rings (mod.rs:86-92).
+            float v[4] = {12.0f, 18.0f, 3.0f, 8.0f};
+            if (id[4] == '.') {
+                const char* p = id + 5;
+                for (int i = 0; i < 4; ++i) {
+                    char* end = nullptr;
+                    v[i] = std::strtof(p, &end);
+                    if (end == p || (i < 3 ? *end != ',' : *end != 0))
+                        throw Error{AH_ERR_INVALID, std::string("mnca ids are mnca.i0,o0,i1,o1: ") + id};
+                    p = end + 1;
+                }
+            }
+            d.n_kernels = 2;
+            d.kernels[0] = ring(v[0], v[1]);
+            d.kernels[1] = ring(v[2], v[3]);
+            d.dt = 1.0f;  // unused by the rule
+            d.init = AH_INIT_NOISE;
+            d.rule = AH_RULE_MNCA;
+        } else {
+            throw Error{AH_ERR_INVALID, std::string("unknown automaton id: ") + id};
+        }
         *out = d;
     });
 }

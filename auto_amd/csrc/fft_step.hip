@@ -580,12 +580,40 @@ __device__ void load_c2r(float2* lds, const RowArgs& a, int k, int y0) {
     lds_sync();
 }
 
+// Rafler's SmoothLife transition (spectral.rs:393-405): c0 is the inner-disc
+// average, c1 the outer-ring average; mix(a, b, t) = a (1 - t) + b t.
+__device__ __forceinline__ float smoothlife_next(float s, float c0, float c1, float dt) {
+    constexpr float B1 = 0.278f, B2 = 0.365f, D1 = 0.267f, D2 = 0.445f, AN = 0.028f, AM = 0.147f;
+    auto sig = [](float x, float a, float alpha) { return 1.0f / (1.0f + __expf((a - x) * 4.0f / alpha)); };
+    const float m = sig(c0, 0.5f, AM);
+    const float lo = B1 * (1.0f - m) + D1 * m, hi = B2 * (1.0f - m) + D2 * m;
+    const float target = sig(c1, lo, AN) * (1.0f - sig(c1, hi, AN));
+    return fminf(fmaxf(s + dt * (2.0f * target - 1.0f), 0.0f), 1.0f);
+}
+
+// MNCA bands (spectral.rs:462-473): each band that holds overrules the ones
+// before it; the state is kept, set to 1 or set to 0.
+__device__ __forceinline__ float mnca_next(float s, float c0, float c1) {
+    auto band = [](float v, float lo, float hi) { return v >= lo && v <= hi; };
+    float o = s;
+    if (band(c0, 0.210f, 0.220f)) o = 1.0f;
+    if (band(c0, 0.350f, 0.500f)) o = 0.0f;
+    if (band(c0, 0.750f, 0.850f)) o = 0.0f;
+    if (band(c0, 0.120f, 0.150f)) o = 0.0f;
+    if (band(c1, 0.100f, 0.280f)) o = 0.0f;
+    if (band(c1, 0.430f, 0.550f)) o = 1.0f;
+    return o;
+}
+
+// RULE: AH_RULE_LENIA sums each kernel's growth into its destination;
+// SmoothLife and MNCA keep the two potentials of their single channel and
+// apply their transition to them.
 // NEXT: the updated rows are also the next step's pass-1 input, so they go
 // back into LDS and through the forward real FFT here, and the state is read
 // once per step instead of twice.  In place when spec_next aliases spec: a
 // workgroup reads and writes only its own rows, and every read of them is
 // behind the barrier at the end of the kernel loop.
-template <int E, bool NEXT>
+template <int E, bool NEXT, int RULE>
 __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     extern __shared__ float2 lds[];
     const int W = a.W, H = a.H, M = W >> 1, rows = a.rows;
@@ -593,7 +621,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     const int ch = blockIdx.y;  // destination channel
     const int nt = blockDim.x, tid = threadIdx.x;
     constexpr int NQ = E / 2;  // float4 (4 output cells) per thread
-    float acc[NQ][4];
+    float acc[NQ][4];                                  // growth sum, or potential 0
+    float pot1[RULE == AH_RULE_LENIA ? 1 : NQ][4];     // potential 1 (SmoothLife, MNCA)
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
 #pragma unroll
@@ -611,11 +640,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
             if (a.conv_out) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = u[j];
-            } else {
+            } else if constexpr (RULE == AH_RULE_LENIA) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     float d = (u[j] - kp.mu) * kp.inv_sigma;
                     acc[i][j] += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (k == 0) acc[i][j] = u[j];
+                    else pot1[i][j] = u[j];
                 }
             }
         }
@@ -637,7 +672,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
         float4 s = s4[q];
         float r[4] = {s.x, s.y, s.z, s.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) r[j] = fminf(fmaxf(r[j] + a.dt * acc[i][j], 0.0f), 1.0f);
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (RULE == AH_RULE_SMOOTHLIFE) r[j] = smoothlife_next(r[j], acc[i][j], pot1[i][j], a.dt);
+            else if constexpr (RULE == AH_RULE_MNCA) r[j] = mnca_next(r[j], acc[i][j], pot1[i][j]);
+            else r[j] = fminf(fmaxf(r[j] + a.dt * acc[i][j], 0.0f), 1.0f);
+        }
         o4[q] = make_float4(r[0], r[1], r[2], r[3]);
         if constexpr (NEXT) {
             lds[pad(2 * q)] = make_float2(r[0], r[1]);
@@ -743,7 +782,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
-                    float* conv_out, int conv_k, float2* spec_next) {
+                    float* conv_out, int conv_k, float2* spec_next, int rule) {
     RowArgs a;
     a.W = W;
     a.H = H;
@@ -761,15 +800,19 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     const int elems = a.rows * (W / 2);
     const size_t lds = padded(elems) * sizeof(float2);
     const int gy = conv_out ? 1 : channels;
-    if (a.spec_next) {
-        Launch L(p, s, "row_c2r_update_r2c");
-        set_lds(k_row_c2r<8, true>, lds);
-        hipLaunchKernelGGL((k_row_c2r<8, true>), dim3(H / a.rows, gy), dim3(elems / 8), lds, s, a);
-    } else {
-        Launch L(p, s, conv_out ? "row_c2r_conv" : "row_c2r_update");
-        set_lds(k_row_c2r<8, false>, lds);
-        hipLaunchKernelGGL((k_row_c2r<8, false>), dim3(H / a.rows, gy), dim3(elems / 8), lds, s, a);
-    }
+    const dim3 grid(H / a.rows, gy), block(elems / 8);
+    if (!conv_out && rule != AH_RULE_LENIA && !a.spec_next)
+        throw Error{AH_ERR_INVALID, "SmoothLife and MNCA steps always write the next spectra"};
+    auto go = [&](auto kernel, const char* name) {
+        Launch L(p, s, name);
+        set_lds(kernel, lds);
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
+    };
+    if (conv_out) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_conv");
+    else if (!a.spec_next) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_update");
+    else if (rule == AH_RULE_SMOOTHLIFE) go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
+    else if (rule == AH_RULE_MNCA) go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
+    else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");
     AH_HIP(hipGetLastError());
 }
 

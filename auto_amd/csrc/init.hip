@@ -3,15 +3,17 @@
 //  - twiddle tables: exp(-2*pi*i*t/n) evaluated in FP64, rounded to FP32
 //    (the reference forms cos/sin of an f32 angle per butterfly instead,
 //    crates/auto-core/src/automata/spectral.rs:47-48);
-//  - kernel taps: Chan's multi-ring shell of exp-bump cores, normalised to
-//    sum 1 in FP64 (the reference's core and box sum: spectral.rs:75-83,
-//    288-303; one ring with beta={1} is exactly its kernel);
+//  - kernel taps: Chan's multi-ring shell of exp-bump cores, SmoothLife's
+//    antialiased disc and ring, or an MNCA band, normalised to sum 1 in FP64
+//    (the reference's shapes and box sum: spectral.rs:75-83,288-303,357-361,
+//    432-440; one ring with beta={1} is exactly its Lenia kernel);
 //  - kernel spectrum for the FFT path: the kernel is even in x and in y, so
 //    its DFT is real and separable into two cosine sums, done in FP64 and
 //    stored in FP32 already divided by W*H (which the reference applies after
 //    the inverse transform instead, spectral.rs:215,227-234);
-//  - seeds: the Orbium lattice (spectral.rs:183-197,313-335) and the
-//    patch/blob inits, all from the PCG hash of program.rs:140-147;
+//  - seeds: the Orbium lattice (spectral.rs:183-197,313-335), SmoothLife's
+//    discs (:381-391), MNCA's noise (:456-460) and the patch/blob inits, all
+//    from the PCG hash of program.rs:140-147;
 //  - statistics: FP64 mass, min, max and non-finite count per channel, with
 //    wavefront shuffles and a fixed-order second pass (bitwise reproducible).
 #include "ah_internal.h"
@@ -69,6 +71,10 @@ struct ShellArgs {
     int rings;
     double beta[AH_MAX_RINGS];
     int reach;
+    int shape;      // ah_shape
+    double inner;   // SMOOTH_RING, BAND
+    double aa;      // SMOOTH_DISC, SMOOTH_RING
+    float radius_f, inner_f;  // BAND compares a float32 distance, as the reference does
 };
 
 __device__ double shell64(double r, const ShellArgs& a) {
@@ -81,6 +87,26 @@ __device__ double shell64(double r, const ShellArgs& a) {
     return a.beta[k] * exp(4.0 - 1.0 / (q * (1.0 - q)));
 }
 
+__device__ __forceinline__ double clamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
+
+// Radial profile at offset (dx, dy).  The smooth SmoothLife shapes are
+// spectral.rs:357-361 in FP64; the MNCA band keeps the reference's float32
+// distance and comparisons (spectral.rs:432-440) so its support is the same
+// set of cells.
+__device__ double profile64(int dx, int dy, const ShellArgs& a) {
+    const double r = sqrt((double)(dx * dx + dy * dy));
+    switch (a.shape) {
+        case AH_SHAPE_SMOOTH_DISC: return clamp01((a.radius + a.aa - r) / (2.0 * a.aa));
+        case AH_SHAPE_SMOOTH_RING:
+            return clamp01((a.radius + a.aa - r) / (2.0 * a.aa)) * clamp01((r - a.inner + a.aa) / (2.0 * a.aa));
+        case AH_SHAPE_BAND: {
+            const float rf = sqrtf((float)(dx * dx + dy * dy));
+            return (rf >= a.inner_f && rf < a.radius_f) ? 1.0 : 0.0;
+        }
+        default: return shell64(r, a);
+    }
+}
+
 // One workgroup: taps of the (2*reach+1)^2 box, FP64 sum (fixed order), normalise.
 __global__ void __launch_bounds__(256) k_kernel_taps64(ShellArgs a, double* taps, double* sum_out) {
     __shared__ double part[256];
@@ -88,7 +114,7 @@ __global__ void __launch_bounds__(256) k_kernel_taps64(ShellArgs a, double* taps
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) {
         int dy = i / side - a.reach, dx = i % side - a.reach;
-        double v = shell64(sqrt((double)(dx * dx + dy * dy)), a);
+        double v = profile64(dx, dy, a);
         taps[i] = v;
         acc += v;
     }
@@ -174,6 +200,16 @@ __global__ void k_seed(SeedArgs a, float* state) {
     } else if (a.init == AH_INIT_PATCH) {
         uint32_t x0 = (a.W - a.size) / 2, y0 = (a.H - a.size) / 2;
         if (x >= x0 && x < x0 + a.size && y >= y0 && y < y0 + a.size) v = unit_f(rand3(x, y, a.seed + c));
+    } else if (a.init == AH_INIT_DISCS) {
+        // spectral.rs:381-391: a disc in every 96-cell tile whose hash is odd.
+        const uint32_t tile = 96u;
+        if (rand3(x / tile, y / tile, a.seed) & 1u) {
+            float dx = (float)(x % tile) - (float)tile / 2.0f, dy = (float)(y % tile) - (float)tile / 2.0f;
+            v = (dx * dx + dy * dy < 24.0f * 24.0f) ? 1.0f : 0.0f;
+        }
+    } else if (a.init == AH_INIT_NOISE) {
+        // spectral.rs:456-460: 3x3 blocks alive with probability 0.32.
+        v = unit_f(rand3(x / 3u, y / 3u, a.seed)) < 0.32f ? 1.0f : 0.0f;
     } else if (a.init == AH_INIT_BLOBS) {
         for (uint32_t b = 0; b < a.count; ++b) {
             uint32_t dx = (x + a.W - a.corners[2 * b]) % a.W, dy = (y + a.H - a.corners[2 * b + 1]) % a.H;
@@ -307,6 +343,11 @@ void launch_kernel_taps64(hipStream_t s, Profiler* p, const ah_kernel& k, int re
     a.rings = (int)k.rings;
     for (int i = 0; i < AH_MAX_RINGS; ++i) a.beta[i] = (double)k.beta[i];
     a.reach = reach;
+    a.shape = (int)k.shape;
+    a.inner = (double)k.inner;
+    a.aa = (double)k.aa;
+    a.radius_f = k.radius;
+    a.inner_f = k.inner;
     Launch L(p, s, "kernel_taps64");
     hipLaunchKernelGGL(k_kernel_taps64, dim3(1), dim3(256), 0, s, a, taps, sum);
     AH_HIP(hipGetLastError());

@@ -16,7 +16,8 @@ with u01(x) = (x >> 11) * 2^-53.
 """
 from __future__ import annotations
 
-from ._abi import (LeniaDesc, INIT_ORBIUM, INIT_PATCH, INIT_BLOBS, PATHS, MAX_KERNELS)
+from ._abi import (LeniaDesc, INIT_ORBIUM, INIT_PATCH, INIT_BLOBS, INIT_DISCS, INIT_NOISE, PATHS, MAX_KERNELS,
+                   SHAPE_SMOOTH_DISC, SHAPE_SMOOTH_RING, SHAPE_BAND, RULES)
 
 MASK64 = (1 << 64) - 1
 
@@ -46,7 +47,8 @@ def _pow2(n: int) -> int:
 
 
 def desc(width: int, height: int, kernels: list[dict], dt: float, path: str = "fft_fp32",
-         channels: int = 1, init: int = INIT_PATCH, init_size: int = 0, init_count: int = 0) -> LeniaDesc:
+         channels: int = 1, init: int = INIT_PATCH, init_size: int = 0, init_count: int = 0,
+         rule: str = "lenia") -> LeniaDesc:
     d = LeniaDesc()
     d.width, d.height, d.channels = width, height, channels
     if not 1 <= len(kernels) <= MAX_KERNELS:
@@ -55,16 +57,18 @@ def desc(width: int, height: int, kernels: list[dict], dt: float, path: str = "f
     for i, k in enumerate(kernels):
         kd = d.kernels[i]
         kd.radius = k["radius"]
-        beta = list(k.get("beta", [1.0]))
+        beta = list(k.get("beta", [1.0] if k.get("shape", 0) == 0 else []))
         kd.rings = len(beta)
         for j, b in enumerate(beta):
             kd.beta[j] = b
-        kd.mu, kd.sigma = k["mu"], k["sigma"]
+        kd.mu, kd.sigma = k.get("mu", 0.0), k.get("sigma", 0.0)
         kd.weight = k.get("weight", 1.0)
         kd.src, kd.dst = k.get("src", 0), k.get("dst", 0)
+        kd.shape, kd.inner, kd.aa = k.get("shape", 0), k.get("inner", 0.0), k.get("aa", 0.0)
     d.dt = dt
     d.path = PATHS[path] if isinstance(path, str) else int(path)
     d.init, d.init_size, d.init_count = init, init_size, init_count
+    d.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     return d
 
 
@@ -72,6 +76,26 @@ def lenia_reference(width: int, height: int, path: str = "fft_fp32") -> LeniaDes
     """The reference "lenia" entry (spectral.rs:482-490 -> lenia(w, h, 13.0))."""
     return desc(_pow2(width), _pow2(height), [dict(radius=13.0, mu=0.15, sigma=0.015)], 0.1,
                 path, init=INIT_ORBIUM)
+
+
+def smoothlife(width: int, height: int, ra: float = 21.0) -> LeniaDesc:
+    """The reference "smoothlife" entry (spectral.rs:354-413, built with
+    ra = 21 at :499): inner disc of radius ra/3 and the ring out to ra, both
+    antialiased over 0.5 cells, dt = 0.1, scattered-disc seed."""
+    ri, aa = ra / 3.0, 0.5
+    return desc(_pow2(width), _pow2(height),
+                [dict(shape=SHAPE_SMOOTH_DISC, radius=ri, aa=aa),
+                 dict(shape=SHAPE_SMOOTH_RING, radius=ra, inner=ri, aa=aa)], 0.1,
+                init=INIT_DISCS, rule="smoothlife")
+
+
+def mnca(width: int, height: int, rings=(12.0, 18.0, 3.0, 8.0)) -> LeniaDesc:
+    """The reference "mnca" entry (spectral.rs:415-480; rings [12,18) and
+    [3,8) by default, :427), noise seed."""
+    i0, o0, i1, o1 = (float(v) for v in rings)
+    return desc(_pow2(width), _pow2(height),
+                [dict(shape=SHAPE_BAND, radius=o0, inner=i0), dict(shape=SHAPE_BAND, radius=o1, inner=i1)], 1.0,
+                init=INIT_NOISE, rule="mnca")
 
 
 def single_ring_256(path: str = "fft_fp32") -> LeniaDesc:

@@ -141,6 +141,23 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
             "measure": m}
 
 
+def time_rule(rule: str, side: int, steps: int, seed: int):
+    """SmoothLife (ra = 21) or MNCA (rings [12,18), [3,8)) on the FFT path:
+    two kernels, so the column pass writes two product spectra and the row
+    pass inverts both before the transition."""
+    from auto_amd import Sim, workloads as W
+    d = W.smoothlife(side, side) if rule == "smoothlife" else W.mnca(side, side)
+    sim = Sim(d, seed)
+    sim.step(10)
+    sim.wait()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.wait()
+    dt = time.perf_counter() - t0
+    return {"value": d.width * d.height * steps / dt, "unit": "cell-updates/s", "ms_per_step": 1e3 * dt / steps,
+            "workload": f"{rule} {d.width}x{d.height}, reference parameters, {steps} steps"}
+
+
 def time_multichannel(path: str, steps: int, seed: int):
     """1024^2 x 3 channels, 9 kernels (k = 3*src + dst) drawn from SplitMix64."""
     from auto_amd import Sim, workloads as W
@@ -414,6 +431,11 @@ def main() -> int:
         mc[path] = rm
         sims["mc_" + path] = (dm, sm)
     out["multichannel"] = mc
+    rules = {}
+    for rule in ("smoothlife", "mnca"):
+        rules[rule] = time_rule(rule, args.side, 200, args.seed)
+        log(f"{rule}: {rules[rule]['value']:.3e} cell-updates/s")
+    out["rules"] = rules
     if args.surrogate:
         out["surrogate"] = time_surrogate(args.seed, args.surrogate_train_steps)
     if not args.no_cpu and rank == 0:

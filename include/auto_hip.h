@@ -47,7 +47,7 @@ enum { AH_MAX_CHANNELS = 3, AH_MAX_KERNELS = 9, AH_MAX_RINGS = 3 };
 /* How the periodic convolution is computed.  The direct paths need width
  * and height >= 64 and kernel radius <= 31. */
 typedef enum {
-    AH_PATH_FFT_FP32 = 0,       /* fused three-pass FP32 real FFT                        */
+    AH_PATH_FFT_FP32 = 0,       /* fused FP32 real FFT, two passes per step              */
     AH_PATH_DIRECT_BF16 = 1,    /* direct convolution on BF16 MFMA, FP32 accumulate; the
                                    engine picks 1 or 3 bf16 terms per kernel (below)      */
     AH_PATH_DIRECT_BF16_X1 = 2, /* force one term: bf16(K) * bf16(s)                      */
@@ -66,22 +66,58 @@ typedef enum {
     AH_INIT_ORBIUM = 0, /* reference Orbium lattice (spectral.rs:313-335)             */
     AH_INIT_PATCH = 1,  /* centred init_size^2 patch, unit_f(rand3(x,y,seed+c))       */
     AH_INIT_BLOBS = 2,  /* init_count blobs of init_size^2, corners from SplitMix64    */
-    AH_INIT_ZERO = 3
+    AH_INIT_ZERO = 3,
+    AH_INIT_DISCS = 4,  /* SmoothLife seed: a radius-24 disc of 1s in each 96x96 tile
+                           whose rand3(tx,ty,seed) is odd (spectral.rs:381-391)       */
+    AH_INIT_NOISE = 5   /* MNCA seed: 1 where unit_f(rand3(x/3,y/3,seed)) < 0.32
+                           (spectral.rs:456-460)                                      */
 } ah_init;
 
+/* Radial profile of a kernel, as a function of the distance r in cells. */
+typedef enum {
+    AH_SHAPE_LENIA = 0,       /* `rings` exp-bump rings over [0, radius) with weights beta
+                                 (spectral.rs:288-303; Chan's multi-ring form)          */
+    AH_SHAPE_SMOOTH_DISC = 1, /* clamp((radius + aa - r) / (2 aa), 0, 1): SmoothLife's
+                                 inner disc (spectral.rs:357-358)                       */
+    AH_SHAPE_SMOOTH_RING = 2, /* clamp((radius + aa - r) / (2 aa), 0, 1) *
+                                 clamp((r - inner + aa) / (2 aa), 0, 1): its outer ring
+                                 (spectral.rs:359-361)                                  */
+    AH_SHAPE_BAND = 3         /* 1 for inner <= r < radius, r = sqrt in float32: an MNCA
+                                 ring (spectral.rs:432-440)                             */
+} ah_shape;
+
+/* What a step does with the kernels' potentials. */
+typedef enum {
+    AH_RULE_LENIA = 0,      /* s += dt * sum_k h_k (2 exp(-((u_k - mu_k)/sigma_k)^2 / 2) - 1),
+                               clipped to [0, 1] (spectral.rs:337-345)                  */
+    AH_RULE_SMOOTHLIFE = 1, /* Rafler's transition on u_0 (inner) and u_1 (outer) with
+                               B1=.278 B2=.365 D1=.267 D2=.445 AN=.028 AM=.147
+                               (spectral.rs:393-405); FFT path, one channel, 2 kernels */
+    AH_RULE_MNCA = 2        /* threshold bands on u_0 and u_1, each overruling the one
+                               before (spectral.rs:462-473); FFT path, one channel,
+                               2 kernels                                                */
+} ah_rule;
+
 /*
- * One convolution kernel: a radial shell of `rings` exp-bump rings with ring
- * weights beta (Chan's multi-ring Lenia; rings = 1, beta = {1} is exactly the
- * reference's core, spectral.rs:288-303), normalised to sum 1 on the torus,
- * with its own Gaussian growth (mu, sigma) and mixing weight into `dst`.
+ * One convolution kernel: a radial profile (`shape`), normalised to sum 1 on
+ * the torus as the reference divides by its box sum (spectral.rs:75-83,
+ * 150-171).  For AH_SHAPE_LENIA: `rings` exp-bump rings with ring weights
+ * beta (rings = 1, beta = {1} is exactly the reference's core,
+ * spectral.rs:288-303); mu, sigma and weight give its Gaussian growth and
+ * mixing weight into `dst` under AH_RULE_LENIA.  The reach of a kernel (the
+ * largest |dx|, |dy| it touches) is ceil(radius) for LENIA and BAND and
+ * ceil(radius + aa) for the smooth shapes; 2*reach < min(width, height).
  */
 typedef struct {
-    float radius;             /* R in cells, 2*ceil(R) < min(width, height)   */
-    uint32_t rings;           /* 1..AH_MAX_RINGS                              */
-    float beta[AH_MAX_RINGS]; /* ring weights                                 */
-    float mu, sigma;          /* growth centre and width                      */
+    float radius;             /* R (LENIA), outer radius (SMOOTH_*, BAND)     */
+    uint32_t rings;           /* 1..AH_MAX_RINGS (LENIA)                      */
+    float beta[AH_MAX_RINGS]; /* ring weights (LENIA)                         */
+    float mu, sigma;          /* growth centre and width (RULE_LENIA)         */
     float weight;             /* mixing weight h; normalised per dst at create */
     uint32_t src, dst;        /* source and destination channel               */
+    uint32_t shape;           /* ah_shape                                     */
+    float inner;              /* inner radius (SMOOTH_RING, BAND)             */
+    float aa;                 /* antialiasing half-width (SMOOTH_*), > 0      */
 } ah_kernel;
 
 typedef struct {
@@ -94,6 +130,7 @@ typedef struct {
     uint32_t init;            /* ah_init                                      */
     uint32_t init_size;       /* patch side (PATCH) or blob side (BLOBS)      */
     uint32_t init_count;      /* blob count (BLOBS)                           */
+    uint32_t rule;            /* ah_rule                                      */
 } ah_lenia_desc;
 
 typedef struct {
@@ -119,7 +156,11 @@ typedef struct ah_sim ah_sim;
  * Fill `out` with a registry rule, as automata::build(id, w, h) does
  * (mod.rs:63-95): sizes are rounded up to powers of two (mod.rs:51-61).
  * Known ids: "lenia" (the reference rule: R=13, one ring, mu=0.15,
- * sigma=0.015, dt=0.1, Orbium seed; spectral.rs:286-352,482-490).
+ * sigma=0.015, dt=0.1, Orbium seed; spectral.rs:286-352,482-490);
+ * "smoothlife" (ra=21, ri=ra/3, aa=0.5, dt=0.1, disc seed;
+ * spectral.rs:354-413,492-500); "mnca" (rings [12,18) and [3,8), noise
+ * seed; spectral.rs:415-480,502-510) and "mnca.<i0>,<o0>,<i1>,<o1>" with
+ * rings [i0,o0) and [i1,o1) (mod.rs:86-92).
  */
 ah_status ah_lenia_build(const char* id, uint32_t width, uint32_t height, ah_lenia_desc* out);
 

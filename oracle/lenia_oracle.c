@@ -8,19 +8,23 @@
  * Contents (each function cites the reference code it restates):
  *   1. Integer generators: PCG hash, rand3, unit_f (program.rs:140-147) and
  *      SplitMix64 (crates/auto-life-gpu/src/world.rs:153-159).
- *   2. Seeds: the Orbium lattice (spectral.rs:183-197,287,313-335) and the
+ *   2. Seeds: the Orbium lattice (spectral.rs:183-197,287,313-335),
+ *      SmoothLife's discs (:381-391), MNCA's noise (:456-460) and the
  *      patch / blob inits of the benchmark workloads.
  *   3. Kernels: the Lenia core (spectral.rs:288-303) generalised to Chan's
- *      multi-ring shell, normalised in FP64; the reference's own FP32 kernel
+ *      multi-ring shell, SmoothLife's disc and ring (:357-361) and MNCA's
+ *      bands (:432-440), normalised in FP64; the reference's own FP32 kernel
  *      pass and box sum (spectral.rs:75-83,150-171,296).
  *   4. The reference's FP32 step, pass by pass: radix-2 Stockham stages with
  *      per-butterfly FP32 twiddles (spectral.rs:14-64), load / fft x / fft y /
  *      multiply / ifft y / ifft x / scale / update (spectral.rs:199-264,
- *      337-345).  This is the "auto-cpu" Lenia step the reference lacks
+ *      337-345), with the SmoothLife and MNCA transitions (:393-405,462-473)
+ *      in f32.  This is the "auto-cpu" Lenia step the reference lacks
  *      (crates/auto-cpu/src/lib.rs:1-5) and the CPU baseline.
  *   5. FP64 direct periodic convolution and growth/update, as the reference's
  *      known-answer test computes it (crates/auto-core/tests/automata.rs:317-351),
- *      generalised to any kernel, several channels and mixing weights.
+ *      generalised to any kernel, several channels and mixing weights, and the
+ *      SmoothLife / MNCA transitions in FP64.
  *   6. FP64 statistics (mass, min, max, non-finite count).
  *
  * How each part is pinned (tests/test_oracle.py):
@@ -32,7 +36,11 @@
  *     convolution (two independent methods), and a uniform field convolves to
  *     itself (kernel sums to 1).
  *   - Kernel: closed-form peaks (core(1/2)=1; ring k peaks at beta_k),
- *     support r<R, 8-fold symmetry, FP64 normalisation.
+ *     support r<R, 8-fold symmetry, FP64 normalisation; SmoothLife shapes:
+ *     the box sum tends to the disc / annulus area; MNCA band: lattice-point
+ *     counts by brute force.
+ *   - SmoothLife / MNCA transitions: closed-form cases (birth interval,
+ *     survival interval, band overrides) and f32 vs FP64 agreement.
  *   - PCG/SplitMix64: SplitMix64's published first outputs for seed 0;
  *     PCG: bijectivity on samples and the Orbium KAT (a seed in 1..63 places
  *     a creature at 128x64).  PCG's bit values vs the reference's WGSL are
@@ -156,6 +164,22 @@ int or_init_state(const ah_lenia_desc* d, uint32_t seed, float* out) {
                         out[c * cells + (size_t)y * w + x] = or_unit_f(or_rand3(x, y, seed + c));
                     }
         }
+    } else if (d->init == AH_INIT_DISCS) {
+        /* SmoothLife seed(), spectral.rs:381-391 (every channel alike). */
+        const uint32_t tile = 96u;
+        for (uint32_t c = 0; c < C; ++c)
+            for (uint32_t y = 0; y < h; ++y)
+                for (uint32_t x = 0; x < w; ++x) {
+                    if ((or_rand3(x / tile, y / tile, seed) & 1u) == 0u) continue;
+                    float dx = (float)(x % tile) - (float)tile / 2.0f, dy = (float)(y % tile) - (float)tile / 2.0f;
+                    out[c * cells + (size_t)y * w + x] = (dx * dx + dy * dy < 24.0f * 24.0f) ? 1.0f : 0.0f;
+                }
+    } else if (d->init == AH_INIT_NOISE) {
+        /* MNCA seed(), spectral.rs:456-460. */
+        for (uint32_t c = 0; c < C; ++c)
+            for (uint32_t y = 0; y < h; ++y)
+                for (uint32_t x = 0; x < w; ++x)
+                    out[c * cells + (size_t)y * w + x] = or_unit_f(or_rand3(x / 3u, y / 3u, seed)) < 0.32f ? 1.0f : 0.0f;
     } else if (d->init != AH_INIT_ZERO) {
         return -1;
     }
@@ -189,19 +213,47 @@ double or_shell64(double r, double R, uint32_t rings, const float* beta) {
     return (double)beta[k] * or_core64(b - (double)k);
 }
 
-/* Half-width of the box that holds every tap: ceil(R). */
+/* Half-width of the box that holds every tap of a Lenia shell: ceil(R). */
 int or_reach(float R) { return (int)ceil((double)R); }
+
+/* The same for any shape: the smooth SmoothLife shapes reach radius + aa. */
+int or_kernel_reach(const ah_kernel* k) {
+    if (k->shape == AH_SHAPE_SMOOTH_DISC || k->shape == AH_SHAPE_SMOOTH_RING)
+        return (int)ceil((double)k->radius + (double)k->aa);
+    return or_reach(k->radius);
+}
+
+static double clamp01d(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
+
+/*
+ * Radial profile of any kernel shape at offset (dx, dy).  SmoothLife's inner
+ * disc and outer ring, spectral.rs:357-361, in FP64; the MNCA band of
+ * spectral.rs:432-440 with the reference's float32 distance and comparisons.
+ */
+double or_profile64(const ah_kernel* k, int dx, int dy) {
+    const double r = sqrt((double)(dx * dx + dy * dy));
+    const double R = k->radius, ri = k->inner, aa = k->aa;
+    switch (k->shape) {
+        case AH_SHAPE_SMOOTH_DISC: return clamp01d((R + aa - r) / (2.0 * aa));
+        case AH_SHAPE_SMOOTH_RING: return clamp01d((R + aa - r) / (2.0 * aa)) * clamp01d((r - ri + aa) / (2.0 * aa));
+        case AH_SHAPE_BAND: {
+            float rf = sqrtf((float)(dx * dx + dy * dy));
+            return (rf >= k->inner && rf < k->radius) ? 1.0 : 0.0;
+        }
+        default: return or_shell64(r, R, k->rings, k->beta);
+    }
+}
 
 /*
  * FP64 kernel taps in a (2*reach+1)^2 box, normalised to sum 1.
  * taps[(dy+reach)*(2*reach+1) + (dx+reach)].  Returns the raw sum.
  */
 double or_kernel_box64(const ah_kernel* k, double* taps) {
-    int reach = or_reach(k->radius), side = 2 * reach + 1;
+    int reach = or_kernel_reach(k), side = 2 * reach + 1;
     double sum = 0.0;
     for (int dy = -reach; dy <= reach; ++dy)
         for (int dx = -reach; dx <= reach; ++dx) {
-            double v = or_shell64(sqrt((double)(dx * dx + dy * dy)), (double)k->radius, k->rings, k->beta);
+            double v = or_profile64(k, dx, dy);
             taps[(dy + reach) * side + (dx + reach)] = v;
             sum += v;
         }
@@ -294,7 +346,7 @@ void or_kernel_spectrum_ref32(uint32_t w, uint32_t h, float R, float* kern) {
  * normalised shell rounded to f32 on the torus (min-image), then fft x, fft y.
  */
 void or_kernel_spectrum32(uint32_t w, uint32_t h, const ah_kernel* kd, float* kern) {
-    int reach = or_reach(kd->radius), side = 2 * reach + 1;
+    int reach = or_kernel_reach(kd), side = 2 * reach + 1;
     double* taps = (double*)malloc(sizeof(double) * side * side);
     or_kernel_box64(kd, taps);
     memset(kern, 0, sizeof(float) * 2 * (size_t)w * h);
@@ -350,9 +402,43 @@ static inline float clamp01f(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0
  * reference update (spectral.rs:236-264,337-345).  `kerns` holds n_kernels
  * spectra of w*h complex f32 (or_kernel_spectrum32).
  */
+/* SmoothLife transition, spectral.rs:393-405, in f32 as the WGSL does. */
+static float sig32(float x, float a, float alpha) { return 1.0f / (1.0f + expf((a - x) * 4.0f / alpha)); }
+float or_smoothlife32(float s, float c0, float c1, float dt) {
+    const float B1 = 0.278f, B2 = 0.365f, D1 = 0.267f, D2 = 0.445f, AN = 0.028f, AM = 0.147f;
+    float m = sig32(c0, 0.5f, AM);
+    float lo = B1 * (1.0f - m) + D1 * m, hi = B2 * (1.0f - m) + D2 * m;
+    float target = sig32(c1, lo, AN) * (1.0f - sig32(c1, hi, AN));
+    float v = s + dt * (2.0f * target - 1.0f);
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+/* MNCA bands, spectral.rs:462-473: later bands overrule earlier ones. */
+static int band32(float v, float lo, float hi) { return v >= lo && v <= hi; }
+float or_mnca32(float s, float c0, float c1) {
+    float o = s;
+    if (band32(c0, 0.210f, 0.220f)) o = 1.0f;
+    if (band32(c0, 0.350f, 0.500f)) o = 0.0f;
+    if (band32(c0, 0.750f, 0.850f)) o = 0.0f;
+    if (band32(c0, 0.120f, 0.150f)) o = 0.0f;
+    if (band32(c1, 0.100f, 0.280f)) o = 0.0f;
+    if (band32(c1, 0.430f, 0.550f)) o = 1.0f;
+    return o;
+}
+
 void or_step_ref32(const ah_lenia_desc* d, const float* kerns, const float* s, float* out) {
     const uint32_t w = d->width, h = d->height, C = d->channels, K = d->n_kernels;
     const size_t cells = (size_t)w * h;
+    if (d->rule != AH_RULE_LENIA) {
+        /* SmoothLife / MNCA: one channel, u0 and u1 through the FP32 passes. */
+        float* u = (float*)malloc(sizeof(float) * cells * 2);
+        for (uint32_t k = 0; k < 2; ++k) or_conv_ref32(w, h, kerns + 2 * cells * k, s, u + cells * k);
+        for (size_t i = 0; i < cells; ++i)
+            out[i] = d->rule == AH_RULE_SMOOTHLIFE ? or_smoothlife32(s[i], u[i], u[cells + i], d->dt)
+                                                   : or_mnca32(s[i], u[i], u[cells + i]);
+        free(u);
+        return;
+    }
     float hsum[AH_MAX_CHANNELS] = {0};
     for (uint32_t k = 0; k < K; ++k) hsum[d->kernels[k].dst] += d->kernels[k].weight;
     float* acc = (float*)calloc(cells * C, sizeof(float));
@@ -375,7 +461,7 @@ void or_step_ref32(const ah_lenia_desc* d, const float* kerns, const float* s, f
 
 /* u[y][x] = sum over the box of K(dx,dy) * s[(y+dy) mod h][(x+dx) mod w], FP64. */
 void or_conv_direct64(uint32_t w, uint32_t h, const ah_kernel* kd, const float* s, double* u) {
-    int reach = or_reach(kd->radius), side = 2 * reach + 1;
+    int reach = or_kernel_reach(kd), side = 2 * reach + 1;
     double* taps = (double*)malloc(sizeof(double) * side * side);
     or_kernel_box64(kd, taps);
 #pragma omp parallel for schedule(static)
@@ -395,10 +481,39 @@ void or_conv_direct64(uint32_t w, uint32_t h, const ah_kernel* kd, const float* 
     free(taps);
 }
 
+/* SmoothLife transition in FP64 (constants as the WGSL's f32 literals). */
+static double sig64(double x, double a, double alpha) { return 1.0 / (1.0 + exp((a - x) * 4.0 / alpha)); }
+double or_smoothlife64(double s, double c0, double c1, double dt) {
+    const double B1 = 0.278f, B2 = 0.365f, D1 = 0.267f, D2 = 0.445f, AN = 0.028f, AM = 0.147f;
+    double m = sig64(c0, 0.5, AM);
+    double lo = B1 * (1.0 - m) + D1 * m, hi = B2 * (1.0 - m) + D2 * m;
+    double target = sig64(c1, lo, AN) * (1.0 - sig64(c1, hi, AN));
+    return clamp01d(s + dt * (2.0 * target - 1.0));
+}
+
+/* MNCA bands in FP64 against the WGSL's f32 thresholds. */
+static int band64(double v, float lo, float hi) { return v >= (double)lo && v <= (double)hi; }
+double or_mnca64(double s, double c0, double c1) {
+    double o = s;
+    if (band64(c0, 0.210f, 0.220f)) o = 1.0;
+    if (band64(c0, 0.350f, 0.500f)) o = 0.0;
+    if (band64(c0, 0.750f, 0.850f)) o = 0.0;
+    if (band64(c0, 0.120f, 0.150f)) o = 0.0;
+    if (band64(c1, 0.100f, 0.280f)) o = 0.0;
+    if (band64(c1, 0.430f, 0.550f)) o = 1.0;
+    return o;
+}
+
 /* Growth/update in FP64 from given potentials, rounding the result to f32. */
 void or_update64(const ah_lenia_desc* d, const double* u, const float* s, float* out) {
     const size_t cells = (size_t)d->width * d->height;
     const uint32_t C = d->channels, K = d->n_kernels;
+    if (d->rule != AH_RULE_LENIA) {
+        for (size_t i = 0; i < cells; ++i)
+            out[i] = (float)(d->rule == AH_RULE_SMOOTHLIFE ? or_smoothlife64(s[i], u[i], u[cells + i], d->dt)
+                                                           : or_mnca64(s[i], u[i], u[cells + i]));
+        return;
+    }
     double hsum[AH_MAX_CHANNELS] = {0};
     for (uint32_t k = 0; k < K; ++k) hsum[d->kernels[k].dst] += d->kernels[k].weight;
     for (uint32_t c = 0; c < C; ++c)

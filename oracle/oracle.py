@@ -45,6 +45,12 @@ def lib() -> ctypes.CDLL:
             "or_core64": (ctypes.c_double, [ctypes.c_double]),
             "or_shell64": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _F]),
             "or_reach": (ctypes.c_int, [ctypes.c_float]),
+            "or_kernel_reach": (ctypes.c_int, [ctypes.POINTER(Kernel)]),
+            "or_profile64": (ctypes.c_double, [ctypes.POINTER(Kernel), ctypes.c_int, ctypes.c_int]),
+            "or_smoothlife32": (ctypes.c_float, [ctypes.c_float] * 4),
+            "or_smoothlife64": (ctypes.c_double, [ctypes.c_double] * 4),
+            "or_mnca32": (ctypes.c_float, [ctypes.c_float] * 3),
+            "or_mnca64": (ctypes.c_double, [ctypes.c_double] * 3),
             "or_kernel_box64": (ctypes.c_double, [ctypes.POINTER(Kernel), _D]),
             "or_kernel_sum_ref32": (ctypes.c_float, [ctypes.c_float]),
             "or_fft_axis_ref32": (None, [ctypes.c_uint32, ctypes.c_uint32, _F, _F, ctypes.c_int, ctypes.c_int]),
@@ -106,7 +112,7 @@ def init_state(desc: LeniaDesc, seed: int) -> np.ndarray:
 # ------------------------------------------------------------------- kernels
 
 def kernel_box64(k: Kernel) -> np.ndarray:
-    reach = lib().or_reach(k.radius)
+    reach = lib().or_kernel_reach(ctypes.byref(k))
     taps = np.zeros((2 * reach + 1, 2 * reach + 1), np.float64)
     lib().or_kernel_box64(ctypes.byref(k), _dp(taps))
     return taps

@@ -56,6 +56,40 @@ def test_registry_matches_reference_entry():
     assert b"unknown" in lib.ah_last_error()
 
 
+def test_registry_smoothlife_and_mnca():
+    """"smoothlife" and "mnca" as the reference builds them (spectral.rs:354-510),
+    and "mnca.i0,o0,i1,o1" custom rings (mod.rs:86-92)."""
+    lib = _abi.load()
+    d = _abi.LeniaDesc()
+    assert lib.ah_lenia_build(b"smoothlife", 300, 200, ctypes.byref(d)) == 0
+    assert bytes(d) == bytes(W.smoothlife(300, 200)) and (d.width, d.height) == (512, 256)
+    assert (d.kernels[0].radius, d.kernels[1].inner, d.kernels[1].radius) == (7.0, 7.0, 21.0)
+    assert lib.ah_lenia_build(b"mnca", 256, 256, ctypes.byref(d)) == 0
+    assert bytes(d) == bytes(W.mnca(256, 256))
+    assert lib.ah_lenia_build(b"mnca.10,15.5,2,6", 256, 256, ctypes.byref(d)) == 0
+    assert bytes(d) == bytes(W.mnca(256, 256, rings=(10, 15.5, 2, 6)))
+    for bad in (b"mnca.1,2", b"mnca.1,2,3,x", b"mnca.", b"mnca.1,2,3,4,5"):
+        assert lib.ah_lenia_build(bad, 64, 64, ctypes.byref(d)) == _abi.ERR_INVALID, bad
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (lambda d: setattr(d, "rule", 5), _abi.ERR_INVALID),
+    (lambda d: setattr(d, "n_kernels", 1), _abi.ERR_INVALID),
+    (lambda d: setattr(d, "path", _abi.PATH_DIRECT_BF16), _abi.ERR_UNSUPPORTED),
+    (lambda d: setattr(d.kernels[0], "shape", 9), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[1], "aa", 0.0), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[1], "inner", 30.0), _abi.ERR_INVALID),
+    (lambda d: setattr(d.kernels[1], "radius", 31.5), _abi.ERR_INVALID),  # reach 32: world smaller than its kernel
+])
+def test_invalid_smoothlife_descriptors(mutate, code):
+    lib = _abi.load()
+    d = W.smoothlife(64, 64)
+    mutate(d)
+    h = ctypes.c_void_p()
+    assert lib.ah_sim_new(ctypes.byref(d), 1, ctypes.byref(h)) == code
+    assert not h.value
+
+
 @pytest.mark.parametrize("mutate,code", [
     (lambda d: setattr(d, "width", 100), _abi.ERR_INVALID),
     (lambda d: setattr(d.kernels[0], "radius", 40.0), _abi.ERR_INVALID),  # world smaller than its kernel
