@@ -725,6 +725,18 @@ ah_status ah_sim_profile_read(ah_sim* s, ah_kernel_time* out, uint32_t capacity,
     });
 }
 
+ah_status ah_device_name(char* out, size_t capacity) {
+    return guard([&] {
+        if (!out || capacity == 0) throw Error{AH_ERR_INVALID, "null or empty buffer"};
+        int dev = 0;
+        AH_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        AH_HIP(hipGetDeviceProperties(&p, dev));
+        std::string name = std::string(p.name) + " (" + p.gcnArchName + ")";
+        std::snprintf(out, capacity, "%s", name.c_str());
+    });
+}
+
 ah_status ah_copy_probe(size_t bytes, uint32_t iters, double* ms_per_copy) {
     void* a = nullptr;
     void* b = nullptr;

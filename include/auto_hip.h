@@ -328,6 +328,13 @@ ah_status ah_surrogate_advance(ah_surrogate* s, ah_sim* sim, uint32_t steps, uin
  */
 ah_status ah_copy_probe(size_t bytes, uint32_t iters, double* ms_per_copy);
 
+/*
+ * Name of the current HIP device ("AMD Instinct MI355X" and its gfx target),
+ * the `adapter` a benchmark row records (the reference takes it from wgpu,
+ * crates/auto-cli/src/bench.rs:163).  Truncated to capacity-1 bytes.
+ */
+ah_status ah_device_name(char* out, size_t capacity);
+
 #ifdef __cplusplus
 }
 #endif
