@@ -172,6 +172,8 @@ _SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_uint32)]),
     "ah_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "ah_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "ah_sim_render": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t]),
     "ah_batch_new": (ctypes.c_int, [ctypes.POINTER(LeniaDesc), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(_P)]),
     "ah_batch_step": (ctypes.c_int, [_P, ctypes.c_uint32]),

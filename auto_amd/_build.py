@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["engine.cpp", "init.hip", "fft_step.hip", "direct_bf16.hip", "batch.cpp", "surrogate.hip", "surrogate.cpp"]
+SOURCES = ["engine.cpp", "init.hip", "fft_step.hip", "direct_bf16.hip", "batch.cpp", "surrogate.hip", "surrogate.cpp", "render.hip"]
 OUT = os.path.join(HERE, "libauto_hip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

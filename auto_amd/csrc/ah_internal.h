@@ -102,6 +102,10 @@ struct KernelParams {  // growth of one kernel, as the update kernels read it
     uint32_t src, dst;
 };
 
+// render.hip: rgba8 view of channel 0 (center, scale: cells per output pixel).
+void launch_render(hipStream_t s, Profiler* prof, const float* state, int W, int H, uint32_t rule, float center_x,
+                   float center_y, float scale, int out_w, int out_h, uint32_t* rgba);
+
 // ------------------------------------------------------------ launchers
 // init.hip
 void launch_cos_table64(hipStream_t s, Profiler* p, double* tab, int n);

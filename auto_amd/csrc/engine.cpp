@@ -648,6 +648,28 @@ ah_status ah_sim_write_field(ah_sim* s, const char* name, const void* src, size_
     });
 }
 
+ah_status ah_sim_render(ah_sim* s, uint32_t out_w, uint32_t out_h, float center_x, float center_y, float scale,
+                        uint8_t* rgba, size_t bytes) {
+    return guard([&] {
+        if (!s || !rgba) throw Error{AH_ERR_INVALID, "null argument"};
+        if (out_w == 0 || out_h == 0 || out_w > 16384 || out_h > 16384)
+            throw Error{AH_ERR_INVALID, "output size must be 1..16384"};
+        if (!(scale > 0.0f) || !std::isfinite(scale) || !std::isfinite(center_x) || !std::isfinite(center_y))
+            throw Error{AH_ERR_INVALID, "camera must be finite with a positive scale"};
+        const size_t n = (size_t)out_w * out_h * 4;
+        if (bytes != n) throw Error{AH_ERR_SIZE, "rgba buffer is out_w*out_h*4 bytes"};
+        AH_HIP(hipSetDevice(s->device));
+        uint32_t* d = nullptr;
+        AH_HIP(hipMallocAsync((void**)&d, n, s->stream));
+        launch_render(s->stream, &s->prof, s->state, (int)s->d.width, (int)s->d.height, s->d.rule, center_x,
+                      center_y, scale, (int)out_w, (int)out_h, d);
+        hipError_t e = hipMemcpyAsync(rgba, d, n, hipMemcpyDeviceToHost, s->stream);
+        (void)hipFreeAsync(d, s->stream);
+        AH_HIP(e);
+        AH_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
 ah_status ah_sim_stats(ah_sim* s, ah_stats* out) {
     return guard([&] {
         if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};

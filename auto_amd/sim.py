@@ -104,6 +104,20 @@ class Sim:
         check(self._lib.ah_sim_stats(self._h, ctypes.byref(s)))
         return s
 
+    # -- view -------------------------------------------------------------
+    def render(self, out_w: int, out_h: int, center=None, scale: float | None = None) -> np.ndarray:
+        """rgba8 [out_h][out_w][4] view of channel 0 (Sim::render, sim.rs:452-514).
+        Without a camera the world is fitted to the frame (Camera::fit,
+        sim.rs:52-61)."""
+        if scale is None:
+            scale = max(self.width / out_w, self.height / out_h)
+        if center is None:
+            center = (self.width / 2.0, self.height / 2.0)
+        out = np.empty((out_h, out_w, 4), np.uint8)
+        check(self._lib.ah_sim_render(self._h, out_w, out_h, float(center[0]), float(center[1]), float(scale),
+                                      out.ctypes.data, out.nbytes))
+        return out
+
     # -- kernel timing ----------------------------------------------------
     def profile(self, enable: bool = True, every: int = 1) -> None:
         """Time kernel launches with HIP events: every launch, or every

@@ -184,6 +184,16 @@ ah_status ah_sim_wait(ah_sim* sim);
 ah_status ah_sim_read_field(ah_sim* sim, const char* name, void* dst, size_t bytes);
 ah_status ah_sim_write_field(ah_sim* sim, const char* name, const void* src, size_t bytes);
 
+/*
+ * Sim::render (sim.rs:452-514) into host memory: channel 0 of the current
+ * state through the rule's colour ramp (spectral.rs:346-350,407-413,473-478),
+ * rgba8 [out_h][out_w][4], camera centre (center_x, center_y) in cells and
+ * `scale` cells per output pixel (Camera::fit, sim.rs:52-61: scale =
+ * max(width/out_w, height/out_h), centre = world centre).  Synchronises.
+ */
+ah_status ah_sim_render(ah_sim* sim, uint32_t out_w, uint32_t out_h, float center_x, float center_y, float scale,
+                        uint8_t* rgba, size_t bytes);
+
 /* Mass/min/max/non-finite reduction of the current state (synchronises). */
 ah_status ah_sim_stats(ah_sim* sim, ah_stats* out);
 
