@@ -36,7 +36,7 @@ def test_uniform_states_give_palette_stops():
         # within one unorm step: the ramp forms c_k + (c_k+1 - c_k) t in float32
         assert np.all(np.abs(img[..., :3].astype(int) - unorm(LENIA[stop]).astype(int)) <= 1), value
         assert np.all(img[..., 3] == 255)
-    img = sim_with(W.lenia_reference(256, 128), 0.0).render(16, 16)
+    img = sim_with(W.lenia_reference(256, 128), 0.0).render(32, 16)
     assert np.all(img[..., :3] == unorm(LENIA[0]))  # t = 0 is exact
     img = sim_with(W.mnca(256, 256), 0.5).render(32, 32)
     assert np.all(np.abs(img[..., :3].astype(int) - unorm((MNCA[0] + MNCA[1]) * np.float32(0.5)).astype(int)) <= 1)
