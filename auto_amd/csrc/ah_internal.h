@@ -113,8 +113,12 @@ void launch_twiddles(hipStream_t s, Profiler* p, float2* tw, int n);
 // Normalised FP64 taps of one kernel in a (2*reach+1)^2 box; writes taps and raw sum.
 void launch_kernel_taps64(hipStream_t s, Profiler* p, const ah_kernel& k, int reach, double* taps, double* sum);
 // khat/kpack for the FFT path from FP64 taps (separable cosine transform); T is scratch.
+// col_major: khat[kx][p] (for col_stream_ok worlds) instead of khat[kb][p][c].
 void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const double* taps, const double* cosW,
-                   const double* cosH, double* T, float* khat, float2* kpack);
+                   const double* cosH, double* T, float* khat, float2* kpack, bool col_major);
+// Whether the column pass of this world runs the streamed one-column kernel
+// (which reads khat column-major).
+bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc);
 void launch_seed(hipStream_t s, Profiler* p, const ah_lenia_desc& d, uint32_t seed, const uint32_t* corners,
                  float* state);
 int stats_blocks(size_t cells);

@@ -415,7 +415,8 @@ void setup(ah_sim* s) {
                 const int side = 2 * s->reach[k] + 1;
                 double* T = (double*)tmp(sizeof(double) * side * (W / 2 + 1));
                 launch_khat64(st, &s->prof, W, H, s->reach[k], taps[k], cosW, cosH, T,
-                              s->khat + (size_t)k * (W / 8) * np * BC, s->kpack + (size_t)k * np);
+                              s->khat + (size_t)k * (W / 8) * np * BC, s->kpack + (size_t)k * np,
+                              col_stream_ok(W, H, C, K, s->ksrc));
             }
         } else {
             setup_direct(s, taps);

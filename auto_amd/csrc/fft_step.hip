@@ -400,55 +400,65 @@ __device__ __forceinline__ void prefetch_sync() {
 // stage s+1, w = exp(-2*pi*i*k/(8*Ls)), k = t mod Ls, read once per kernel
 // from the FP64-built table; its powers are products (a few ulp).
 template <int SIGN>
-__device__ __forceinline__ void fft_column(float2* buf, const float2 (&tw)[3]) {
+__device__ __forceinline__ void twiddle_column(float2 (&v)[8], float2 w1) {
+    // Opaque copy: keeps the compiler from hoisting the powers of all stages
+    // out of the column loop, which would hold ~70 registers.
+    asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+    if (SIGN > 0) w1.y = -w1.y;
+    float2 w[8];
+    w[1] = w1;
+    w[2] = cmul(w[1], w[1]);
+    w[4] = cmul(w[2], w[2]);
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w[r]);
+}
+
+// A stage's loads: positions t + 512 r.
+__device__ __forceinline__ void column_load(const float2* buf, float2 (&v)[8]) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = buf[pad(threadIdx.x + r * ST)];
+}
+
+// Stage s's stores: base + r*Ls with k = t mod Ls, base = (t - k)*8 + k.
+__device__ __forceinline__ void column_store(float2* buf, const float2 (&v)[8], int s) {
+    const int t = threadIdx.x, Ls = 1 << (3 * s), k = t & (Ls - 1), base = (t - k) * 8 + k;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[pad(base + r * Ls)] = v[r];
+}
+
+template <int SIGN>
+__device__ __forceinline__ void column_stage(float2* buf, int s, const float2 (&tw)[3]) {
+    float2 v[8];
+    column_load(buf, v);
+    lds_sync();
+    if (s > 0) twiddle_column<SIGN>(v, tw[s - 1]);
+    dft8<SIGN>(v);
+    column_store(buf, v, s);
+    lds_sync();
+}
+
+// K-hat factors of the 8 rows this thread holds after the last forward
+// stage, y = t + 512 r: the kernel spectrum is real and even in ky, so row y
+// takes khat[p = min(y, H - y)], stored column-major for this kernel so a
+// wave's loads are contiguous.  Not used for the packed kx=0 column.
+__device__ __forceinline__ void load_khat(float (&kf)[8], const ColArgs& a, int col) {
     const int t = threadIdx.x;
+    constexpr int NP = SH / 2 + 1;
+    const float* kh = a.khat + (size_t)col * NP;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int Ls = 1 << (3 * s);
-        float2 v[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = buf[pad(t + r * ST)];
-        const int k = t & (Ls - 1);
-        const int base = (t - k) * 8 + k;
-        lds_sync();
-        if (s > 0) {
-            float2 w[8];
-            w[1] = tw[s - 1];
-            // Opaque copy: keeps the compiler from hoisting the powers of all
-            // stages out of the column loop, which would hold ~70 registers.
-            asm volatile("" : "+v"(w[1].x), "+v"(w[1].y));
-            if (SIGN > 0) w[1].y = -w[1].y;
-            w[2] = cmul(w[1], w[1]);
-            w[4] = cmul(w[2], w[2]);
-            w[3] = cmul(w[1], w[2]);
-            w[5] = cmul(w[1], w[4]);
-            w[6] = cmul(w[2], w[4]);
-            w[7] = cmul(w[3], w[4]);
-#pragma unroll
-            for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], w[r]);
-        }
-        dft8<SIGN>(v);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) buf[pad(base + r * Ls)] = v[r];
-        lds_sync();
+    for (int r = 0; r < 8; ++r) {
+        const int y = t + r * ST;
+        kf[r] = kh[min(y, SH - y)];
     }
 }
 
-// K-hat factors of the pairs (p, H-p) this thread multiplies: p = t + 512i,
-// i < 4, and thread 0 also owns p = H/2 (the fifth load of every other thread
-// repeats that one and goes unused).  Always issued, also for the packed
-// kx=0 column, whose (A, B) factors multiply_column reads itself, so the
-// number of global operations in flight does not depend on the column.
-__device__ __forceinline__ void load_khat(float (&kf)[5], const ColArgs& a, int col) {
-    const int t = threadIdx.x;
-    constexpr int NP = SH / 2 + 1;
-    const float* kh = a.khat + (size_t)(col >> 2) * NP * BC + (col & 3);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) kf[i] = kh[(size_t)(i < 4 ? t + i * ST : SH / 2) * BC];
-}
-
-__device__ __forceinline__ void multiply_column(float2* buf, const float (&kf)[5], const float2* __restrict__ kpack,
-                                                bool packed) {
+// The packed kx=0 column: D[ky] = A C[ky] + B conj C[-ky] needs row pairs
+// (p, H-p) that different threads hold, so it goes through LDS.
+__device__ __forceinline__ void multiply_packed_column(float2* buf, const float2* __restrict__ kpack) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -456,15 +466,9 @@ __device__ __forceinline__ void multiply_column(float2* buf, const float (&kf)[5
         const int p = t + i * ST;
         const int y1 = p, y2 = (SH - p) & (SH - 1);
         float2 c1 = buf[pad(y1)], c2 = buf[pad(y2)];
-        float2 d1, d2;
-        if (packed) {
-            const float2 ab = kpack[p];
-            d1 = cadd(make_float2(ab.x * c1.x, ab.x * c1.y), make_float2(ab.y * c2.x, -ab.y * c2.y));
-            d2 = cadd(make_float2(ab.x * c2.x, ab.x * c2.y), make_float2(ab.y * c1.x, -ab.y * c1.y));
-        } else {
-            d1 = make_float2(kf[i] * c1.x, kf[i] * c1.y);
-            d2 = make_float2(kf[i] * c2.x, kf[i] * c2.y);
-        }
+        const float2 ab = kpack[p];
+        float2 d1 = cadd(make_float2(ab.x * c1.x, ab.x * c1.y), make_float2(ab.y * c2.x, -ab.y * c2.y));
+        float2 d2 = cadd(make_float2(ab.x * c2.x, ab.x * c2.y), make_float2(ab.y * c1.x, -ab.y * c1.y));
         buf[pad(y1)] = d1;
         if (y2 != y1) buf[pad(y2)] = d2;
     }
@@ -480,11 +484,35 @@ __device__ __forceinline__ void store_column(const float2* buf, float2* __restri
     }
 }
 
+// Forward FFT, multiply, inverse FFT and write-back of one column.  After
+// the last forward stage thread t holds rows t + 512 r, which are exactly
+// the rows the first inverse stage (span 1) reads: the multiply and that
+// stage run in registers, saving one LDS round trip and two barriers.
 __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const float2 (&tw)[3],
-                                            const float (&kf)[5], int col) {
-    fft_column<-1>(buf, tw);
-    multiply_column(buf, kf, a.kpack, col == 0);
-    fft_column<1>(buf, tw);
+                                            const float (&kf)[8], int col) {
+    column_stage<-1>(buf, 0, tw);
+    column_stage<-1>(buf, 1, tw);
+    column_stage<-1>(buf, 2, tw);
+    float2 v[8];
+    column_load(buf, v);
+    lds_sync();
+    twiddle_column<-1>(v, tw[2]);
+    dft8<-1>(v);
+    if (col != 0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = make_float2(kf[r] * v[r].x, kf[r] * v[r].y);
+        dft8<1>(v);
+        column_store(buf, v, 0);
+        lds_sync();
+    } else {
+        column_store(buf, v, 3);
+        lds_sync();
+        multiply_packed_column(buf, a.kpack);
+        column_stage<1>(buf, 0, tw);
+    }
+    column_stage<1>(buf, 1, tw);
+    column_stage<1>(buf, 2, tw);
+    column_stage<1>(buf, 3, tw);
     store_column(buf, a.spec_out, col);
 }
 
@@ -506,7 +534,7 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
     }
     // K-hat for column i is loaded with column i's prefetch, during column
     // i-1, so both are complete at the wait that opens column i.
-    float kf[2][5];
+    float kf[2][8];
     load_khat(kf[0], a, col0);
     prefetch_column(bufA, a.spec_in, col0);
 #pragma unroll
@@ -711,6 +739,14 @@ int row_rows(int W, int H) {
     return rows;
 }
 
+}  // namespace
+
+bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
+    return channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == SH && W >= 64;
+}
+
+namespace {
+
 // Compute units of the current device (cached per device).
 int cu_count() {
     static int cached[64] = {};
@@ -757,7 +793,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.twH = twH;
     a.n_kernels = n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
-    if (channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == SH && W >= 64) {
+    if (col_stream_ok(W, H, channels, n_kernels, ksrc)) {
         // Up to four columns per workgroup, aiming at two workgroups per CU
         // (each holds 2 x 33 KiB of LDS).
         const int ncols = W / 2, slots = 2 * cu_count();
