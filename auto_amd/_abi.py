@@ -25,6 +25,8 @@ PATH_FFT_FP32 = 0
 PATH_DIRECT_BF16 = 1
 PATH_DIRECT_BF16_X1 = 2
 PATH_DIRECT_BF16_X3 = 3
+PATH_AUTO = 4
+BATCH_FFT_ABOVE = 60
 
 INIT_ORBIUM = 0
 INIT_PATCH = 1
@@ -44,7 +46,7 @@ RULE_MNCA = 2
 RULES = {"lenia": RULE_LENIA, "smoothlife": RULE_SMOOTHLIFE, "mnca": RULE_MNCA}
 
 PATHS = {"fft_fp32": PATH_FFT_FP32, "direct_bf16": PATH_DIRECT_BF16,
-         "direct_bf16_x1": PATH_DIRECT_BF16_X1, "direct_bf16_x3": PATH_DIRECT_BF16_X3}
+         "direct_bf16_x1": PATH_DIRECT_BF16_X1, "direct_bf16_x3": PATH_DIRECT_BF16_X3, "auto": PATH_AUTO}
 
 
 class Kernel(ctypes.Structure):
@@ -132,6 +134,8 @@ class BatchInfo(ctypes.Structure):
         ("generation", ctypes.c_uint64),
         ("n_tune", ctypes.c_uint32),
         ("tune", BatchTune * BATCH_MAX_CANDIDATES),
+        ("fft_worlds", ctypes.c_uint32),
+        ("fft_groups", ctypes.c_uint32),
     ]
 
 

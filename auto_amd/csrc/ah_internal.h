@@ -130,14 +130,17 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
                     const float2* twW);
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH, const int* ksrc,
-                     int n_kernels);
+                     int n_kernels, bool diag = false);
 // conv_out != nullptr: write kernel conv_k's potential there instead of updating the state.
 // spec_next != nullptr (update only): also write the half spectra of the updated
 // state there, which is what launch_row_r2c would produce from it next.
 // rule: ah_rule applied to the potentials.
+// kp_dev, dt_dev (device arrays, one entry per channel): batched same-size
+// worlds, channel c being world c with its own kernel c, growth and dt.
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
-                    float* conv_out, int conv_k, float2* spec_next, int rule);
+                    float* conv_out, int conv_k, float2* spec_next, int rule,
+                    const KernelParams* kp_dev = nullptr, const float* dt_dev = nullptr);
 size_t fft_lds_limit();
 
 // direct_bf16.hip

@@ -1,16 +1,19 @@
 // Batched uneven worlds: hundreds of single-channel Lenia worlds of different
-// sizes and rules advanced together, one persistent launch per step.
+// sizes, rules and paths advanced together.
 //
 // The reference runs one world per Sim (crates/auto-core/src/sim.rs:202-309);
-// it has no batching.  Here every world's state lives in one pair of
-// ping-pong buffers; a tile list (64 columns x 32*NB rows per tile, grouped by
-// world) is cut into contiguous segments of near-equal MFMA cost, one per
-// workgroup of the persistent direct-BF16 kernel (direct_bf16.hip
-// k_direct_batch).  Tile rows NB and the grid size are chosen by an autotuner
-// that times every admissible candidate on the batch itself at creation.
-// Steps run as hipGraph replays of `graph_steps` launches (an even count, so
-// the ping-pong parity is unchanged by a replay), with single eager launches
-// for the remainder.
+// it has no batching.  Direct-BF16 worlds share one pair of ping-pong
+// buffers; a tile list (grouped by world) is cut into contiguous segments of
+// near-equal MFMA cost, one per workgroup of the persistent kernel
+// (direct_bf16.hip k_direct_batch).  FFT worlds are grouped by size; a group
+// keeps its worlds as the channels of one state, one half spectrum and one
+// K-hat table per world, and steps as one column-pass and one row-pass launch
+// in diagonal mode (world c's kernel reads and writes channel c only, with
+// its own growth and dt).  The direct launch's tile shape and grid are chosen
+// by an autotuner that times every admissible candidate on the batch itself
+// at creation.  Steps run as hipGraph replays of `graph_steps` steps (an even
+// count, so the ping-pong parity is unchanged by a replay), with single eager
+// steps for the remainder.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,6 +24,21 @@
 #include "ah_internal.h"
 
 using namespace ah;
+
+// Same-size FFT worlds, one per channel.
+struct FftGroup {
+    int W = 0, H = 0, n = 0;
+    std::vector<int> worlds;     // batch world index of each channel
+    float* state = nullptr;      // [n][H][W], updated in place
+    float2* spec = nullptr;      // [n][W/8][H][4] half spectra, in place through the column pass
+    float* khat = nullptr;       // [n][W/8][H/2+1][4]
+    float2* kpack = nullptr;     // [n][H/2+1]
+    float2* twW = nullptr;
+    float2* twH = nullptr;
+    KernelParams* kp = nullptr;  // [n] growth per world
+    float* dt = nullptr;         // [n]
+    bool ready = false;          // spec holds the spectra of state
+};
 
 struct ah_batch {
     std::vector<ah_lenia_desc> d;
@@ -34,6 +52,9 @@ struct ah_batch {
     std::vector<BatchWorld> worlds_h;
     BatchWorld* worlds = nullptr;
     std::vector<int> cost;  // MFMA chunks per 16x16 output block, per world
+    std::vector<char> fft;  // world runs the FFT path
+    std::vector<int> group_of, slot_of;
+    std::vector<FftGroup> groups;
     int4* sched = nullptr;
     uint4* frags = nullptr;
     int halo_y = 0, halo_x = 0, nfrag_max = 0;
@@ -103,8 +124,7 @@ bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 void validate_world(const ah_lenia_desc& d, uint32_t i) {
     auto bad = [&](const std::string& m) { throw Error{AH_ERR_INVALID, "world " + std::to_string(i) + ": " + m}; };
-    if (d.path == AH_PATH_FFT_FP32 || d.path > AH_PATH_DIRECT_BF16_X3)
-        bad("batched worlds run the direct BF16 path (path must be one of the AH_PATH_DIRECT_BF16 values)");
+    if (d.path > AH_PATH_AUTO) bad("unknown path");
     if (!pow2(d.width) || !pow2(d.height)) bad("width and height must be powers of two");
     if (d.width < 64 || d.height < 64 || d.width > 8192 || d.height > 8192) bad("width and height 64..8192");
     if (d.channels != 1 || d.n_kernels != 1) bad("batched worlds are single-channel with one kernel");
@@ -113,7 +133,11 @@ void validate_world(const ah_lenia_desc& d, uint32_t i) {
     if (!(k.radius > 0.0f) || !std::isfinite(k.radius)) bad("kernel radius must be positive");
     const int reach = (int)std::ceil((double)k.radius);
     if (2 * reach >= (int)std::min(d.width, d.height)) bad("world smaller than its kernel");
-    if (reach > 31) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": radius above 31"};
+    const bool direct = d.path == AH_PATH_DIRECT_BF16 || d.path == AH_PATH_DIRECT_BF16_X1 || d.path == AH_PATH_DIRECT_BF16_X3;
+    if (direct && reach > 31) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": direct path radius above 31"};
+    if (reach > 127) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": radius above 127"};
+    if ((d.path == AH_PATH_FFT_FP32 || (d.path == AH_PATH_AUTO && reach > 31)) && d.height > 4096)
+        throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": FFT path height above 4096"};
     if (k.rings < 1 || k.rings > AH_MAX_RINGS) bad("rings must be 1..3");
     if (!(k.sigma > 0.0f) || !std::isfinite(k.mu)) bad("sigma must be positive, mu finite");
     if (!std::isfinite(d.dt) || d.dt <= 0.0f) bad("dt must be a positive finite number");
@@ -128,6 +152,7 @@ void build_tiling(ah_batch* b, int cols, int rows, int grid, std::vector<int4>& 
     tiles.clear();
     std::vector<double> tcost;
     for (size_t w = 0; w < b->d.size(); ++w) {
+        if (b->fft[w]) continue;
         const int W = (int)b->d[w].width, H = (int)b->d[w].height;
         for (int y0 = 0; y0 < H; y0 += rows)
             for (int x0 = 0; x0 < W; x0 += cols) {
@@ -169,16 +194,37 @@ void install_tiling(ah_batch* b, int cols, int rows, int grid) {
 }
 
 void launch_step(ah_batch* b, Profiler* p) {
-    launch_direct_batch(b->stream, p, b->args(b->parity), b->cols, b->rows, b->grid);
+    if (b->ntiles) launch_direct_batch(b->stream, p, b->args(b->parity), b->cols, b->rows, b->grid);
     b->parity ^= 1;
+    for (FftGroup& g : b->groups) {
+        launch_col_conv(b->stream, p, g.W, g.H, g.n, g.spec, g.spec, g.khat, g.kpack, g.twH, nullptr, 0, true);
+        launch_row_c2r(b->stream, p, g.W, g.H, g.n, g.n, g.spec, g.state, g.state, nullptr, 0.0f, g.twW, nullptr, 0,
+                       g.spec, AH_RULE_LENIA, g.kp, g.dt);
+    }
+}
+
+// Pass 1 for FFT groups whose state changed outside a step (seeded or
+// written); every step's row pass leaves the next spectra behind.
+void ensure_spectra(ah_batch* b) {
+    for (FftGroup& g : b->groups)
+        if (!g.ready) {
+            launch_row_r2c(b->stream, nullptr, g.W, g.H, g.n, g.state, g.spec, g.twW);
+            g.ready = true;
+        }
 }
 
 // Times each admissible (tile shape, grid) on the batch and keeps the fastest.
 void autotune(ah_batch* b) {
     int min_h = 1 << 30, min_w = 1 << 30;
-    for (auto& d : b->d) {
-        min_h = std::min(min_h, (int)d.height);
-        min_w = std::min(min_w, (int)d.width);
+    for (size_t w = 0; w < b->d.size(); ++w) {
+        if (b->fft[w]) continue;
+        min_h = std::min(min_h, (int)b->d[w].height);
+        min_w = std::min(min_w, (int)b->d[w].width);
+    }
+    b->n_tune = 0;
+    if (min_h == 1 << 30) {  // every world runs the FFT path
+        b->cols = b->rows = b->grid = b->ntiles = 0;
+        return;
     }
     hipDeviceProp_t prop;
     AH_HIP(hipGetDeviceProperties(&prop, b->device));
@@ -236,29 +282,70 @@ void setup(ah_batch* b) {
     const size_t n = b->d.size();
     AH_HIP(hipGetDevice(&b->device));
     AH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    // World layout and kernels.
+    b->fft.assign(n, 0);
+    b->group_of.assign(n, -1);
+    b->slot_of.assign(n, -1);
+    // Direct worlds: fragments and schedules; every world: its path.
     std::vector<int> sched;
     std::vector<uint16_t> frags;
-    b->worlds_h.resize(n);
-    b->cost.resize(n);
+    b->worlds_h.assign(n, BatchWorld{});
+    b->cost.assign(n, 0);
     size_t off = 0;
-    int rmax = 0, nfrag_total = 0;
+    int rmax = 0, nfrag_total = 0, reach_max = 0;
+    for (auto& d : b->d) reach_max = std::max(reach_max, (int)std::ceil((double)d.kernels[0].radius));
+    const size_t box = (size_t)(2 * reach_max + 1) * (2 * reach_max + 1);
     std::vector<void*> scratch;
-    double* taps = nullptr;
-    double* sum = nullptr;
-    AH_HIP(hipMalloc((void**)&taps, sizeof(double) * 63 * 63));
-    scratch.push_back(taps);
-    AH_HIP(hipMalloc((void**)&sum, sizeof(double)));
-    scratch.push_back(sum);
+    auto tmp = [&](size_t bytes) {
+        void* p = nullptr;
+        AH_HIP(hipMalloc(&p, bytes));
+        scratch.push_back(p);
+        return p;
+    };
     try {
+        double* taps = (double*)tmp(sizeof(double) * box);
+        double* sum = (double*)tmp(sizeof(double));
         for (size_t w = 0; w < n; ++w) {
             const ah_lenia_desc& d = b->d[w];
             const int R = (int)std::ceil((double)d.kernels[0].radius), side = 2 * R + 1;
-            rmax = std::max(rmax, R);
             launch_kernel_taps64(b->stream, nullptr, d.kernels[0], R, taps, sum);
             std::vector<double> t((size_t)side * side);
             AH_HIP(hipMemcpyAsync(t.data(), taps, sizeof(double) * t.size(), hipMemcpyDeviceToHost, b->stream));
             AH_HIP(hipStreamSynchronize(b->stream));
+            bool use_fft = d.path == AH_PATH_FFT_FP32;
+            if (d.path == AH_PATH_AUTO) {
+                if (R > 31) {
+                    use_fft = true;
+                } else {
+                    // Direct cost: 32-column chunks over the kernel rows, times the terms.
+                    std::vector<int> s_tmp;
+                    std::vector<uint16_t> f_tmp;
+                    int nf = 0;
+                    const int terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, AH_PATH_DIRECT_BF16);
+                    (void)build_fragments(t, R, terms, s_tmp, f_tmp, nf);
+                    int chunks = 0;
+                    for (int dy = -R; dy <= R; ++dy) chunks += s_tmp[std::abs(dy) * 4 + 1];
+                    use_fft = chunks * terms > AH_BATCH_FFT_ABOVE && d.height <= 4096;
+                }
+            }
+            if (use_fft) {
+                b->fft[w] = 1;
+                int gi = -1;
+                for (size_t g = 0; g < b->groups.size(); ++g)
+                    if (b->groups[g].W == (int)d.width && b->groups[g].H == (int)d.height) gi = (int)g;
+                if (gi < 0) {
+                    b->groups.emplace_back();
+                    gi = (int)b->groups.size() - 1;
+                    b->groups[gi].W = (int)d.width;
+                    b->groups[gi].H = (int)d.height;
+                }
+                FftGroup& g = b->groups[gi];
+                b->group_of[w] = gi;
+                b->slot_of[w] = g.n++;
+                g.worlds.push_back((int)w);
+                continue;
+            }
+            const uint32_t path = d.path == AH_PATH_AUTO ? (uint32_t)AH_PATH_DIRECT_BF16 : d.path;
+            rmax = std::max(rmax, R);
             BatchWorld& bw = b->worlds_h[w];
             bw.W = (int)d.width;
             bw.H = (int)d.height;
@@ -266,7 +353,7 @@ void setup(ah_batch* b) {
             bw.reach = R;
             bw.frag0 = nfrag_total;
             bw.sched0 = (int)(sched.size() / 4);
-            bw.terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, d.path);
+            bw.terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, path);
             bw.scale = (float)(1.0 / build_fragments(t, R, bw.terms, sched, frags, bw.nfrag));
             bw.mu = d.kernels[0].mu;
             bw.inv_sigma = 1.0f / d.kernels[0].sigma;
@@ -278,34 +365,90 @@ void setup(ah_batch* b) {
             b->nfrag_max = std::max(b->nfrag_max, bw.nfrag);
             off += (size_t)d.width * d.height;
         }
+
+        // FFT groups: buffers, twiddles, kernel spectra and growth per world.
+        for (FftGroup& g : b->groups) {
+            const size_t cells = (size_t)g.W * g.H, spec = (size_t)(g.W / 2) * g.H, np = (size_t)g.H / 2 + 1;
+            const size_t khat_n = (size_t)(g.W / 8) * np * BC;
+            g.state = (float*)b->alloc(sizeof(float) * cells * g.n);
+            g.spec = (float2*)b->alloc(sizeof(float2) * spec * g.n);
+            g.khat = (float*)b->alloc(sizeof(float) * khat_n * g.n);
+            g.kpack = (float2*)b->alloc(sizeof(float2) * np * g.n);
+            g.twW = (float2*)b->alloc(sizeof(float2) * g.W);
+            g.twH = (float2*)b->alloc(sizeof(float2) * g.H);
+            g.kp = (KernelParams*)b->alloc(sizeof(KernelParams) * g.n);
+            g.dt = (float*)b->alloc(sizeof(float) * g.n);
+            AH_HIP(hipMemsetAsync(g.state, 0, sizeof(float) * cells * g.n, b->stream));
+            AH_HIP(hipMemsetAsync(g.spec, 0, sizeof(float2) * spec * g.n, b->stream));
+            AH_HIP(hipMemsetAsync(g.khat, 0, sizeof(float) * khat_n * g.n, b->stream));
+            launch_twiddles(b->stream, nullptr, g.twW, g.W);
+            launch_twiddles(b->stream, nullptr, g.twH, g.H);
+            double* cosW = (double*)tmp(sizeof(double) * g.W);
+            double* cosH = (double*)tmp(sizeof(double) * g.H);
+            launch_cos_table64(b->stream, nullptr, cosW, g.W);
+            launch_cos_table64(b->stream, nullptr, cosH, g.H);
+            double* T = (double*)tmp(sizeof(double) * (2 * reach_max + 1) * (g.W / 2 + 1));
+            std::vector<KernelParams> kp(g.n);
+            std::vector<float> dt(g.n);
+            for (int c = 0; c < g.n; ++c) {
+                const ah_lenia_desc& d = b->d[g.worlds[c]];
+                const ah_kernel& k = d.kernels[0];
+                const int R = (int)std::ceil((double)k.radius);
+                launch_kernel_taps64(b->stream, nullptr, k, R, taps, sum);
+                launch_khat64(b->stream, nullptr, g.W, g.H, R, taps, cosW, cosH, T, g.khat + khat_n * c,
+                              g.kpack + np * c, false);
+                kp[c] = KernelParams{k.mu, 1.0f / k.sigma, 1.0f, 0u, 0u};
+                dt[c] = d.dt;
+            }
+            AH_HIP(hipMemcpyAsync(g.kp, kp.data(), sizeof(KernelParams) * g.n, hipMemcpyHostToDevice, b->stream));
+            AH_HIP(hipMemcpyAsync(g.dt, dt.data(), sizeof(float) * g.n, hipMemcpyHostToDevice, b->stream));
+            AH_HIP(hipStreamSynchronize(b->stream));  // taps, cos tables and T are reused
+        }
+        AH_HIP(hipStreamSynchronize(b->stream));
     } catch (...) {
+        (void)hipStreamSynchronize(b->stream);
         for (void* p : scratch) (void)hipFree(p);
         throw;
     }
     for (void* p : scratch) (void)hipFree(p);
-    b->total_cells = off;
+    b->total_cells = 0;
+    for (auto& d : b->d) b->total_cells += (uint64_t)d.width * d.height;
     b->halo_y = rmax;
     b->halo_x = 8 * ((rmax + 7) / 8);
-    b->buf[0] = (float*)b->alloc(sizeof(float) * off);
-    b->buf[1] = (float*)b->alloc(sizeof(float) * off);
-    AH_HIP(hipMemsetAsync(b->buf[0], 0, sizeof(float) * off, b->stream));
-    AH_HIP(hipMemsetAsync(b->buf[1], 0, sizeof(float) * off, b->stream));
+    const size_t dcells = std::max<size_t>(off, 1);
+    b->buf[0] = (float*)b->alloc(sizeof(float) * dcells);
+    b->buf[1] = (float*)b->alloc(sizeof(float) * dcells);
+    AH_HIP(hipMemsetAsync(b->buf[0], 0, sizeof(float) * dcells, b->stream));
+    AH_HIP(hipMemsetAsync(b->buf[1], 0, sizeof(float) * dcells, b->stream));
     b->worlds = (BatchWorld*)b->alloc(sizeof(BatchWorld) * n);
-    b->sched = (int4*)b->alloc(sizeof(int) * sched.size());
-    b->frags = (uint4*)b->alloc(sizeof(uint16_t) * frags.size());
+    b->sched = (int4*)b->alloc(sizeof(int) * std::max<size_t>(sched.size(), 4));
+    b->frags = (uint4*)b->alloc(sizeof(uint16_t) * std::max<size_t>(frags.size(), 8));
     AH_HIP(hipMemcpy(b->worlds, b->worlds_h.data(), sizeof(BatchWorld) * n, hipMemcpyHostToDevice));
-    AH_HIP(hipMemcpy(b->sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
-    AH_HIP(hipMemcpy(b->frags, frags.data(), sizeof(uint16_t) * frags.size(), hipMemcpyHostToDevice));
+    if (!sched.empty()) AH_HIP(hipMemcpy(b->sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
+    if (!frags.empty())
+        AH_HIP(hipMemcpy(b->frags, frags.data(), sizeof(uint16_t) * frags.size(), hipMemcpyHostToDevice));
     AH_HIP(hipStreamSynchronize(b->stream));
 
-    // The tuner steps zero states (the work is data-independent), then the
-    // worlds are seeded.
+    // The tuner steps zero states (the work is data-independent); one eager
+    // step warms the FFT launches too.  Then the worlds are seeded.
     autotune(b);
-    for (size_t w = 0; w < n; ++w)
-        launch_seed(b->stream, nullptr, b->d[w], b->seeds[w], nullptr, b->buf[0] + b->worlds_h[w].off);
+    launch_step(b, nullptr);
+    for (size_t w = 0; w < n; ++w) {
+        float* dst = b->fft[w] ? b->groups[b->group_of[w]].state +
+                                     (size_t)b->slot_of[w] * b->d[w].width * b->d[w].height
+                               : b->buf[0] + b->worlds_h[w].off;
+        launch_seed(b->stream, nullptr, b->d[w], b->seeds[w], nullptr, dst);
+    }
     AH_HIP(hipStreamSynchronize(b->stream));
+    for (FftGroup& g : b->groups) g.ready = false;
     b->parity = 0;
     b->generation = 0;
+}
+
+// The state of world w: its slot of the ping-pong buffer or of its FFT group.
+float* world_state(ah_batch* b, uint32_t w) {
+    if (b->fft[w]) return b->groups[b->group_of[w]].state + (size_t)b->slot_of[w] * b->d[w].width * b->d[w].height;
+    return b->buf[b->parity] + b->worlds_h[w].off;
 }
 
 hipGraphExec_t graph_for(ah_batch* b) {
@@ -334,6 +477,7 @@ hipGraphExec_t graph_for(ah_batch* b) {
 }
 
 void run_steps(ah_batch* b, uint32_t steps) {
+    ensure_spectra(b);
     while (steps) {
         if (steps >= b->graph_steps) {
             AH_HIP(hipGraphLaunch(graph_for(b), b->stream));
@@ -383,9 +527,9 @@ ah_status ah_batch_read_world(ah_batch* b, uint32_t world, float* dst, size_t by
     return guard([&] {
         if (!b || !dst) throw Error{AH_ERR_INVALID, "null argument"};
         if (world >= b->d.size()) throw Error{AH_ERR_INVALID, "world index out of range"};
-        const BatchWorld& w = b->worlds_h[world];
-        if (bytes != sizeof(float) * (size_t)w.W * w.H) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
-        AH_HIP(hipMemcpyAsync(dst, b->buf[b->parity] + w.off, bytes, hipMemcpyDeviceToHost, b->stream));
+        const ah_lenia_desc& d = b->d[world];
+        if (bytes != sizeof(float) * (size_t)d.width * d.height) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
+        AH_HIP(hipMemcpyAsync(dst, world_state(b, world), bytes, hipMemcpyDeviceToHost, b->stream));
         AH_HIP(hipStreamSynchronize(b->stream));
     });
 }
@@ -394,9 +538,10 @@ ah_status ah_batch_write_world(ah_batch* b, uint32_t world, const float* src, si
     return guard([&] {
         if (!b || !src) throw Error{AH_ERR_INVALID, "null argument"};
         if (world >= b->d.size()) throw Error{AH_ERR_INVALID, "world index out of range"};
-        const BatchWorld& w = b->worlds_h[world];
-        if (bytes != sizeof(float) * (size_t)w.W * w.H) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
-        AH_HIP(hipMemcpyAsync(b->buf[b->parity] + w.off, src, bytes, hipMemcpyHostToDevice, b->stream));
+        const ah_lenia_desc& d = b->d[world];
+        if (bytes != sizeof(float) * (size_t)d.width * d.height) throw Error{AH_ERR_SIZE, "world is height*width*4 bytes"};
+        if (b->fft[world]) b->groups[b->group_of[world]].ready = false;
+        AH_HIP(hipMemcpyAsync(world_state(b, world), src, bytes, hipMemcpyHostToDevice, b->stream));
         AH_HIP(hipStreamSynchronize(b->stream));
     });
 }
@@ -416,6 +561,8 @@ ah_status ah_batch_info_get(const ah_batch* b, ah_batch_info* out) {
         i.generation = b->generation;
         i.n_tune = b->n_tune;
         for (uint32_t k = 0; k < b->n_tune; ++k) i.tune[k] = b->tune[k];
+        for (char f : b->fft) i.fft_worlds += f ? 1 : 0;
+        i.fft_groups = (uint32_t)b->groups.size();
         *out = i;
     });
 }
@@ -427,6 +574,7 @@ ah_status ah_batch_measure(ah_batch* b, uint32_t steps, double* kernel_ms, doubl
         if (steps == 0 || steps % b->graph_steps) throw Error{AH_ERR_INVALID, "steps must be a multiple of graph_steps"};
         AH_HIP(hipSetDevice(b->device));
         // Eager launches with per-launch events: summed device time and wall time.
+        ensure_spectra(b);
         AH_HIP(hipStreamSynchronize(b->stream));
         b->prof.reset();
         b->prof.enabled = true;

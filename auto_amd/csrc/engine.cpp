@@ -266,7 +266,8 @@ void validate(const ah_lenia_desc& d) {
     for (uint32_t k = 0; k < d.n_kernels; ++k)
         if (d.path != AH_PATH_FFT_FP32 && d.kernels[k].shape != AH_SHAPE_LENIA)
             throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path takes Lenia-shaped kernels"};
-    if (d.path > AH_PATH_DIRECT_BF16_X3) bad("unknown path");
+    if (d.path == AH_PATH_AUTO) bad("AH_PATH_AUTO chooses a path per world in a batch; a Sim takes a fixed path");
+    if (d.path > AH_PATH_AUTO) bad("unknown path");
     if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
         throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
     if (d.path != AH_PATH_FFT_FP32) {

@@ -271,6 +271,7 @@ struct ColArgs {
     const float2* twH;
     int n_kernels;
     int ksrc[AH_MAX_KERNELS];
+    int diag;  // kernel k reads and writes channel k only (batched same-size worlds)
 };
 
 // Element (y, column c of this workgroup) of a block in global memory.
@@ -331,11 +332,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     lds_sync();
     fft_lds<E, -1>(lds, H, cw, a.twH, H);
     int users = 0, last = -1;
-    for (int k = 0; k < a.n_kernels; ++k)
-        if (a.ksrc[k] == ch) {
-            ++users;
-            last = k;
-        }
+    if (a.diag) {
+        users = 1;
+        last = ch;
+    } else {
+        for (int k = 0; k < a.n_kernels; ++k)
+            if (a.ksrc[k] == ch) {
+                ++users;
+                last = k;
+            }
+    }
     if (users == 1) {
         multiply_cols(lds, a, kb, c0, last);
         fft_lds<E, 1>(lds, H, cw, a.twH, H);
@@ -564,6 +570,10 @@ struct RowArgs {
     float* conv_out;  // when set: write kernel conv_k's potential instead of updating
     int conv_k;
     float2* spec_next;  // NEXT: half spectra [channels][W/8][H][4] of the updated state
+    // Batched same-size worlds: channel c is world c with one kernel (k = c)
+    // whose growth and dt come from these device arrays.
+    const KernelParams* kp_dev;
+    const float* dt_dev;
 };
 
 // Load kernel k's spectrum rows into LDS and turn them into the length-M
@@ -655,9 +665,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     for (int i = 0; i < NQ; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
-    for (int k = 0; k < a.n_kernels; ++k) {
-        const KernelParams kp = a.kp[k];
-        if (a.conv_out ? (k != a.conv_k) : ((int)kp.dst != ch)) continue;
+    const bool diag = a.kp_dev != nullptr;
+    const int k_begin = diag ? ch : 0, k_end = diag ? ch + 1 : a.n_kernels;
+    const float dt = diag ? a.dt_dev[ch] : a.dt;
+    for (int k = k_begin; k < k_end; ++k) {
+        const KernelParams kp = diag ? a.kp_dev[k] : a.kp[k];
+        if (!diag && (a.conv_out ? (k != a.conv_k) : ((int)kp.dst != ch))) continue;
         load_c2r(lds, a, k, y0);
         fft_lds<E, 1>(lds, M, rows, a.twW, W);
 #pragma unroll
@@ -701,9 +714,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
         float r[4] = {s.x, s.y, s.z, s.w};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if constexpr (RULE == AH_RULE_SMOOTHLIFE) r[j] = smoothlife_next(r[j], acc[i][j], pot1[i][j], a.dt);
+            if constexpr (RULE == AH_RULE_SMOOTHLIFE) r[j] = smoothlife_next(r[j], acc[i][j], pot1[i][j], dt);
             else if constexpr (RULE == AH_RULE_MNCA) r[j] = mnca_next(r[j], acc[i][j], pot1[i][j]);
-            else r[j] = fminf(fmaxf(r[j] + a.dt * acc[i][j], 0.0f), 1.0f);
+            else r[j] = fminf(fmaxf(r[j] + dt * acc[i][j], 0.0f), 1.0f);
         }
         o4[q] = make_float4(r[0], r[1], r[2], r[3]);
         if constexpr (NEXT) {
@@ -889,7 +902,7 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
 
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH,
-                     const int* ksrc, int n_kernels) {
+                     const int* ksrc, int n_kernels, bool diag) {
     ColArgs a;
     a.W = W;
     a.H = H;
@@ -899,9 +912,10 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.khat = khat;
     a.kpack = kpack;
     a.twH = twH;
-    a.n_kernels = n_kernels;
-    for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = k < n_kernels ? ksrc[k] : -1;
-    if (col_stream_ok(W, H, channels, n_kernels, ksrc)) {
+    a.diag = diag ? 1 : 0;
+    a.n_kernels = diag ? channels : n_kernels;
+    for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = (!diag && k < n_kernels) ? ksrc[k] : -1;
+    if (!diag && col_stream_ok(W, H, channels, n_kernels, ksrc)) {
         // Up to four columns per workgroup, aiming at two workgroups per CU
         // (each holds 2 x 33 KiB of LDS).
         const int ncols = W / 2, slots = 2 * cu_count();
@@ -926,7 +940,8 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
-                    float* conv_out, int conv_k, float2* spec_next, int rule) {
+                    float* conv_out, int conv_k, float2* spec_next, int rule,
+                    const KernelParams* kp_dev, const float* dt_dev) {
     RowArgs a;
     a.W = W;
     a.H = H;
@@ -936,8 +951,11 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     a.state_out = state_out;
     a.twW = twW;
     a.dt = dt;
-    a.n_kernels = n_kernels;
-    for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
+    a.kp_dev = kp_dev;
+    a.dt_dev = dt_dev;
+    a.n_kernels = kp_dev ? channels : n_kernels;
+    if (!kp_dev)
+        for (int k = 0; k < n_kernels; ++k) a.kp[k] = kp[k];
     a.conv_out = conv_out;
     a.conv_k = conv_k;
     a.spec_next = conv_out ? nullptr : spec_next;
@@ -956,7 +974,7 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     else if (!a.spec_next) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_update");
     else if (rule == AH_RULE_SMOOTHLIFE) go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
     else if (rule == AH_RULE_MNCA) go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
-    else if (W == 2 * RM && channels == 1 && n_kernels == 1 && kp[0].dst == 0) {
+    else if (!kp_dev && W == 2 * RM && channels == 1 && n_kernels == 1 && kp[0].dst == 0) {
         Launch L(p, s, "row_c2r_update_r2c");
         hipLaunchKernelGGL(k_row_fused, grid, dim3(RT), lds, s, a);
     } else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");

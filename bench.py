@@ -115,11 +115,13 @@ def time_path(path: str, side: int, steps: int, warmup: int, seed: int):
                         kernels=kernels)
 
 
-def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
+def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "auto"):
     """The 384 uneven worlds: aggregate cell-updates/s over graph-replayed
-    steps, autotuner choices, and the launch-overhead share."""
+    steps, autotuner choices, and the launch-overhead share.  path "auto"
+    lets the batch put each world on the FFT or the direct path by its cost
+    model; "direct_bf16" keeps every world on the direct path."""
     from auto_amd import Batch, workloads as W
-    worlds = W.uneven_worlds(seed=seed, n=n, path="direct_bf16")
+    worlds = W.uneven_worlds(seed=seed, n=n, path=path)
     b = Batch(worlds)
     info = b.info()
     b.step(graph_steps)  # warm-up: captures the graph
@@ -130,7 +132,8 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100):
     dt = time.perf_counter() - t0
     m = b.measure(graph_steps)
     cells = info.total_cells
-    return {"worlds": n, "cells_per_step": cells, "value": cells * 2 * graph_steps / dt,
+    return {"worlds": n, "path": path, "fft_worlds": info.fft_worlds, "fft_groups": info.fft_groups,
+            "cells_per_step": cells, "value": cells * 2 * graph_steps / dt,
             "unit": "cell-updates/s", "ms_per_step": 1e3 * dt / (2 * graph_steps),
             "tile": [info.tile_cols, info.tile_rows], "grid": info.grid, "tiles": info.ntiles,
             "autotune": [{"tile": [info.tune[i].tile_cols, info.tune[i].tile_rows], "grid": info.tune[i].grid,
@@ -423,7 +426,9 @@ def main() -> int:
     out["torch_rocfft"] = tf
     if not args.no_batch:
         out["batch"] = time_batch(args.seed)
-        log(f"batch: {out['batch']['value']:.3e} cell-updates/s")
+        log(f"batch (per-world path, {out['batch']['fft_worlds']} FFT worlds): {out['batch']['value']:.3e} cell-updates/s")
+        out["batch_all_direct"] = time_batch(args.seed, path="direct_bf16")
+        log(f"batch (all direct): {out['batch_all_direct']['value']:.3e} cell-updates/s")
     mc = {}
     for path in ("fft_fp32", "direct_bf16"):
         dm, sm, rm = time_multichannel(path, 200, args.seed)

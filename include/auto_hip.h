@@ -51,8 +51,10 @@ typedef enum {
     AH_PATH_DIRECT_BF16 = 1,    /* direct convolution on BF16 MFMA, FP32 accumulate; the
                                    engine picks 1 or 3 bf16 terms per kernel (below)      */
     AH_PATH_DIRECT_BF16_X1 = 2, /* force one term: bf16(K) * bf16(s)                      */
-    AH_PATH_DIRECT_BF16_X3 = 3  /* force three terms: K_hi s_hi + K_hi s_lo + K_lo s_hi,
+    AH_PATH_DIRECT_BF16_X3 = 3, /* force three terms: K_hi s_hi + K_hi s_lo + K_lo s_hi,
                                    with x = x_hi + x_lo split into two bf16 values        */
+    AH_PATH_AUTO = 4            /* batched worlds only: the batch picks FFT or direct per
+                                   world (see ah_batch_new)                              */
 } ah_path;
 /*
  * AH_PATH_DIRECT_BF16 chooses per kernel: one term when the growth slope
@@ -227,17 +229,23 @@ ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacit
 /*
  * Batched uneven worlds.  No counterpart in the reference, which runs one
  * world per Sim (sim.rs:202-309).  Every world is single-channel with one
- * kernel (radius <= 31), width and height >= 64, a direct BF16 path,
- * and an ORBIUM, PATCH or ZERO init seeded with its own seed.  Worlds differ in
- * size, rule and dt.  All worlds advance together in one persistent launch per
- * step; ah_batch_step replays hipGraph-captured chunks of `graph_steps`
- * launches and launches the remainder one by one.  At creation an autotuner
- * times every admissible (tile rows, grid) pair on the batch and keeps the
- * fastest; the candidates and their times are reported by ah_batch_info_get.
+ * Lenia kernel, width and height >= 64, and an ORBIUM, PATCH or ZERO init
+ * seeded with its own seed; worlds differ in size, rule, dt and path.
+ * Direct BF16 worlds (radius <= 31) advance together in one persistent launch
+ * per step, over tiles of every such world.  FFT worlds (height <= 4096) are
+ * grouped by size; each group runs as one column-pass and one row-pass launch
+ * per step, one world per channel.  AH_PATH_AUTO worlds take the FFT path when
+ * their direct cost (32-column MFMA chunks summed over the kernel rows, times
+ * the bf16 terms) exceeds AH_BATCH_FFT_ABOVE, or their radius exceeds 31.
+ * ah_batch_step replays hipGraph-captured chunks of `graph_steps` steps and
+ * runs the remainder one by one.  At creation an autotuner times every
+ * admissible (tile shape, grid) pair of the direct launch on the batch itself
+ * and keeps the fastest; the candidates and their times are reported by
+ * ah_batch_info_get.
  */
 typedef struct ah_batch ah_batch;
 
-enum { AH_BATCH_MAX_CANDIDATES = 8 };
+enum { AH_BATCH_MAX_CANDIDATES = 8, AH_BATCH_FFT_ABOVE = 60 };
 
 typedef struct {
     uint32_t tile_cols;  /* output columns per tile (64 or 128)               */
@@ -254,6 +262,8 @@ typedef struct {
     uint64_t generation;
     uint32_t n_tune;
     ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES];
+    uint32_t fft_worlds;   /* worlds on the FFT path (the rest run direct BF16) */
+    uint32_t fft_groups;   /* same-size FFT groups, two launches each per step  */
 } ah_batch_info;
 
 ah_status ah_batch_new(const ah_lenia_desc* descs, uint32_t n_worlds, const uint32_t* seeds, ah_batch** out);

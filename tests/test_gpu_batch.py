@@ -82,7 +82,60 @@ def test_batch_measure_reports_overhead():
     print(m)
 
 
-def test_batch_rejects_fft_path():
-    d, s = W.uneven_worlds(seed=1, n=1, path="fft_fp32")[0]
-    with pytest.raises(EngineError):
-        Batch([(d, s)])
+def _world(side, radius, sigma, dt, path, rings=(1.0,)):
+    return W.desc(side, side, [dict(radius=float(radius), beta=list(rings), mu=0.2, sigma=sigma)], dt, path,
+                  init=W.INIT_PATCH, init_size=side // 4)
+
+
+def test_batch_fft_and_auto_worlds(O):
+    """FFT worlds (grouped by size, one world per channel) and AUTO worlds in
+    one batch with direct worlds.  An FFT world equals a standalone Sim of the
+    same world bitwise (the same kernels); every world is within 1e-3 of the
+    FP64 step from the same input; AUTO takes FFT for a wide, high-gain kernel
+    (3 bf16 terms, ~100 chunks) and direct for a small, low-gain one."""
+    worlds = W.uneven_worlds(seed=4, n=5, path="fft_fp32", sides=(128, 256))
+    worlds += W.uneven_worlds(seed=5, n=5, path="direct_bf16", sides=(128, 256))
+    worlds += [(_world(256, 24, 0.01, 0.2, "auto", (1.0, 0.6, 0.3)), 11),   # -> FFT
+               (_world(128, 8, 0.05, 0.05, "auto"), 12)]                   # -> direct
+    b = Batch(worlds)
+    info = b.info()
+    sizes = {(d.width, d.height) for d, _ in worlds[:5]} | {(256, 256)}
+    assert info.fft_worlds == 6 and info.fft_groups == len(sizes)
+    b.step(2)
+    before = [b.read_world(i) for i in range(len(worlds))]
+    b.step(1)
+    worst64 = 0.0
+    for i, (d, seed) in enumerate(worlds):
+        got = b.read_world(i)
+        worst64 = max(worst64, float(np.abs(got - O.step64(d, before[i][None], "fft")[0]).max()))
+        if i < 5 or i == 10:
+            plain = LeniaDesc.from_buffer_copy(d)
+            plain.path = PATH_FFT_FP32
+            sim = Sim(plain, seed)
+            sim.write_field("state", before[i][None])
+            sim.step(1)
+            assert np.array_equal(got, sim.read_field("state")[0]), i
+    print(f"mixed batch: {info.fft_worlds} FFT worlds in {info.fft_groups} groups; worst max-abs vs FP64 {worst64:.3e}")
+    assert worst64 <= 1e-3
+    # A write into an FFT world replaces the spectra its next step uses.
+    x = np.random.default_rng(1).random((worlds[0][0].height, worlds[0][0].width), dtype=np.float32)
+    b.write_world(0, x)
+    b.step(1)
+    plain = LeniaDesc.from_buffer_copy(worlds[0][0])
+    sim = Sim(plain, 1)
+    sim.write_field("state", x[None])
+    sim.step(1)
+    assert np.array_equal(b.read_world(0), sim.read_field("state")[0])
+
+
+def test_batch_of_fft_worlds_only_and_graph():
+    """No direct worlds: no tuner candidates, the FFT groups alone, and graph
+    replays equal eager steps."""
+    worlds = W.uneven_worlds(seed=7, n=4, path="fft_fp32", sides=(128,))
+    a, b = Batch(worlds), Batch(worlds)
+    assert a.info().n_tune == 0 and a.info().fft_groups == 1 and a.info().ntiles == 0
+    a.step(100)
+    for _ in range(100):
+        b.step(1)
+    for i in range(len(worlds)):
+        assert np.array_equal(a.read_world(i), b.read_world(i))
