@@ -26,7 +26,7 @@ PATH_DIRECT_BF16 = 1
 PATH_DIRECT_BF16_X1 = 2
 PATH_DIRECT_BF16_X3 = 3
 PATH_AUTO = 4
-BATCH_FFT_ABOVE = 60
+BATCH_FFT_ABOVE = 30
 
 INIT_ORBIUM = 0
 INIT_PATCH = 1

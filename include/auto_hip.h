@@ -237,6 +237,8 @@ ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacit
  * per step, one world per channel.  AH_PATH_AUTO worlds take the FFT path when
  * their direct cost (32-column MFMA chunks summed over the kernel rows, times
  * the bf16 terms) exceeds AH_BATCH_FFT_ABOVE, or their radius exceeds 31.
+ * The threshold is measured: on the 384-world benchmark batch, 30 and 0 (all
+ * FFT) run at the same rate, 60 at 0.9x and 240 at 0.25x of it.
  * ah_batch_step replays hipGraph-captured chunks of `graph_steps` steps and
  * runs the remainder one by one.  At creation an autotuner times every
  * admissible (tile shape, grid) pair of the direct launch on the batch itself
@@ -245,7 +247,7 @@ ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacit
  */
 typedef struct ah_batch ah_batch;
 
-enum { AH_BATCH_MAX_CANDIDATES = 8, AH_BATCH_FFT_ABOVE = 60 };
+enum { AH_BATCH_MAX_CANDIDATES = 8, AH_BATCH_FFT_ABOVE = 30 };
 
 typedef struct {
     uint32_t tile_cols;  /* output columns per tile (64 or 128)               */
