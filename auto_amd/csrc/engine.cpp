@@ -332,16 +332,14 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         nfrag_total += dk.nfrag * (dk.terms == 3 ? 2 : 1);
         a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
     }
-    bool fits = false;
+    std::vector<std::pair<int, int>> shapes;
     for (auto cr : {std::make_pair(128, 128), std::make_pair(128, 64), std::make_pair(64, 64)}) {
         if (cr.first > (int)d.width || cr.second > (int)d.height) continue;
         shape(cr.first, cr.second);
-        if (direct_lds_bytes(a) <= 160 * 1024) {
-            fits = true;
-            break;
-        }
+        if (direct_lds_bytes(a) <= 160 * 1024) shapes.push_back(cr);
     }
-    if (!fits) throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
+    if (shapes.empty())
+        throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
     int4* dsched = (int4*)s->alloc(sizeof(int) * sched.size());
     uint4* dfrags = (uint4*)s->alloc(sizeof(uint16_t) * frags.size());
     AH_HIP(hipMemcpy(dsched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
@@ -350,6 +348,47 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     a.frags = dfrags;
     a.dt = d.dt;
     s->state_alt = (float*)s->alloc(sizeof(float) * s->cells() * d.channels);
+    // Tile shape: the largest that fits LDS when it is the only candidate;
+    // otherwise each admissible shape is timed on this world (zeroed states,
+    // the work is data-independent) and the fastest kept.  Small worlds can
+    // prefer small tiles: more workgroups than CUs.
+    shape(shapes[0].first, shapes[0].second);
+    if (shapes.size() > 1) {
+        const size_t bytes = sizeof(float) * s->cells() * d.channels;
+        AH_HIP(hipMemsetAsync(s->state, 0, bytes, s->stream));
+        AH_HIP(hipMemsetAsync(s->state_alt, 0, bytes, s->stream));
+        hipEvent_t e0, e1;
+        AH_HIP(hipEventCreate(&e0));
+        AH_HIP(hipEventCreate(&e1));
+        double best = 1e300;
+        std::pair<int, int> pick = shapes[0];
+        try {
+            for (auto cr : shapes) {
+                shape(cr.first, cr.second);
+                DirectArgs t = a;
+                t.state_in = s->state;
+                t.state_out = s->state_alt;
+                launch_direct(s->stream, nullptr, t, (int)d.channels);  // warm-up
+                AH_HIP(hipEventRecord(e0, s->stream));
+                for (int r = 0; r < 3; ++r) launch_direct(s->stream, nullptr, t, (int)d.channels);
+                AH_HIP(hipEventRecord(e1, s->stream));
+                AH_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                AH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    pick = cr;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        shape(pick.first, pick.second);
+    }
 }
 
 void setup(ah_sim* s) {
