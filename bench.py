@@ -144,6 +144,23 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
             "measure": m}
 
 
+def drift_4096(seed: int, steps: int = 1000):
+    """Mass over `steps` steps of the 4096^2 three-ring world on both paths
+    from the same seed: the direct BF16 path's drift relative to the FP32 FFT
+    path (which itself tracks the FP64 oracle's mass to ~2e-5 over 1000 steps,
+    tests/test_gpu_drift.py)."""
+    from auto_amd import Sim, workloads as W
+    masses = {}
+    for path in ("fft_fp32", "direct_bf16"):
+        sim = Sim(W.rings3_4096(path=path), seed)
+        m0 = sim.stats().mass[0]
+        sim.step(steps)
+        masses[path] = (m0, sim.stats().mass[0])
+    f, d = masses["fft_fp32"][1], masses["direct_bf16"][1]
+    return {"steps": steps, "mass0": masses["fft_fp32"][0], "mass_fft": f, "mass_direct": d,
+            "direct_vs_fft": abs(d - f) / f}
+
+
 def time_rule(rule: str, side: int, steps: int, seed: int):
     """SmoothLife (ra = 21) or MNCA (rings [12,18), [3,8)) on the FFT path:
     two kernels, so the column pass writes two product spectra and the row
@@ -441,6 +458,8 @@ def main() -> int:
         rules[rule] = time_rule(rule, args.side, 200, args.seed)
         log(f"{rule}: {rules[rule]['value']:.3e} cell-updates/s")
     out["rules"] = rules
+    out["drift_4096"] = drift_4096(args.seed)
+    log(f"mass drift over 1000 steps at 4096^2, direct vs FFT: {out['drift_4096']['direct_vs_fft']:.2e}")
     if args.surrogate:
         out["surrogate"] = time_surrogate(args.seed, args.surrogate_train_steps)
     if not args.no_cpu and rank == 0:
