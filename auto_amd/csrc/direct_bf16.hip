@@ -58,27 +58,51 @@ __device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float
 // the bf16 rounding of what hi leaves over, x - hi.  Columns outside
 // [x0-hy, x0+DX+hy) are zero: no nonzero tap reaches them, and zeros keep
 // non-finite data out of range.
+//
+// Wave w stages rows w, w+8, ...; lane l covers columns 4l..4l+3 (and +256,
+// ... for wider tiles).  A lane issues the loads of STAGE_BATCH rows before
+// converting any of them, so a tile's halo arrives with up to STAGE_BATCH
+// loads in flight per lane instead of one.  Row wrap is a single +-H: the
+// halo (<= 63 rows) never exceeds the world height (>= 64).
+constexpr int STAGE_BATCH = 16;
+
 __device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int W, int H, int x0, int y0, int DX,
                                            int halo_y, int halo_x, int tr, int tc, int tstride, bool lo) {
-    const int groups = tc >> 2;
-    for (int i = threadIdx.x; i < tr * groups; i += DTHREADS) {
-        int r = i / groups, j = (i - r * groups) * 4;
-        int gy = (y0 - halo_y + r) % H;
-        if (gy < 0) gy += H;
-        int cx = j - halo_x;  // column relative to x0
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 4 * lane; j < tc; j += 256) {
+        const int cx = j - halo_x;  // column relative to x0
         int gx = (x0 + cx) % W;
         if (gx < 0) gx += W;
-        float4 v = *reinterpret_cast<const float4*>(src + (size_t)gy * W + gx);
-        float e[4] = {v.x, v.y, v.z, v.w};
-        uint32_t h[4];
+        for (int r0 = wave; r0 < tr; r0 += 8 * STAGE_BATCH) {
+            float4 v[STAGE_BATCH];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            int c = cx + t;
-            uint32_t hv = bf16_bits(e[t]);
-            if (lo) hv = bf16_bits(e[t] - bf16_value(hv));
-            h[t] = (c >= -halo_y && c < DX + halo_y) ? hv : 0u;
+            for (int u = 0; u < STAGE_BATCH; ++u) {
+                const int r = r0 + 8 * u;
+                if (r < tr) {
+                    int gy = y0 - halo_y + r;
+                    if (gy < 0) gy += H;
+                    else if (gy >= H) gy -= H;
+                    v[u] = *reinterpret_cast<const float4*>(src + (size_t)gy * W + gx);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_BATCH; ++u) {
+                const int r = r0 + 8 * u;
+                if (r < tr) {
+                    const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                    uint32_t h[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int c = cx + t;
+                        uint32_t hv = bf16_bits(e[t]);
+                        if (lo) hv = bf16_bits(e[t] - bf16_value(hv));
+                        h[t] = (c >= -halo_y && c < DX + halo_y) ? hv : 0u;
+                    }
+                    *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) =
+                        make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                }
+            }
         }
-        *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
     }
 }
 
