@@ -105,6 +105,10 @@ class Stats(ctypes.Structure):
     ]
 
 
+class DirectTile(ctypes.Structure):
+    _fields_ = [("cols", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("ring", ctypes.c_uint32)]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 40),
@@ -172,6 +176,8 @@ _SIGNATURES = {
     "ah_sim_destroy": (None, [_P]),
     "ah_last_error": (ctypes.c_char_p, []),
     "ah_sim_profile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ah_sim_direct_tile": (ctypes.c_int, [_P, ctypes.POINTER(DirectTile)]),
+    "ah_sim_set_direct_tile": (ctypes.c_int, [_P, ctypes.POINTER(DirectTile)]),
     "ah_sim_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(KernelTime), ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32)]),
     "ah_copy_probe": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),

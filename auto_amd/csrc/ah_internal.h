@@ -46,10 +46,11 @@ ah_status guard(F&& f) {
 uint64_t splitmix64(uint64_t& state);
 uint16_t f32_to_bf16(float f);
 double bf16_to_f64(uint16_t h);
-// Appends the row schedule and the hi fragments (then, for terms == 3, the lo
+// Appends the row list and the hi fragments (then, for terms == 3, the lo
 // fragments) of one kernel; returns the FP64 sum of the bf16 taps used.
-double build_fragments(const std::vector<double>& taps, int R, int terms, std::vector<int>& sched,
-                       std::vector<uint16_t>& frags, int& nfrag);
+// nrow[c] counts the rows with 3 - c chunks (rows listed in that order).
+double build_fragments(const std::vector<double>& taps, int R, int terms, std::vector<int>& rows,
+                       std::vector<uint16_t>& frags, int& nfrag, int nrow[3]);
 // 1 or 3 bf16 terms for a kernel with these taps and growth (see auto_hip.h).
 int direct_terms(const std::vector<double>& taps, float dt, float weight, float sigma, uint32_t path);
 
@@ -144,8 +145,14 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
 size_t fft_lds_limit();
 
 // direct_bf16.hip
+// Direct path row list: per kernel, the rows dy in [-R, R] that hold taps as
+// (dy, e, first fragment, 0), those with 3 chunks first, then 2, then 1 (dy
+// ascending within a group), so each group runs with a fixed chunk count.
+constexpr int DIRECT_MAX_ROWS = 64;  // 2 * 31 + 1 rows, rounded up
+
 struct DirectKernel {  // one kernel as the direct path reads it
     int reach;         // R: rows dy in [-R, R]
+    int nrow[3];       // rows with 3, 2, 1 chunks
     int frag0, nfrag;  // hi fragments [frag0, frag0+nfrag); lo fragments follow them
     int terms;         // 1: K_hi s_hi; 3: + K_hi s_lo + K_lo s_hi
     float scale;       // 1 / sum of the bf16-rounded taps
@@ -158,9 +165,10 @@ struct DirectArgs {
     int halo_y, halo_x;  // max R over kernels; max R rounded up to a multiple of 8
     int tr, tc, tstride; // LDS tile rows, columns, row stride (bf16 elements)
     int cols, rows;      // output tile shape of the launch (64 or 128 each)
+    int ring;            // 1: persistent ring kernel (one channel, one kernel, one term, rows = 64)
     int n_kernels;
     DirectKernel k[AH_MAX_KERNELS];
-    const int4* sched;   // per kernel, per |dy| in [0, R]: (e, chunks, first fragment, 0)
+    const int4* sched;   // per kernel, its row list
     int sched_base[AH_MAX_KERNELS];
     const uint4* frags;  // [fragment][64 lanes] x 8 bf16
     int nfrag_max;
@@ -174,6 +182,7 @@ struct BatchWorld {  // one world of a batch as the batch kernel reads it
     int W, H;
     size_t off;      // first cell in both state buffers
     int reach, frag0, nfrag, sched0, terms;
+    int nrow[3];     // rows with 3, 2, 1 chunks in the row list at sched0
     float scale, mu, inv_sigma, dt;
 };
 
@@ -191,6 +200,15 @@ struct BatchArgs {
 };
 size_t batch_lds_bytes(const BatchArgs& a);
 void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int cols, int rows, int grid);
+int cu_count();  // compute units of the current device
+
+// Workgroup barrier for LDS traffic only: this wave's LDS operations finish,
+// then all waves meet.  Unlike __syncthreads() it does not wait for global
+// loads and stores in flight (its release fence would), so loads issued
+// before it keep running across it.  Only for workgroups whose threads
+// exchange data through LDS, never through global memory.  The memory
+// clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 size_t direct_lds_bytes(const DirectArgs& a);
 void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels);
 

@@ -321,9 +321,9 @@ void setup(ah_batch* b) {
                     std::vector<uint16_t> f_tmp;
                     int nf = 0;
                     const int terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, AH_PATH_DIRECT_BF16);
-                    (void)build_fragments(t, R, terms, s_tmp, f_tmp, nf);
-                    int chunks = 0;
-                    for (int dy = -R; dy <= R; ++dy) chunks += s_tmp[std::abs(dy) * 4 + 1];
+                    int nrow[3];
+                    (void)build_fragments(t, R, terms, s_tmp, f_tmp, nf, nrow);
+                    const int chunks = 3 * nrow[0] + 2 * nrow[1] + nrow[2];
                     use_fft = chunks * terms > AH_BATCH_FFT_ABOVE && d.height <= 4096;
                 }
             }
@@ -354,13 +354,11 @@ void setup(ah_batch* b) {
             bw.frag0 = nfrag_total;
             bw.sched0 = (int)(sched.size() / 4);
             bw.terms = direct_terms(t, d.dt, 1.0f, d.kernels[0].sigma, path);
-            bw.scale = (float)(1.0 / build_fragments(t, R, bw.terms, sched, frags, bw.nfrag));
+            bw.scale = (float)(1.0 / build_fragments(t, R, bw.terms, sched, frags, bw.nfrag, bw.nrow));
             bw.mu = d.kernels[0].mu;
             bw.inv_sigma = 1.0f / d.kernels[0].sigma;
             bw.dt = d.dt;
-            int chunks = 0;
-            for (int dy = -R; dy <= R; ++dy) chunks += sched[(bw.sched0 + std::abs(dy)) * 4 + 1];
-            b->cost[w] = chunks * bw.terms;
+            b->cost[w] = (3 * bw.nrow[0] + 2 * bw.nrow[1] + bw.nrow[2]) * bw.terms;
             nfrag_total += bw.nfrag * (bw.terms == 3 ? 2 : 1);
             b->nfrag_max = std::max(b->nfrag_max, bw.nfrag);
             off += (size_t)d.width * d.height;

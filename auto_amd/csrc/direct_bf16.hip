@@ -130,24 +130,42 @@ __device__ __forceinline__ void conv_row(f32x4 (&acc)[S][NB], const uint16_t* br
     }
 }
 
+// The rows [i0, i1) of the row list, which all have NCH chunks.  The next
+// row's entry is read from LDS while this row's MFMAs run.
+template <int S, int NB, int NCH>
+__device__ __forceinline__ void conv_rows(f32x4 (&acc)[S][NB], const uint16_t* tile, const uint4* frag_lds,
+                                          const int4* rows, int i0, int i1, int base, int ring_rows, int colP,
+                                          int tstride, int lane) {
+    if (i0 >= i1) return;
+    int4 nxt = rows[i0];
+    for (int i = i0; i < i1; ++i) {
+        const int4 rc = nxt;  // (dy, e, first fragment, 0)
+        if (i + 1 < i1) nxt = rows[i + 1];
+        int slot = base + rc.x;
+        if (slot < 0) slot += ring_rows;
+        else if (slot >= ring_rows) slot -= ring_rows;
+        conv_row<S, NB, NCH>(acc, tile + (size_t)slot * tstride + colP + rc.y, frag_lds + rc.z * 64, lane, tstride);
+    }
+}
+
 // All MFMAs of one kernel term over this wave's S strips x NB blocks,
-// accumulated into acc.
+// accumulated into acc, row group by row group (3, 2, then 1 chunks).
+// Tile row r of the lane sits in LDS row (vbase + r) mod ring_rows: the
+// plain tile passes vbase 0 and a ring_rows no row reaches; the ring kernel
+// (NB = 1) passes its ring.
 template <int S, int NB>
 __device__ __forceinline__ void conv_tile(f32x4 (&acc)[S][NB], const uint16_t* tile, const uint4* frag_lds,
-                                          const int4* sched, int reach, int halo_y, int halo_x, int tstride) {
+                                          const int4* rows, const int (&nrow)[3], int halo_y, int halo_x,
+                                          int tstride, int ring_rows, int vbase) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int par = wave & 1, grp = wave >> 1;
     const int q = lane >> 4, n = lane & 15;
-    const int row0 = grp * 16 * NB + n + halo_y;
+    const int base = (vbase + grp * 16 * NB + n + halo_y) % ring_rows;  // LDS row of dy = 0
     const int colP = 16 * par + halo_x + 8 * q;
-    for (int dy = -reach; dy <= reach; ++dy) {
-        const int4 sc = sched[dy < 0 ? -dy : dy];  // (e, chunks, first fragment)
-        const uint16_t* brow = tile + (size_t)(row0 + dy) * tstride + colP + sc.x;
-        const uint4* afrag = frag_lds + sc.z * 64;
-        if (sc.y == 3) conv_row<S, NB, 3>(acc, brow, afrag, lane, tstride);
-        else if (sc.y == 2) conv_row<S, NB, 2>(acc, brow, afrag, lane, tstride);
-        else if (sc.y == 1) conv_row<S, NB, 1>(acc, brow, afrag, lane, tstride);
-    }
+    const int a3 = nrow[0], a2 = a3 + nrow[1], a1 = a2 + nrow[2];
+    conv_rows<S, NB, 3>(acc, tile, frag_lds, rows, 0, a3, base, ring_rows, colP, tstride, lane);
+    conv_rows<S, NB, 2>(acc, tile, frag_lds, rows, a3, a2, base, ring_rows, colP, tstride, lane);
+    conv_rows<S, NB, 1>(acc, tile, frag_lds, rows, a2, a1, base, ring_rows, colP, tstride, lane);
 }
 
 // One kernel's potential over the tile: K_hi s_hi, plus K_lo s_hi and
@@ -155,12 +173,12 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[S][NB], const uint16_t* t
 // in LDS (source*2 + lo) so that consecutive kernels on one source skip the
 // restaging; `loaded` tracks the fragment block in LDS the same way.
 template <int S, int NB>
-__device__ __forceinline__ void kernel_potential(f32x4 (&acc)[S][NB], uint16_t* tile, uint4* frag_lds, int& staged,
-                                                 int& loaded,
+__device__ __forceinline__ void kernel_potential(f32x4 (&acc)[S][NB], uint16_t* tile, uint4* frag_lds,
+                                                 int4* rows_lds, int& staged, int& loaded,
                                                  const float* src_plane, int src, int W, int H, int x0, int y0,
                                                  int halo_y, int halo_x, int tr, int tc, int tstride,
                                                  const uint4* __restrict__ frags, int frag0, int nfrag, int terms,
-                                                 const int4* sched, int reach) {
+                                                 const int4* rows, const int (&nrow)[3]) {
 #pragma unroll
     for (int k = 0; k < S; ++k)
 #pragma unroll
@@ -178,16 +196,38 @@ __device__ __forceinline__ void kernel_potential(f32x4 (&acc)[S][NB], uint16_t* 
         }
         if (loaded != fbase) {
             for (int i = threadIdx.x; i < nfrag * 64; i += DTHREADS) frag_lds[i] = frags[(size_t)fbase * 64 + i];
+            const int nr = nrow[0] + nrow[1] + nrow[2];
+            for (int i = threadIdx.x; i < nr; i += DTHREADS) rows_lds[i] = rows[i];
             loaded = fbase;
         }
         __syncthreads();
-        conv_tile<S, NB>(acc, tile, frag_lds, sched, reach, halo_y, halo_x, tstride);
+        conv_tile<S, NB>(acc, tile, frag_lds, rows_lds, nrow, halo_y, halo_x, tstride, tr + 1, 0);
     }
 }
 
+// Epilogue arithmetic shared by the tile, ring and batch kernels.  Each
+// rounding is spelled out (no contraction into FMAs), so the compiler cannot
+// fuse differently in different kernels: the same potential gives the same
+// bits in every variant.
+__device__ __forceinline__ float potential(float acc, float scale) {
+#pragma clang fp contract(off)
+    return acc * scale;
+}
+
 __device__ __forceinline__ float growth(float u, float mu, float inv_sigma) {
-    float d = (u - mu) * inv_sigma;
+#pragma clang fp contract(off)
+    const float d = (u - mu) * inv_sigma;
     return 2.0f * __expf(-0.5f * d * d) - 1.0f;
+}
+
+__device__ __forceinline__ float add_weighted(float g, float weight, float gr) {
+#pragma clang fp contract(off)
+    return g + weight * gr;
+}
+
+__device__ __forceinline__ float lenia_update(float r, float dt, float g) {
+#pragma clang fp contract(off)
+    return fminf(fmaxf(r + dt * g, 0.0f), 1.0f);
 }
 
 // Output (strip k, block b, element r) of this lane: column
@@ -204,6 +244,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
     const int q = lane >> 4, n = lane & 15;
     uint16_t* tile = lds_u16;
     uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)a.tr * a.tstride);
+    int4* rows_lds = reinterpret_cast<int4*>(frag_lds + (size_t)a.nfrag_max * 64);
 
     float g[S][NB][4];
 #pragma unroll
@@ -218,18 +259,18 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
         const DirectKernel kk = a.k[kn];
         if (a.conv_out ? (kn != a.conv_k) : (kk.dst != dst)) continue;
         f32x4 acc[S][NB];
-        kernel_potential<S, NB>(acc, tile, frag_lds, staged, loaded, a.state_in + (size_t)kk.src * W * H, kk.src, W,
-                                H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, a.frags, kk.frag0, kk.nfrag,
-                                kk.terms, a.sched + a.sched_base[kn], kk.reach);
+        kernel_potential<S, NB>(acc, tile, frag_lds, rows_lds, staged, loaded, a.state_in + (size_t)kk.src * W * H,
+                                kk.src, W, H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, a.frags, kk.frag0,
+                                kk.nfrag, kk.terms, a.sched + a.sched_base[kn], kk.nrow);
 #pragma unroll
         for (int k = 0; k < S; ++k)
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const float u = acc[k][b][r] * kk.scale;
+                    const float u = potential(acc[k][b][r], kk.scale);
                     if (a.conv_out) g[k][b][r] = u;
-                    else g[k][b][r] += kk.weight * growth(u, kk.mu, kk.inv_sigma);
+                    else g[k][b][r] = add_weighted(g[k][b][r], kk.weight, growth(u, kk.mu, kk.inv_sigma));
                 }
     }
 
@@ -247,9 +288,101 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
                 float4 sv = *reinterpret_cast<const float4*>(a.state_in + plane + o);
                 float r[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
-                for (int t = 0; t < 4; ++t) r[t] = fminf(fmaxf(r[t] + a.dt * g[k][b][t], 0.0f), 1.0f);
+                for (int t = 0; t < 4; ++t) r[t] = lenia_update(r[t], a.dt, g[k][b][t]);
                 *reinterpret_cast<float4*>(a.state_out + plane + o) = make_float4(r[0], r[1], r[2], r[3]);
             }
+        }
+    }
+}
+
+// Persistent variant for one channel, one kernel and one bf16 term.
+// Workgroup b walks a run of vertically adjacent 64-row tiles of one
+// 32S-column strip (the torus wraps a run past the last tile row).  Two
+// consecutive tiles share 2*halo_y staged rows, so LDS holds a ring of
+// tr + 64 rows: tile i of the run reads ring rows (64i + r) mod (tr + 64),
+// r < tr, and while its MFMAs run the 64 rows tile i+1 adds are loaded into
+// registers and then written to the ring rows tile i-1 used.  Only the run's
+// first tile is staged before its MFMAs; the MFMA order per output is that
+// of k_direct<S, 1>, so both give the same bits.
+template <int S>
+__global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
+    constexpr int DX = 32 * S, DY = 64, NEW_PER_WAVE = DY / (DTHREADS / 64);
+    const int W = a.W, H = a.H, RT = a.tr + DY;
+    const int strips = W / DX, TY = H / DY;
+    const int segs = gridDim.x / strips;
+    const int strip = blockIdx.x / segs, seg = blockIdx.x - strip * segs;
+    const int t0 = seg * TY / segs, t1 = (seg + 1) * TY / segs;
+    const int x0 = strip * DX;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int par = wave & 1, grp = wave >> 1;
+    const int q = lane >> 4, n = lane & 15;
+    uint16_t* ring = lds_u16;
+    uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)RT * a.tstride);
+    int4* rows_lds = reinterpret_cast<int4*>(frag_lds + (size_t)a.nfrag_max * 64);
+    const DirectKernel kk = a.k[0];
+    const int4* rows = a.sched + a.sched_base[0];
+    for (int i = tid; i < kk.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)kk.frag0 * 64 + i];
+    for (int i = tid; i < kk.nrow[0] + kk.nrow[1] + kk.nrow[2]; i += DTHREADS) rows_lds[i] = rows[i];
+    const int ybase = t0 * DY;
+    stage_tile(ring, a.state_in, W, H, x0, ybase, DX, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, false);
+
+    // This lane's column of the streamed rows (the tile is at most 256 wide).
+    const int jn = 4 * lane;
+    const bool col_ok = jn < a.tc;
+    const int cxn = jn - a.halo_x;
+    int gxn = (x0 + cxn) % W;
+    if (gxn < 0) gxn += W;
+
+    for (int t = t0; t < t1; ++t) {
+        const int i = t - t0;
+        const bool more = t + 1 < t1;
+        // Rows 64i + tr + wave + 8u (u < 8) of the run: the next tile's new rows.
+        float4 nv[NEW_PER_WAVE];
+        if (more && col_ok) {
+#pragma unroll
+            for (int u = 0; u < NEW_PER_WAVE; ++u) {
+                const int v = DY * i + a.tr + wave + 8 * u;
+                int gy = (ybase - a.halo_y + v) % H;
+                if (gy < 0) gy += H;
+                nv[u] = *reinterpret_cast<const float4*>(a.state_in + (size_t)gy * W + gxn);
+            }
+        }
+        // This tile's ring rows and the fragments are in LDS.  The barrier
+        // does not wait for the loads just issued: they land during the MFMAs.
+        lds_sync();
+        f32x4 acc[S][1];
+#pragma unroll
+        for (int k = 0; k < S; ++k) acc[k][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        conv_tile<S, 1>(acc, ring, frag_lds, rows_lds, kk.nrow, a.halo_y, a.halo_x, a.tstride, RT, (DY * i) % RT);
+        if (more && col_ok) {
+#pragma unroll
+            for (int u = 0; u < NEW_PER_WAVE; ++u) {
+                const int v = DY * i + a.tr + wave + 8 * u;
+                const float e[4] = {nv[u].x, nv[u].y, nv[u].z, nv[u].w};
+                uint32_t h[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int cx = cxn + c;
+                    h[c] = (cx >= -a.halo_y && cx < DX + a.halo_y) ? bf16_bits(e[c]) : 0u;
+                }
+                *reinterpret_cast<uint2*>(ring + (size_t)(v % RT) * a.tstride + jn) =
+                    make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            }
+        }
+        const int yo = t * DY + grp * 16 + n;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const int xo = x0 + 16 * (par + 2 * k) + 4 * q;
+            const size_t o = (size_t)yo * W + xo;
+            float4 sv = *reinterpret_cast<const float4*>(a.state_in + o);
+            float r[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                r[c] = lenia_update(r[c], a.dt,
+                                    add_weighted(0.0f, kk.weight, growth(potential(acc[k][0][c], kk.scale), kk.mu,
+                                                                         kk.inv_sigma)));
+            *reinterpret_cast<float4*>(a.state_out + o) = make_float4(r[0], r[1], r[2], r[3]);
         }
     }
 }
@@ -263,6 +396,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
     uint16_t* tile = lds_u16;
     uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)a.tr * a.tstride);
+    int4* rows_lds = reinterpret_cast<int4*>(frag_lds + (size_t)a.nfrag_max * 64);
     const float* in = a.parity ? a.buf1 : a.buf0;
     float* out = a.parity ? a.buf0 : a.buf1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -276,9 +410,9 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
         const int x0 = tj.y, y0 = tj.z;
         int staged = -1;  // a new tile always restages
         f32x4 acc[S][NB];
-        kernel_potential<S, NB>(acc, tile, frag_lds, staged, loaded, in + w.off, 0, w.W, w.H, x0, y0, a.halo_y,
-                                a.halo_x, a.tr, a.tc, a.tstride, a.frags, w.frag0, w.nfrag, w.terms, a.sched + w.sched0,
-                                w.reach);
+        kernel_potential<S, NB>(acc, tile, frag_lds, rows_lds, staged, loaded, in + w.off, 0, w.W, w.H, x0, y0,
+                                a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, a.frags, w.frag0, w.nfrag, w.terms,
+                                a.sched + w.sched0, w.nrow);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const int xo = x0 + 16 * (par + 2 * k) + 4 * q;
@@ -290,7 +424,9 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
                 float r[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    r[e] = fminf(fmaxf(r[e] + w.dt * growth(acc[k][b][e] * w.scale, w.mu, w.inv_sigma), 0.0f), 1.0f);
+                    r[e] = lenia_update(r[e], w.dt,
+                                        add_weighted(0.0f, 1.0f, growth(potential(acc[k][b][e], w.scale), w.mu,
+                                                                        w.inv_sigma)));
                 *reinterpret_cast<float4*>(out + o) = make_float4(r[0], r[1], r[2], r[3]);
             }
         }
@@ -311,12 +447,16 @@ void allow_lds(K kernel, size_t bytes) {
 
 }  // namespace
 
+// LDS: the bf16 tile (or ring), the fragment table, the row list.
 size_t direct_lds_bytes(const DirectArgs& a) {
-    return (size_t)a.tr * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4);
+    const int rows = a.ring ? a.tr + 64 : a.tr;
+    return (size_t)rows * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4) +
+           DIRECT_MAX_ROWS * sizeof(int4);
 }
 
 size_t batch_lds_bytes(const BatchArgs& a) {
-    return (size_t)a.tr * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4);
+    return (size_t)a.tr * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4) +
+           DIRECT_MAX_ROWS * sizeof(int4);
 }
 
 template <int S, int NB>
@@ -342,14 +482,31 @@ void launch_direct_shape(hipStream_t s, const DirectArgs& a, int channels, size_
     hipLaunchKernelGGL((k_direct<S, NB>), grid, dim3(DTHREADS), lds, s, a);
 }
 
+template <int S>
+void launch_ring_shape(hipStream_t s, const DirectArgs& a, size_t lds) {
+    const int strips = a.W / (32 * S), TY = a.H / 64;
+    const int segs = std::max(1, std::min(TY, (cu_count() + strips / 2) / strips));
+    allow_lds(k_direct_ring<S>, lds);
+    hipLaunchKernelGGL((k_direct_ring<S>), dim3(strips * segs), dim3(DTHREADS), lds, s, a);
+}
+
 void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels) {
-    const size_t lds = direct_lds_bytes(a);
     const int cols = a.cols, rows = a.rows;
     Launch L(p, s, a.conv_out ? "direct_bf16_conv" : "direct_bf16_update");
-    if (cols == 128 && rows == 128) launch_direct_shape<4, 2>(s, a, channels, lds);
-    else if (cols == 128) launch_direct_shape<4, 1>(s, a, channels, lds);
-    else if (rows == 128) launch_direct_shape<2, 2>(s, a, channels, lds);
-    else launch_direct_shape<2, 1>(s, a, channels, lds);
+    if (a.ring && !a.conv_out) {
+        const size_t lds = direct_lds_bytes(a);
+        if (cols == 128) launch_ring_shape<4>(s, a, lds);
+        else launch_ring_shape<2>(s, a, lds);
+        AH_HIP(hipGetLastError());
+        return;
+    }
+    DirectArgs plain = a;  // the potential diagnostic runs the tile kernel of the same shape
+    plain.ring = 0;
+    const size_t lds = direct_lds_bytes(plain);
+    if (cols == 128 && rows == 128) launch_direct_shape<4, 2>(s, plain, channels, lds);
+    else if (cols == 128) launch_direct_shape<4, 1>(s, plain, channels, lds);
+    else if (rows == 128) launch_direct_shape<2, 2>(s, plain, channels, lds);
+    else launch_direct_shape<2, 1>(s, plain, channels, lds);
     AH_HIP(hipGetLastError());
 }
 

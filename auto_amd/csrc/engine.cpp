@@ -98,11 +98,12 @@ double bf16_to_f64(uint16_t h) {
 // e = -8*ceil(dxm/8) (16-byte aligned B reads) and needs
 // floor((dxm + 15 - e)/32) + 1 chunks of 32 input columns.  Fragment lane l,
 // element t holds bf16(K[dy][e + 32c + 8q + t - i]) with i = l%16, q = l/16.
-// Appends (e, chunks, first fragment, 0) per dy in [0, R] to `sched`, the hi
-// fragments to `frags` and, for three terms, the lo fragments
-// bf16(K - bf16(K)) right after them (same schedule, offset nfrag).
-double build_fragments(const std::vector<double>& t, int R, int terms, std::vector<int>& sched,
-                       std::vector<uint16_t>& frags, int& nfrag) {
+// Fragments are numbered by |dy| (rows dy and -dy share them).  Appends the
+// row list (dy, e, first fragment, 0) of the rows with 3, then 2, then 1
+// chunks to `rows`, the hi fragments to `frags` and, for three terms, the lo
+// fragments bf16(K - bf16(K)) right after them (same numbering, offset nfrag).
+double build_fragments(const std::vector<double>& t, int R, int terms, std::vector<int>& rowlist,
+                       std::vector<uint16_t>& frags, int& nfrag, int nrow[3]) {
     const int side = 2 * R + 1;
     std::vector<uint16_t> hi(t.size()), lo(t.size());
     double sum = 0.0;
@@ -112,7 +113,7 @@ double build_fragments(const std::vector<double>& t, int R, int terms, std::vect
         sum += bf16_to_f64(hi[i]);
         if (terms == 3) sum += bf16_to_f64(lo[i]);
     }
-    std::vector<int> rows;  // (e, chunks) per dy
+    std::vector<int> rows;  // (e, chunks, first fragment) per |dy|
     int nf = 0;
     for (int dy = 0; dy <= R; ++dy) {
         int dxm = -1;
@@ -123,14 +124,23 @@ double build_fragments(const std::vector<double>& t, int R, int terms, std::vect
             e = -8 * ((dxm + 7) / 8);
             chunks = (dxm + 15 - e) / 32 + 1;
         }
-        sched.insert(sched.end(), {e, chunks, nf, 0});
-        rows.insert(rows.end(), {e, chunks});
+        if (chunks > 3) throw Error{AH_ERR_UNSUPPORTED, "direct path: more than 3 chunks in a kernel row"};
+        rows.insert(rows.end(), {e, chunks, nf});
         nf += chunks;
+    }
+    for (int c = 3; c >= 1; --c) {
+        nrow[3 - c] = 0;
+        for (int dy = -R; dy <= R; ++dy) {
+            const int* r = &rows[3 * std::abs(dy)];
+            if (r[1] != c) continue;
+            rowlist.insert(rowlist.end(), {dy, r[0], r[2], 0});
+            ++nrow[3 - c];
+        }
     }
     for (int part = 0; part < (terms == 3 ? 2 : 1); ++part) {
         const std::vector<uint16_t>& tb = part ? lo : hi;
         for (int dy = 0; dy <= R; ++dy) {
-            const int e = rows[2 * dy], chunks = rows[2 * dy + 1];
+            const int e = rows[3 * dy], chunks = rows[3 * dy + 1];
             for (int c = 0; c < chunks; ++c)
                 for (int lane = 0; lane < 64; ++lane) {
                     const int i = lane & 15, q = lane >> 4;
@@ -191,6 +201,7 @@ struct ah_sim {
 
     // direct path
     DirectArgs direct{};
+    std::vector<ah_direct_tile> direct_tiles;  // admissible variants
 
     // stats
     double* pmass = nullptr;
@@ -285,6 +296,18 @@ void validate(const ah_lenia_desc& d) {
 }
 
 
+// LDS tile geometry of one direct-path variant: tr rows x tc bf16 columns,
+// row stride tc + 8 (the 16-byte B reads of 16 consecutive rows then start
+// in distinct banks).
+void direct_shape(DirectArgs& a, int cols, int rows, int ring) {
+    a.cols = cols;
+    a.rows = rows;
+    a.ring = ring;
+    a.tr = rows + 2 * a.halo_y;
+    a.tc = cols + 2 * a.halo_x + 32;
+    a.tstride = a.tc + 8;
+}
+
 // Fragments and row schedule of every kernel for the direct path, from the
 // FP64 taps on the device.  For kernel row dy with chord half-width dxm the
 // window starts at e = -8*ceil(dxm/8) (16-byte aligned B reads) and needs
@@ -303,13 +326,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     a.halo_y = rmax;
     a.halo_x = 8 * ((rmax + 7) / 8);
     // Output tile: the largest of 128x128, 128x64, 64x64 that the world and LDS allow.
-    auto shape = [&](int cols, int rows) {
-        a.cols = cols;
-        a.rows = rows;
-        a.tr = rows + 2 * a.halo_y;
-        a.tc = cols + 2 * a.halo_x + 32;
-        a.tstride = a.tc + 8;
-    };
+    auto shape = [&](int cols, int rows, int ring) { direct_shape(a, cols, rows, ring); };
     a.n_kernels = K;
     std::vector<int> sched;
     std::vector<uint16_t> frags;  // [fragment][64][8]
@@ -323,7 +340,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         dk.frag0 = nfrag_total;
         a.sched_base[k] = (int)(sched.size() / 4);
         dk.terms = direct_terms(t, d.dt, s->kp[k].weight, d.kernels[k].sigma, d.path);
-        dk.scale = (float)(1.0 / build_fragments(t, R, dk.terms, sched, frags, dk.nfrag));
+        dk.scale = (float)(1.0 / build_fragments(t, R, dk.terms, sched, frags, dk.nfrag, dk.nrow));
         dk.src = (int)d.kernels[k].src;
         dk.dst = (int)d.kernels[k].dst;
         dk.mu = s->kp[k].mu;
@@ -332,12 +349,23 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         nfrag_total += dk.nfrag * (dk.terms == 3 ? 2 : 1);
         a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
     }
-    std::vector<std::pair<int, int>> shapes;
+    // Candidates: the tile kernel at 128x128, 128x64, 64x64, and for one
+    // channel, one kernel and one bf16 term the persistent ring kernel at
+    // 64 rows (128 or 64 columns).
+    using Shape = ah_direct_tile;
+    std::vector<Shape> shapes;
+    const bool ring_ok = K == 1 && d.channels == 1 && a.k[0].terms == 1;
     for (auto cr : {std::make_pair(128, 128), std::make_pair(128, 64), std::make_pair(64, 64)}) {
         if (cr.first > (int)d.width || cr.second > (int)d.height) continue;
-        shape(cr.first, cr.second);
-        if (direct_lds_bytes(a) <= 160 * 1024) shapes.push_back(cr);
+        shape(cr.first, cr.second, 0);
+        if (direct_lds_bytes(a) <= 160 * 1024) shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 0});
     }
+    for (int cols : {128, 64}) {
+        if (!ring_ok || cols > (int)d.width || d.width % cols || d.height % 64) continue;
+        shape(cols, 64, 1);
+        if (a.tc <= 256 && direct_lds_bytes(a) <= 160 * 1024) shapes.push_back({(uint32_t)cols, 64, 1});
+    }
+    s->direct_tiles = shapes;
     if (shapes.empty())
         throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path: kernel tile and fragments exceed 160 KiB of LDS"};
     int4* dsched = (int4*)s->alloc(sizeof(int) * sched.size());
@@ -352,7 +380,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     // otherwise each admissible shape is timed on this world (zeroed states,
     // the work is data-independent) and the fastest kept.  Small worlds can
     // prefer small tiles: more workgroups than CUs.
-    shape(shapes[0].first, shapes[0].second);
+    shape(shapes[0].cols, shapes[0].rows, shapes[0].ring);
     if (shapes.size() > 1) {
         const size_t bytes = sizeof(float) * s->cells() * d.channels;
         AH_HIP(hipMemsetAsync(s->state, 0, bytes, s->stream));
@@ -361,10 +389,10 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         AH_HIP(hipEventCreate(&e0));
         AH_HIP(hipEventCreate(&e1));
         double best = 1e300;
-        std::pair<int, int> pick = shapes[0];
+        Shape pick = shapes[0];
         try {
-            for (auto cr : shapes) {
-                shape(cr.first, cr.second);
+            for (const Shape& cr : shapes) {
+                shape(cr.cols, cr.rows, cr.ring);
                 DirectArgs t = a;
                 t.state_in = s->state;
                 t.state_out = s->state_alt;
@@ -387,7 +415,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        shape(pick.first, pick.second);
+        shape(pick.cols, pick.rows, pick.ring);
     }
 }
 
@@ -751,6 +779,29 @@ ah_status ah_sim_info(const ah_sim* s, ah_info* out) {
 }
 
 void ah_sim_destroy(ah_sim* s) { delete s; }
+
+ah_status ah_sim_direct_tile(const ah_sim* s, ah_direct_tile* out) {
+    return guard([&] {
+        if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        if (s->d.path == AH_PATH_FFT_FP32) throw Error{AH_ERR_INVALID, "FFT sims have no direct-path tile"};
+        *out = ah_direct_tile{(uint32_t)s->direct.cols, (uint32_t)s->direct.rows, (uint32_t)s->direct.ring};
+    });
+}
+
+ah_status ah_sim_set_direct_tile(ah_sim* s, const ah_direct_tile* t) {
+    return guard([&] {
+        if (!s || !t) throw Error{AH_ERR_INVALID, "null argument"};
+        if (s->d.path == AH_PATH_FFT_FP32) throw Error{AH_ERR_INVALID, "FFT sims have no direct-path tile"};
+        for (const ah_direct_tile& c : s->direct_tiles)
+            if (c.cols == t->cols && c.rows == t->rows && c.ring == t->ring) {
+                AH_HIP(hipStreamSynchronize(s->stream));
+                direct_shape(s->direct, (int)c.cols, (int)c.rows, (int)c.ring);
+                return;
+            }
+        throw Error{AH_ERR_UNSUPPORTED, "direct tile " + std::to_string(t->cols) + "x" + std::to_string(t->rows) +
+                                            (t->ring ? " (ring)" : "") + " is not admissible for this world"};
+    });
+}
 
 ah_status ah_sim_profile(ah_sim* s, int enable) {
     return guard([&] {

@@ -44,13 +44,9 @@ __device__ __forceinline__ float2 mul_si(float2 a) {
     return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
-// Workgroup barrier for LDS traffic only: this wave's LDS operations finish,
-// then all waves meet.  Unlike __syncthreads() it does not wait for global
-// loads and stores in flight (its release fence would), so global stores and
-// global->LDS prefetches keep running across FFT stages.  Nothing in this
-// file exchanges global memory between the threads of a workgroup.  The
-// memory clobber keeps the compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// lds_sync() (ah_internal.h) keeps global stores and global->LDS prefetches
+// running across FFT stages.  Nothing in this file exchanges global memory
+// between the threads of a workgroup.
 
 // log2 of a power of two.  Every size here (row and column lengths, rows and
 // columns per workgroup, stage spans) is one, so index math is shifts and
@@ -866,8 +862,6 @@ bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
     return channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == SH && W >= 64;
 }
 
-namespace {
-
 // Compute units of the current device (cached per device).
 int cu_count() {
     static int cached[64] = {};
@@ -877,6 +871,8 @@ int cu_count() {
     if (!cached[dev]) AH_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount, dev));
     return cached[dev];
 }
+
+namespace {
 
 // Columns per column-pass workgroup (1, 2 or 4 of a 4-column block).
 int col_cols(int H) {

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import LeniaDesc, Info, Stats, KernelTime, check, load
+from ._abi import LeniaDesc, Info, Stats, KernelTime, DirectTile, check, load
 from . import workloads
 
 
@@ -117,6 +117,18 @@ class Sim:
         check(self._lib.ah_sim_render(self._h, out_w, out_h, float(center[0]), float(center[1]), float(scale),
                                       out.ctypes.data, out.nbytes))
         return out
+
+    # -- direct-path kernel variant ----------------------------------------
+    def direct_tile(self) -> tuple:
+        """(cols, rows, ring) of the direct-path kernel variant in use."""
+        t = DirectTile()
+        check(self._lib.ah_sim_direct_tile(self._h, ctypes.byref(t)))
+        return (t.cols, t.rows, t.ring)
+
+    def set_direct_tile(self, cols: int, rows: int, ring: int = 0) -> None:
+        """Force a direct-path variant (raises if not admissible here)."""
+        t = DirectTile(cols, rows, ring)
+        check(self._lib.ah_sim_set_direct_tile(self._h, ctypes.byref(t)))
 
     # -- kernel timing ----------------------------------------------------
     def profile(self, enable: bool = True, every: int = 1) -> None:

@@ -227,6 +227,24 @@ ah_status ah_sim_profile(ah_sim* sim, int enable);
 ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacity, uint32_t* count);
 
 /*
+ * Direct-path kernel variant.  No counterpart in the reference (its only
+ * path is the FFT).  A DIRECT_BF16* sim times every admissible variant on its
+ * world at create and keeps the fastest: the tile kernel with a cols x rows
+ * output tile (128x128, 128x64, 64x64), or, for one channel, one kernel and
+ * one bf16 term, the persistent ring kernel (ring = 1, rows = 64) that
+ * carries the shared halo rows from one tile to the next.  Every variant
+ * gives the same bits.  ah_sim_direct_tile reports the variant in use;
+ * ah_sim_set_direct_tile forces one (AH_ERR_UNSUPPORTED if it is not
+ * admissible for this world; AH_ERR_INVALID on an FFT sim).
+ */
+typedef struct {
+    uint32_t cols, rows, ring;
+} ah_direct_tile;
+
+ah_status ah_sim_direct_tile(const ah_sim* sim, ah_direct_tile* out);
+ah_status ah_sim_set_direct_tile(ah_sim* sim, const ah_direct_tile* tile);
+
+/*
  * Batched uneven worlds.  No counterpart in the reference, which runs one
  * world per Sim (sim.rs:202-309).  Every world is single-channel with one
  * Lenia kernel, width and height >= 64, and an ORBIUM, PATCH or ZERO init

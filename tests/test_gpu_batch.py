@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from auto_amd import Sim, Batch, workloads as W
-from auto_amd._abi import EngineError, LeniaDesc, PATH_FFT_FP32
+from auto_amd._abi import EngineError, LeniaDesc, PATH_DIRECT_BF16, PATH_FFT_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,15 @@ def test_batch_fft_and_auto_worlds(O):
         if i < 5 or i == 10:
             plain = LeniaDesc.from_buffer_copy(d)
             plain.path = PATH_FFT_FP32
+            sim = Sim(plain, seed)
+            sim.write_field("state", before[i][None])
+            sim.step(1)
+            assert np.array_equal(got, sim.read_field("state")[0]), i
+        else:
+            # A direct world equals a standalone direct Sim: the same MFMA
+            # order per cell and the same epilogue arithmetic.
+            plain = LeniaDesc.from_buffer_copy(d)
+            plain.path = PATH_DIRECT_BF16
             sim = Sim(plain, seed)
             sim.write_field("state", before[i][None])
             sim.step(1)

@@ -141,3 +141,72 @@ def test_direct_auto_precision_meets_gate_on_high_gain_rule(O):
     print(errs)
     assert errs["direct_bf16"] <= 1e-4
     assert errs["direct_bf16"] < errs["direct_bf16_x1"]
+
+
+VARIANTS = [(128, 128, 0), (128, 64, 0), (64, 64, 0), (128, 64, 1), (64, 64, 1)]
+
+
+def _run_variants(d, s, steps):
+    """State after `steps` steps from s under every admissible variant."""
+    out = {}
+    for v in VARIANTS:
+        sim = Sim(d, 1)
+        try:
+            sim.set_direct_tile(*v)
+        except EngineError:
+            continue
+        assert sim.direct_tile() == v
+        sim.write_field("state", s)
+        sim.step(steps)
+        out[v] = sim.read_field("state")
+    return out
+
+
+@pytest.mark.parametrize("w,h,radius", [(64, 64, 9.0), (256, 128, 13.0), (2048, 2048, 26.0), (1024, 4096, 31.0)])
+def test_direct_tile_variants_agree_bitwise(O, w, h, radius):
+    """Every admissible kernel variant (three tile shapes and the persistent
+    ring kernel at two widths) runs the same MFMA sequence per cell, so steps
+    from the same state give the same bits.  At 2048^2 and 1024x4096 a ring
+    workgroup walks 2-4 tiles, so the halo rows it carries from one tile to
+    the next (and the rows it streams in during the MFMAs) must equal what the
+    tile kernel restages.  The ring kernel's step is also checked against FP64."""
+    d = W.desc(w, h, [dict(radius=radius, beta=[1.0], mu=0.15, sigma=0.015)], 0.1, path="direct_bf16_x1",
+               init=INIT_ZERO)
+    s = np.random.default_rng(w + 3 * h).random((1, h, w), dtype=np.float32)
+    got = _run_variants(d, s, 4)
+    assert (128, 64, 0) in got or (64, 64, 0) in got
+    rings = [v for v in got if v[2]]
+    assert rings, "the ring kernel is admissible for one channel, one kernel, one term"
+    ref = next(iter(got.values()))
+    for v, g in got.items():
+        assert np.array_equal(g, ref), v
+    sim = Sim(d, 1)
+    sim.set_direct_tile(*rings[0])
+    sim.write_field("state", s)
+    sim.step(1)
+    err = np.abs(sim.read_field("state") - O.step64(d, s, "fft")).max()
+    print(f"{w}x{h} R={radius}: variants {sorted(got)}; ring step max-abs vs FP64 {err:.3e}")
+    assert err <= 1e-3
+
+
+def test_direct_ring_4096_matches_tile_kernel():
+    """4096^2 three-ring world: ring workgroups walk 8 tiles each."""
+    d = W.rings3_4096(path="direct_bf16")
+    a, b = Sim(d, 5), Sim(d, 5)
+    a.set_direct_tile(128, 64, 0)
+    b.set_direct_tile(128, 64, 1)
+    a.step(3)
+    b.step(3)
+    assert np.array_equal(a.read_field("state"), b.read_field("state"))
+
+
+def test_direct_ring_not_admissible_for_three_terms_or_channels():
+    d = W.desc(256, 256, [dict(radius=13.0, beta=[1.0], mu=0.15, sigma=0.015)], 0.1, path="direct_bf16_x3",
+               init=INIT_ZERO)
+    sim = Sim(d, 1)
+    with pytest.raises(EngineError):
+        sim.set_direct_tile(128, 64, 1)
+    assert sim.direct_tile()[2] == 0
+    m = Sim(W.multichannel_1024(seed=4, side=256, path="direct_bf16"), 2)
+    with pytest.raises(EngineError):
+        m.set_direct_tile(128, 64, 1)
