@@ -106,45 +106,75 @@ __device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int
     }
 }
 
-// One kernel row with NCH chunks: NCH A fragments, then for every shared
-// chunk u of the strips' joint window the NB B fragments, each feeding the
-// strips k whose own chunk u-k exists.
+// One kernel row's operands: NCH A fragments and, for every chunk u of the
+// S strips' joint window, NB B fragments.
 template <int S, int NB, int NCH>
-__device__ __forceinline__ void conv_row(f32x4 (&acc)[S][NB], const uint16_t* brow, const uint4* afrag, int lane,
-                                         int tstride) {
+struct RowFrags {
     bf16x8 A[NCH];
+    bf16x8 B[NCH + S - 1][NB];
+};
+
+template <int S, int NB, int NCH>
+__device__ __forceinline__ void load_row(RowFrags<S, NB, NCH>& f, const uint16_t* brow, const uint4* afrag, int lane,
+                                         int tstride) {
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) A[c] = __builtin_bit_cast(bf16x8, afrag[c * 64 + lane]);
+    for (int c = 0; c < NCH; ++c) f.A[c] = __builtin_bit_cast(bf16x8, afrag[c * 64 + lane]);
 #pragma unroll
-    for (int u = 0; u < NCH + S - 1; ++u) {
+    for (int u = 0; u < NCH + S - 1; ++u)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const bf16x8 bv =
+        for (int b = 0; b < NB; ++b)
+            f.B[u][b] =
                 __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(brow + (size_t)16 * b * tstride + 32 * u));
+}
+
+// The MFMAs of one row: B fragment u feeds the strips k whose own chunk
+// u - k exists.  Per accumulator the chunks run in increasing order.
+template <int S, int NB, int NCH>
+__device__ __forceinline__ void mfma_row(f32x4 (&acc)[S][NB], const RowFrags<S, NB, NCH>& f) {
+#pragma unroll
+    for (int u = 0; u < NCH + S - 1; ++u)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 const int c = u - k;
-                if (c >= 0 && c < NCH) acc[k][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], bv, acc[k][b], 0, 0, 0);
+                if (c >= 0 && c < NCH)
+                    acc[k][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.A[c], f.B[u][b], acc[k][b], 0, 0, 0);
             }
-        }
-    }
 }
 
-// The rows [i0, i1) of the row list, which all have NCH chunks.  The next
-// row's entry is read from LDS while this row's MFMAs run.
+// The rows [i0, i1) of the row list, which all have NCH chunks.  Software
+// pipelined over two register sets: the LDS reads of row i+1 are issued
+// before the MFMAs of row i, and row entries are read two rows ahead.
 template <int S, int NB, int NCH>
 __device__ __forceinline__ void conv_rows(f32x4 (&acc)[S][NB], const uint16_t* tile, const uint4* frag_lds,
                                           const int4* rows, int i0, int i1, int base, int ring_rows, int colP,
                                           int tstride, int lane) {
     if (i0 >= i1) return;
-    int4 nxt = rows[i0];
-    for (int i = i0; i < i1; ++i) {
-        const int4 rc = nxt;  // (dy, e, first fragment, 0)
-        if (i + 1 < i1) nxt = rows[i + 1];
-        int slot = base + rc.x;
+    auto brow = [&](const int4& rc) {
+        int slot = base + rc.x;  // rc = (dy, e, first fragment, 0)
         if (slot < 0) slot += ring_rows;
         else if (slot >= ring_rows) slot -= ring_rows;
-        conv_row<S, NB, NCH>(acc, tile + (size_t)slot * tstride + colP + rc.y, frag_lds + rc.z * 64, lane, tstride);
+        return tile + (size_t)slot * tstride + colP + rc.y;
+    };
+    RowFrags<S, NB, NCH> f0, f1;
+    int4 rc = rows[i0];
+    load_row(f0, brow(rc), frag_lds + rc.z * 64, lane, tstride);
+    if (i0 + 1 < i1) rc = rows[i0 + 1];
+    for (int i = i0; i < i1; i += 2) {
+        // f0 holds row i; rc is row i + 1's entry when that row exists.
+        if (i + 1 < i1) {
+            load_row(f1, brow(rc), frag_lds + rc.z * 64, lane, tstride);
+            if (i + 2 < i1) rc = rows[i + 2];
+        }
+        mfma_row(acc, f0);
+        if (i + 1 < i1) {
+            if (i + 2 < i1) {
+                load_row(f0, brow(rc), frag_lds + rc.z * 64, lane, tstride);
+                if (i + 3 < i1) rc = rows[i + 3];
+            }
+            mfma_row(acc, f1);
+        }
     }
 }
 
