@@ -202,6 +202,21 @@ size_t batch_lds_bytes(const BatchArgs& a);
 void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int cols, int rows, int grid);
 int cu_count();  // compute units of the current device
 
+// Row stride (bf16 elements) of a direct-path LDS tile at least tc wide:
+// the smallest multiple of 8 elements (16 bytes) that is 2 or 14 mod 16 in
+// 16-byte units.  The B fragments of one MFMA read 16 consecutive tile rows
+// at one column; with this stride their 16-byte reads fall in distinct bank
+// groups (measured at 4096^2, R = 26: 121 us per step against 129 us for a
+// stride of 13 units and 292 us for 32 units, where all rows share banks).
+// Where that stride does not fit the LDS, `compact` gives tc + 8 (an odd
+// unit count, the next best).
+inline int direct_tile_stride(int tc, bool compact) {
+    if (compact) return tc + 8;
+    int u = (tc + 7) / 8;
+    while (u % 16 != 2 && u % 16 != 14) ++u;
+    return 8 * u;
+}
+
 // Workgroup barrier for LDS traffic only: this wave's LDS operations finish,
 // then all waves meet.  Unlike __syncthreads() it does not wait for global
 // loads and stores in flight (its release fence would), so loads issued

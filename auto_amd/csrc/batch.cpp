@@ -112,8 +112,9 @@ struct ah_batch {
         a.halo_x = halo_x;
         a.tr = rows + 2 * halo_y;
         a.tc = cols + 2 * halo_x + 32;
-        a.tstride = a.tc + 8;
         a.nfrag_max = nfrag_max;
+        a.tstride = direct_tile_stride(a.tc, false);
+        if (batch_lds_bytes(a) > 160 * 1024) a.tstride = direct_tile_stride(a.tc, true);
         return a;
     }
 };

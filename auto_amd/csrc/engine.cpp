@@ -296,16 +296,15 @@ void validate(const ah_lenia_desc& d) {
 }
 
 
-// LDS tile geometry of one direct-path variant: tr rows x tc bf16 columns,
-// row stride tc + 8 (the 16-byte B reads of 16 consecutive rows then start
-// in distinct banks).
+// LDS tile geometry of one direct-path variant: tr rows x tc bf16 columns.
 void direct_shape(DirectArgs& a, int cols, int rows, int ring) {
     a.cols = cols;
     a.rows = rows;
     a.ring = ring;
     a.tr = rows + 2 * a.halo_y;
     a.tc = cols + 2 * a.halo_x + 32;
-    a.tstride = a.tc + 8;
+    a.tstride = direct_tile_stride(a.tc, false);
+    if (direct_lds_bytes(a) > 160 * 1024) a.tstride = direct_tile_stride(a.tc, true);
 }
 
 // Fragments and row schedule of every kernel for the direct path, from the
