@@ -145,33 +145,40 @@ __device__ __forceinline__ void mfma_row(f32x4 (&acc)[S][NB], const RowFrags<S, 
 
 // The rows [i0, i1) of the row list, which all have NCH chunks.  Software
 // pipelined over two register sets: the LDS reads of row i+1 are issued
-// before the MFMAs of row i, and row entries are read two rows ahead.
+// before the MFMAs of row i.  Lane i of `ent` holds row i's entry
+// (dy, e, first fragment), read out with readlane, so no row waits on LDS
+// for its own entry.
 template <int S, int NB, int NCH>
 __device__ __forceinline__ void conv_rows(f32x4 (&acc)[S][NB], const uint16_t* tile, const uint4* frag_lds,
-                                          const int4* rows, int i0, int i1, int base, int ring_rows, int colP,
+                                          const int4& ent, int i0, int i1, int base, int ring_rows, int colP,
                                           int tstride, int lane) {
     if (i0 >= i1) return;
-    auto brow = [&](const int4& rc) {
-        int slot = base + rc.x;  // rc = (dy, e, first fragment, 0)
+    auto ptrs = [&](int i, const uint16_t*& brow, const uint4*& afrag) {
+        const int dy = __builtin_amdgcn_readlane(ent.x, i);
+        const int e = __builtin_amdgcn_readlane(ent.y, i);
+        const int fr = __builtin_amdgcn_readlane(ent.z, i);
+        int slot = base + dy;
         if (slot < 0) slot += ring_rows;
         else if (slot >= ring_rows) slot -= ring_rows;
-        return tile + (size_t)slot * tstride + colP + rc.y;
+        brow = tile + (size_t)slot * tstride + colP + e;
+        afrag = frag_lds + fr * 64;
     };
     RowFrags<S, NB, NCH> f0, f1;
-    int4 rc = rows[i0];
-    load_row(f0, brow(rc), frag_lds + rc.z * 64, lane, tstride);
-    if (i0 + 1 < i1) rc = rows[i0 + 1];
+    const uint16_t* brow;
+    const uint4* afrag;
+    ptrs(i0, brow, afrag);
+    load_row(f0, brow, afrag, lane, tstride);
     for (int i = i0; i < i1; i += 2) {
-        // f0 holds row i; rc is row i + 1's entry when that row exists.
+        // f0 holds row i.
         if (i + 1 < i1) {
-            load_row(f1, brow(rc), frag_lds + rc.z * 64, lane, tstride);
-            if (i + 2 < i1) rc = rows[i + 2];
+            ptrs(i + 1, brow, afrag);
+            load_row(f1, brow, afrag, lane, tstride);
         }
         mfma_row(acc, f0);
         if (i + 1 < i1) {
             if (i + 2 < i1) {
-                load_row(f0, brow(rc), frag_lds + rc.z * 64, lane, tstride);
-                if (i + 3 < i1) rc = rows[i + 3];
+                ptrs(i + 2, brow, afrag);
+                load_row(f0, brow, afrag, lane, tstride);
             }
             mfma_row(acc, f1);
         }
@@ -193,9 +200,10 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[S][NB], const uint16_t* t
     const int base = (vbase + grp * 16 * NB + n + halo_y) % ring_rows;  // LDS row of dy = 0
     const int colP = 16 * par + halo_x + 8 * q;
     const int a3 = nrow[0], a2 = a3 + nrow[1], a1 = a2 + nrow[2];
-    conv_rows<S, NB, 3>(acc, tile, frag_lds, rows, 0, a3, base, ring_rows, colP, tstride, lane);
-    conv_rows<S, NB, 2>(acc, tile, frag_lds, rows, a3, a2, base, ring_rows, colP, tstride, lane);
-    conv_rows<S, NB, 1>(acc, tile, frag_lds, rows, a2, a1, base, ring_rows, colP, tstride, lane);
+    const int4 ent = lane < a1 ? rows[lane] : int4{0, 0, 0, 0};  // at most 63 rows
+    conv_rows<S, NB, 3>(acc, tile, frag_lds, ent, 0, a3, base, ring_rows, colP, tstride, lane);
+    conv_rows<S, NB, 2>(acc, tile, frag_lds, ent, a3, a2, base, ring_rows, colP, tstride, lane);
+    conv_rows<S, NB, 1>(acc, tile, frag_lds, ent, a2, a1, base, ring_rows, colP, tstride, lane);
 }
 
 // One kernel's potential over the tile: K_hi s_hi, plus K_lo s_hi and
