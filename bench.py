@@ -334,19 +334,34 @@ def cpu_leg(d, sim, budget_s: float, time_cpu: bool = True):
                       "rel_l2": float(np.linalg.norm(diff)) / norm if norm > 0 else None,
                       "live_cells": int(np.count_nonzero(want))}
     spectra = O.kernel_spectra32(d)
-    t0 = time.perf_counter()
-    n = 0
-    x = s
-    while True:
-        x = O.step_ref32(d, x, spectra)
-        n += 1
-        if time.perf_counter() - t0 >= budget_s or n >= 50:
-            break
-    cpu_s = time.perf_counter() - t0
     cells = d.width * d.height
+
+    def timed(budget):
+        t0 = time.perf_counter()
+        n, x = 0, s
+        while True:
+            x = O.step_ref32(d, x, spectra)
+            n += 1
+            if time.perf_counter() - t0 >= budget or n >= 50:
+                return n, time.perf_counter() - t0
+
+    n, cpu_s = timed(budget_s)
+    # One thread as well (the reference times its CPU engines both ways,
+    # crates/auto-cli/src/cpu.rs:78-110); a shorter sample.
+    O.set_threads(1)
+    n1, cpu1_s = timed(budget_s / 2)
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     base = {"value": cells * n / cpu_s, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} step(s) of the same {d.width}x{d.height} world on {threads} OpenMP threads "
-                      f"(C restatement of the reference FP32 Stockham step, {cpu_s:.1f} s)"}
+                      f"(C restatement of the reference FP32 Stockham step, {cpu_s:.1f} s)",
+            "single_thread": {"value": cells * n1 / cpu1_s, "steps": n1, "seconds": cpu1_s},
+            "cpu_model": model, "os_cpu_count": os.cpu_count()}
     # Error of one GPU step vs FP64 from the same input state.
     sim.step(1)
     got = sim.read_field("state")
