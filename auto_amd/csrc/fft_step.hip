@@ -25,6 +25,7 @@
 // packs the two real bins X[0] and X[W/2] as (re, im); the column pass treats
 // that column as two real sequences sharing one complex transform.
 #include <algorithm>
+#include <type_traits>
 
 #include "ah_internal.h"
 
@@ -727,88 +728,109 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
     }
 }
 
-// Pass 3 fused with the next pass 1 for W = 4096 (rows of M = 2048 complex),
-// one channel, one kernel, the Lenia rule: k_row_c2r<8, true, LENIA> with
-// the middle in registers.  The last inverse stage (radix 4, span 512) leaves
-// thread t holding row positions t + 256 m, m = 0..7 (two butterflies, four
-// outputs each); complex position p is cells 2p and 2p+1, so the thread
-// applies the growth to exactly those cells; and the first forward stage
-// (radix 8, span 1) reads positions t + 256 m, the same set.  The LDS round
-// trips of the epilogue and of that stage disappear.  The arithmetic is
-// unchanged, so results are bitwise those of k_row_c2r.
-constexpr int RM = 2048, RT = RM / 8;  // complex per row, threads
+// Pass 3 fused with the next pass 1 for the Lenia rule with one kernel per
+// destination channel (a single world, or batched same-size worlds in
+// diagonal mode): k_row_c2r<8, true, LENIA> with the middle in registers.
+// Rows of M complex (W = 2M, 128 <= W <= 4096), 2048/M rows per workgroup
+// of 256 threads, TPR = M/8 threads per row.  The inverse FFT's last stage
+// (radix RL = 8, 4 or 2, span L = M/RL) leaves thread t of a row holding
+// positions t + TPR m, m = 0..7; complex position p is cells 2p and 2p+1,
+// so the thread applies the growth to exactly those cells; and the forward
+// FFT's first stage (radix 8, span 1) reads positions t + TPR m, the same
+// set.  The LDS round trips of the epilogue and of that stage disappear.
+template <int M>
+struct RowPlan {
+    static constexpr int TPR = M / 8, ROWS = 2048 / M;
+    static constexpr int N8 = M >= 512 ? 3 : 2;                   // radix-8 stages with 8^N8 <= M
+    static constexpr int RLAST = M / (N8 == 3 ? 512 : 64);        // 1, 2 or 4
+    static constexpr int RL = RLAST == 1 ? 8 : RLAST, L = M / RL, NBL = 8 / RL;
+    static constexpr int NLDS = RLAST == 1 ? N8 - 1 : N8;         // inverse stages through LDS
+    static_assert(M >= 64 && M <= 2048 && (M & (M - 1)) == 0, "row length");
+};
 
-// Pass-3 input of one 4096-wide row, twisted on the way in: thread t loads
-// bins kk = t + 256 i and M - kk of the product spectrum straight from memory
-// and writes Z[kk], Z[M-kk] of load_c2r (kk = 0 also takes the packed DC /
-// Nyquist bin and the self-paired middle bin M/2), saving an LDS pass.
-__device__ __forceinline__ void load_twist_row(float2* lds, const RowArgs& a, int y0) {
-    const int t = threadIdx.x, H = a.H;
-    constexpr int half = RM / 2;
-    auto bin = [&](int k) { return a.spec[((size_t)(k >> 2) * H + y0) * BC + (k & 3)]; };
+// Pass-3 input of the workgroup's rows, twisted on the way in: thread t of
+// a row loads bins kk = t + TPR i and M - kk of the product spectrum straight
+// from memory and writes Z[kk], Z[M-kk] of load_c2r (kk = 0 also takes the
+// packed DC / Nyquist bin and the self-paired middle bin M/2), saving an
+// LDS pass.
+template <int M>
+__device__ __forceinline__ void load_twist_rows(float2* lds, const RowArgs& a, const float2* __restrict__ spec,
+                                                int y0) {
+    using P = RowPlan<M>;
+    constexpr int half = M / 2;
+    const int row = P::ROWS == 1 ? 0 : threadIdx.x / P::TPR, t = P::ROWS == 1 ? threadIdx.x : threadIdx.x % P::TPR;
+    const int off = row * M, H = a.H, y = y0 + row;
+    auto bin = [&](int k) { return spec[((size_t)(k >> 2) * H + y) * BC + (k & 3)]; };
     auto z_of = [&](float2 x1, float2 x2, float2 cw) {
         float2 s1 = cadd(x1, conj2(x2)), d1 = csub(x1, conj2(x2));
         return cadd(s1, cmul(make_float2(-d1.y, d1.x), cw));
     };
-    float2 x1[half / RT], x2[half / RT];
+    float2 x1[half / P::TPR], x2[half / P::TPR];
 #pragma unroll
-    for (int i = 0; i < half / RT; ++i) {
-        const int kk = t + RT * i;
+    for (int i = 0; i < half / P::TPR; ++i) {
+        const int kk = t + P::TPR * i;
         x1[i] = bin(kk);
-        x2[i] = bin(kk == 0 ? half : RM - kk);
+        x2[i] = bin(kk == 0 ? half : M - kk);
     }
 #pragma unroll
-    for (int i = 0; i < half / RT; ++i) {
-        const int kk = t + RT * i;
+    for (int i = 0; i < half / P::TPR; ++i) {
+        const int kk = t + P::TPR * i;
         if (kk == 0) {
-            lds[pad(0)] = make_float2(x1[i].x + x1[i].y, x1[i].x - x1[i].y);
-            lds[pad(half)] = z_of(x2[i], x2[i], conj2(a.twW[half]));
+            lds[pad(off)] = make_float2(x1[i].x + x1[i].y, x1[i].x - x1[i].y);
+            lds[pad(off + half)] = z_of(x2[i], x2[i], conj2(a.twW[half]));
             continue;
         }
         const float2 w1 = a.twW[kk];
-        lds[pad(kk)] = z_of(x1[i], x2[i], conj2(w1));
-        lds[pad(RM - kk)] = z_of(x2[i], x1[i], make_float2(-w1.x, -w1.y));
+        lds[pad(off + kk)] = z_of(x1[i], x2[i], conj2(w1));
+        lds[pad(off + M - kk)] = z_of(x2[i], x1[i], make_float2(-w1.x, -w1.y));
     }
     lds_sync();
 }
 
-__global__ void __launch_bounds__(RT) k_row_fused(RowArgs a) {
+template <int M, bool DIAG>
+__global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
+    using P = RowPlan<M>;
     extern __shared__ float2 lds[];
-    const int W = a.W, H = a.H;
-    const int y0 = xcd_remap(blockIdx.x, gridDim.x);
-    const int t = threadIdx.x;
-    const float2* srow = reinterpret_cast<const float2*>(a.state_in + (size_t)y0 * W);
-    float2* orow = reinterpret_cast<float2*>(a.state_out + (size_t)y0 * W);
-    float2 sv[8];  // state cells (2p, 2p+1) of positions p = t + 256 m
-    load_twist_row(lds, a, y0);
-    // Inverse stages of span 1, 8, 64 through LDS.
-    stage<8, 8, 1>(lds, RM, 1, 1, a.twW, W);
-    stage<8, 8, 1>(lds, RM, 1, 8, a.twW, W);
-    stage<8, 8, 1>(lds, RM, 1, 64, a.twW, W);
+    constexpr int W = 2 * M;
+    const int H = a.H;
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * P::ROWS;
+    const int ch = blockIdx.y;  // destination channel (world in diagonal mode)
+    const int row = P::ROWS == 1 ? 0 : threadIdx.x / P::TPR, t = P::ROWS == 1 ? threadIdx.x : threadIdx.x % P::TPR;
+    const int off = row * M;
+    const size_t plane = (size_t)W * H * ch, cell0 = plane + (size_t)(y0 + row) * W;
+    const float2* srow = reinterpret_cast<const float2*>(a.state_in + cell0);
+    float2* orow = reinterpret_cast<float2*>(a.state_out + cell0);
+    float2 sv[8];  // state cells (2p, 2p+1) of positions p = t + TPR m
+    load_twist_rows<M>(lds, a, a.spec + (size_t)ch * M * H, y0);
+    // Inverse stages through LDS (spans 1, 8, ...).
+    int Ls = 1;
+#pragma unroll
+    for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
     // The state is loaded here rather than at the start: in flight during the
     // last stage, without holding 16 registers through the whole transform.
 #pragma unroll
-    for (int m = 0; m < 8; ++m) sv[m] = srow[t + RT * m];
-    // Last inverse stage (radix 4, span 512): butterflies b = t, t + 256;
-    // output r of butterfly i lands on position t + 256 i + 512 r = t + 256 m
-    // with m = i + 2 r.
+    for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
+    // Last inverse stage (radix RL, span L): butterflies b = t + TPR i,
+    // output r of butterfly i lands on position b + L r = t + TPR m with
+    // m = i + NBL r.
     float2 v[8];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        float2 q[4];
-        const int b = t + RT * i;
+    for (int i = 0; i < P::NBL; ++i) {
+        float2 q[P::RL];
+        const int b = t + P::TPR * i;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) q[r] = lds[pad(b + r * (RM / 4))];
-        float2 w[4];
-        twiddles<4, 1>(w, a.twW, b, (2 * RM) / (512 * 4));
+        for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
+        float2 w[P::RL];
+        twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
 #pragma unroll
-        for (int r = 1; r < 4; ++r) q[r] = cmul(q[r], w[r]);
-        dft<4, 1>(q);
+        for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
+        dft<P::RL, 1>(q);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[i + 2 * r] = q[r];
+        for (int r = 0; r < P::RL; ++r) v[i + P::NBL * r] = q[r];
     }
     // Growth, dt, clip on cells (2p, 2p+1); new state to memory.
-    const KernelParams kp = a.kp[0];
+    const KernelParams kp = DIAG ? a.kp_dev[ch] : a.kp[0];
+    const float dt = DIAG ? a.dt_dev[ch] : a.dt;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         float u[2] = {v[m].x, v[m].y}, s2[2] = {sv[m].x, sv[m].y};
@@ -817,22 +839,24 @@ __global__ void __launch_bounds__(RT) k_row_fused(RowArgs a) {
             float d = (u[j] - kp.mu) * kp.inv_sigma;
             float acc = 0.0f;
             acc += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
-            s2[j] = fminf(fmaxf(s2[j] + a.dt * acc, 0.0f), 1.0f);
+            s2[j] = fminf(fmaxf(s2[j] + dt * acc, 0.0f), 1.0f);
         }
         v[m] = make_float2(s2[0], s2[1]);
-        orow[t + RT * m] = v[m];
+        orow[t + P::TPR * m] = v[m];
     }
     lds_sync();  // every thread has read its last-stage inputs
     // First forward stage (radix 8, span 1) on the registers: v[m] is
-    // position t + 256 m, its input r = m.
+    // position t + TPR m, its input r = m.
     dft8<-1>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) lds[pad(8 * t + r)] = v[r];
+    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = v[r];
     lds_sync();
-    stage<8, 8, -1>(lds, RM, 1, 8, a.twW, W);
-    stage<8, 8, -1>(lds, RM, 1, 64, a.twW, W);
-    stage<4, 8, -1>(lds, RM, 1, 512, a.twW, W);
-    store_half_spectrum(lds, RM, 1, H, y0, a.twW, a.spec_next, RT);
+    Ls = 8;
+#pragma unroll
+    for (int s = 1; s < P::N8; ++s, Ls *= 8) stage<8, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
+    if constexpr (P::RLAST == 4) stage<4, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
+    if constexpr (P::RLAST == 2) stage<2, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
+    store_half_spectrum(lds, M, P::ROWS, H, y0, a.twW, a.spec_next + (size_t)ch * M * H, 256);
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel.  Done once per kernel
@@ -970,9 +994,23 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     else if (!a.spec_next) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_update");
     else if (rule == AH_RULE_SMOOTHLIFE) go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
     else if (rule == AH_RULE_MNCA) go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
-    else if (!kp_dev && W == 2 * RM && channels == 1 && n_kernels == 1 && kp[0].dst == 0) {
+    else if (W >= 128 && W <= 4096 && a.rows * W == 4096 &&
+             (kp_dev || (channels == 1 && n_kernels == 1 && kp[0].dst == 0))) {
         Launch L(p, s, "row_c2r_update_r2c");
-        hipLaunchKernelGGL(k_row_fused, grid, dim3(RT), lds, s, a);
+        auto fused = [&](auto diag_tag) {
+            constexpr bool D = decltype(diag_tag)::value;
+            auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), lds, s, a); };
+            switch (W) {
+                case 128: go_fused(k_row_fused<64, D>); break;
+                case 256: go_fused(k_row_fused<128, D>); break;
+                case 512: go_fused(k_row_fused<256, D>); break;
+                case 1024: go_fused(k_row_fused<512, D>); break;
+                case 2048: go_fused(k_row_fused<1024, D>); break;
+                default: go_fused(k_row_fused<2048, D>); break;
+            }
+        };
+        if (kp_dev) fused(std::true_type{});
+        else fused(std::false_type{});
     } else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");
     AH_HIP(hipGetLastError());
 }

@@ -39,7 +39,7 @@ def test_reference_known_answer(O):
 
 
 @pytest.mark.parametrize("w,h", [(16, 16), (32, 64), (64, 32), (128, 128), (256, 64), (512, 1024), (2048, 256),
-                                 (64, 4096), (2048, 4096)])
+                                 (64, 4096), (2048, 4096), (8192, 64), (8192, 16)])
 def test_conv_matches_fp64(O, w, h):
     """The convolution alone (the reference's conv0 field) vs FP64, random state."""
     radius = min(w, h) / 2 - 1.5
@@ -212,13 +212,18 @@ def test_profile_counts_sampled_launches():
     assert sim._lib.ah_sim_profile(sim._h, -1) == ERR_INVALID
 
 
-@pytest.mark.parametrize("h", [64, 256])
-def test_wide_world_steps_and_resume(O, h):
-    """4096-wide worlds run the streamed row pass: every step vs FP64 from the
-    same input, and a run resumed from a saved state (whose first step
-    transforms the state in pass 1) is bitwise equal to the uninterrupted run
-    (whose spectra come from the previous step's pass 3)."""
-    d = W.desc(4096, h, [dict(W.RINGS3)], 0.1, init=W.INIT_BLOBS, init_size=48, init_count=16)
+@pytest.mark.parametrize("w,h", [(4096, 64), (4096, 256), (8192, 128), (2048, 64), (1024, 256), (512, 128),
+                                 (256, 512), (128, 128), (128, 16)])
+def test_wide_world_steps_and_resume(O, w, h):
+    """Worlds 128..4096 wide run the fused row pass (2048/M rows per
+    workgroup), 8192-wide ones (the widest a descriptor allows) and worlds
+    with fewer rows than a fused workgroup takes (128x16) the generic one:
+    every step vs FP64 from the same input, and a run resumed from a saved
+    state (whose first step transforms the state in pass 1) is bitwise equal
+    to the uninterrupted run (whose spectra come from the previous step's
+    pass 3)."""
+    k = dict(W.RINGS3, radius=float(min(26, min(w, h) // 2 - 2)))
+    d = W.desc(w, h, [k], 0.1, init=W.INIT_BLOBS, init_size=min(48, h // 2), init_count=16)
     a = Sim(d, 5)
     khats = O.kernel_hats64(d)
     s = a.read_field("state")
@@ -228,7 +233,7 @@ def test_wide_world_steps_and_resume(O, h):
         got = a.read_field("state")
         worst = max(worst, np.abs(got - O.step64(d, s, "fft", khats)).max())
         s = got
-    print(f"4096x{h} worst per-step max-abs {worst:.3e}")
+    print(f"{w}x{h} worst per-step max-abs {worst:.3e}")
     assert worst < 1e-5
     b = Sim(d, 77)
     b.write_field("state", s)
