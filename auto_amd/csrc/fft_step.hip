@@ -554,6 +554,104 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
     }
 }
 
+// Fused column pass for H = 64..2048, one kernel per source channel (one
+// world, or batched same-size worlds in diagonal mode), for the blocks
+// kb >= 1 (the packed kx = 0 column of block 0 pairs rows p and H - p held by
+// different threads and goes through k_col_conv).  A workgroup takes CPW
+// columns of one block (all four for H <= 1024) with TPC = H/8 threads per
+// column.  The forward transform's last stage (radix RL, span L = H/RL)
+// leaves thread t of a column holding rows t + TPC m, m = 0..7, exactly the
+// rows the inverse transform's first stage (radix 8, span 1) reads: the
+// K-hat multiply and that stage run in registers, saving two LDS round trips.
+template <int H>
+struct ColPlan {
+    static constexpr int TPC = H / 8, CPW = H <= 1024 ? 4 : 2, THREADS = TPC * CPW;
+    static constexpr int N8 = H >= 512 ? 3 : 2;                   // radix-8 stages with 8^N8 <= H
+    static constexpr int RLAST = H / (N8 == 3 ? 512 : 64);        // 1, 2 or 4
+    static constexpr int RL = RLAST == 1 ? 8 : RLAST, L = H / RL, NBL = 8 / RL;
+    static constexpr int NLDS = RLAST == 1 ? N8 - 1 : N8;         // forward stages through LDS
+    static_assert(H >= 64 && H <= 2048 && (H & (H - 1)) == 0, "column length");
+};
+
+template <int H>
+__global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
+    using P = ColPlan<H>;
+    extern __shared__ float2 lds[];
+    constexpr int GPB = 4 / P::CPW;  // workgroups per block
+    const int L0 = xcd_remap(blockIdx.x, gridDim.x);
+    const int kb = 1 + L0 / GPB, c0 = (L0 % GPB) * P::CPW;
+    const int ch = blockIdx.y;
+    const int k = a.diag ? ch : 0;  // the one kernel reading this channel
+    const size_t chan = (size_t)(a.W >> 1) * H;
+    const float2* src = a.spec_in + ch * chan;
+    float2* dst = a.spec_out + k * chan;
+    // Load: thread t takes (row, column) = (t / CPW + THREADS / CPW * i, t % CPW);
+    // the CPW columns of a row are adjacent in memory.
+    {
+        float2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + P::THREADS * i;
+            v[i] = src[col_index(kb, c0 + e % P::CPW, e / P::CPW, H)];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + P::THREADS * i;
+            lds[pad((e % P::CPW) * H + e / P::CPW)] = v[i];
+        }
+    }
+    lds_sync();
+    int Ls = 1;
+#pragma unroll
+    for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, -1>(lds, H, P::CPW, Ls, a.twH, H);
+    const int col = threadIdx.x / P::TPC, t = threadIdx.x % P::TPC, off = col * H;
+    // K-hat factors of this thread's rows y = t + TPC m: real and even in ky,
+    // khat[p = min(y, H - y)] of column (kb, c0 + col).
+    constexpr int NP = H / 2 + 1;
+    const float* kh = a.khat + ((size_t)k * (a.W >> 3) + kb) * (size_t)NP * BC + c0 + col;
+    float kf[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int y = t + P::TPC * m;
+        kf[m] = kh[(size_t)min(y, H - y) * BC];
+    }
+    // Last forward stage (radix RL, span L) into registers: output r of
+    // butterfly i = t + TPC i lands on row t + TPC (i + NBL r).
+    float2 v[8];
+#pragma unroll
+    for (int i = 0; i < P::NBL; ++i) {
+        float2 q[P::RL];
+        const int b = t + P::TPC * i;
+#pragma unroll
+        for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
+        float2 w[P::RL];
+        twiddles<P::RL, -1>(w, a.twH, b, 1);  // table stride H / (L * RL)
+#pragma unroll
+        for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
+        dft<P::RL, -1>(q);
+#pragma unroll
+        for (int r = 0; r < P::RL; ++r) v[i + P::NBL * r] = q[r];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = make_float2(kf[m] * v[m].x, kf[m] * v[m].y);
+    lds_sync();  // every thread has read its last-stage inputs
+    // First inverse stage (radix 8, span 1) on the registers.
+    dft8<1>(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = v[r];
+    lds_sync();
+    Ls = 8;
+#pragma unroll
+    for (int s = 1; s < P::N8; ++s, Ls *= 8) stage<8, 8, 1>(lds, H, P::CPW, Ls, a.twH, H);
+    if constexpr (P::RLAST == 4) stage<4, 8, 1>(lds, H, P::CPW, Ls, a.twH, H);
+    if constexpr (P::RLAST == 2) stage<2, 8, 1>(lds, H, P::CPW, Ls, a.twH, H);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + P::THREADS * i;
+        dst[col_index(kb, c0 + e % P::CPW, e / P::CPW, H)] = lds[pad((e % P::CPW) * H + e / P::CPW)];
+    }
+}
+
 // ------------------------------------------------------------ pass 3
 struct RowArgs {
     int W, H, rows;
@@ -954,6 +1052,28 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
+    const bool one_kernel = diag || (channels == 1 && n_kernels == 1 && ksrc[0] == 0);
+    if (one_kernel && H >= 64 && H <= 2048 && W >= 16) {
+        // Block 0 (with the packed column) on the generic kernel, blocks
+        // 1.. on the fused one.
+        hipLaunchKernelGGL(k_col_conv<8>, dim3(4 / a.cw, channels), dim3(elems / 8), lds, s, a);
+        AH_HIP(hipGetLastError());
+        auto fused = [&](auto kernel, int threads, int cpw) {
+            const size_t flds = padded((size_t)cpw * H) * sizeof(float2);
+            set_lds(kernel, flds);
+            hipLaunchKernelGGL(kernel, dim3((W / 8 - 1) * (4 / cpw), channels), dim3(threads), flds, s, a);
+        };
+        switch (H) {
+            case 64: fused(k_col_fused<64>, ColPlan<64>::THREADS, ColPlan<64>::CPW); break;
+            case 128: fused(k_col_fused<128>, ColPlan<128>::THREADS, ColPlan<128>::CPW); break;
+            case 256: fused(k_col_fused<256>, ColPlan<256>::THREADS, ColPlan<256>::CPW); break;
+            case 512: fused(k_col_fused<512>, ColPlan<512>::THREADS, ColPlan<512>::CPW); break;
+            case 1024: fused(k_col_fused<1024>, ColPlan<1024>::THREADS, ColPlan<1024>::CPW); break;
+            default: fused(k_col_fused<2048>, ColPlan<2048>::THREADS, ColPlan<2048>::CPW); break;
+        }
+        AH_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / 8), lds, s, a);
     AH_HIP(hipGetLastError());
 }
