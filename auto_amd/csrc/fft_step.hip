@@ -554,12 +554,14 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
     }
 }
 
-// Fused column pass for H = 64..2048, one kernel per source channel (one
-// world, or batched same-size worlds in diagonal mode), for the blocks
-// kb >= 1 (the packed kx = 0 column of block 0 pairs rows p and H - p held by
+// Fused column pass for batched same-size worlds (diagonal mode: world c's
+// kernel reads and writes channel c), H = 64..2048, for the blocks kb >= 1
+// (the packed kx = 0 column of block 0 pairs rows p and H - p held by
 // different threads and goes through k_col_conv).  A workgroup takes CPW
 // columns of one block (all four for H <= 1024) with TPC = H/8 threads per
-// column.  The forward transform's last stage (radix RL, span L = H/RL)
+// column.  Its large workgroups pay off only with many of them: measured on
+// single worlds (128-512 workgroups) the generic kernel is faster, on a
+// batch group of ~100 worlds this one is.  The forward transform's last stage (radix RL, span L = H/RL)
 // leaves thread t of a column holding rows t + TPC m, m = 0..7, exactly the
 // rows the inverse transform's first stage (radix 8, span 1) reads: the
 // K-hat multiply and that stage run in registers, saving two LDS round trips.
@@ -578,13 +580,16 @@ __global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
     using P = ColPlan<H>;
     extern __shared__ float2 lds[];
     constexpr int GPB = 4 / P::CPW;  // workgroups per block
+    // The grid covers every block, so its size stays a multiple of 8 and
+    // xcd_remap keeps the GPB workgroups of a block on one XCD (one L2);
+    // block 0's workgroups leave at once.
     const int L0 = xcd_remap(blockIdx.x, gridDim.x);
-    const int kb = 1 + L0 / GPB, c0 = (L0 % GPB) * P::CPW;
-    const int ch = blockIdx.y;
-    const int k = a.diag ? ch : 0;  // the one kernel reading this channel
+    const int kb = L0 / GPB, c0 = (L0 % GPB) * P::CPW;
+    if (kb == 0) return;
+    const int ch = blockIdx.y;  // world ch: channel ch, kernel ch
     const size_t chan = (size_t)(a.W >> 1) * H;
     const float2* src = a.spec_in + ch * chan;
-    float2* dst = a.spec_out + k * chan;
+    float2* dst = a.spec_out + ch * chan;
     // Load: thread t takes (row, column) = (t / CPW + THREADS / CPW * i, t % CPW);
     // the CPW columns of a row are adjacent in memory.
     {
@@ -608,7 +613,7 @@ __global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
     // K-hat factors of this thread's rows y = t + TPC m: real and even in ky,
     // khat[p = min(y, H - y)] of column (kb, c0 + col).
     constexpr int NP = H / 2 + 1;
-    const float* kh = a.khat + ((size_t)k * (a.W >> 3) + kb) * (size_t)NP * BC + c0 + col;
+    const float* kh = a.khat + ((size_t)ch * (a.W >> 3) + kb) * (size_t)NP * BC + c0 + col;
     float kf[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -885,11 +890,15 @@ __device__ __forceinline__ void load_twist_rows(float2* lds, const RowArgs& a, c
     lds_sync();
 }
 
-template <int M, bool DIAG>
+// RULE = AH_RULE_LENIA: one kernel (NK = 1), its growth; SmoothLife and
+// MNCA: two kernels of the one channel (NK = 2), whose potentials the
+// transition combines.  Kernel k's potentials stay in registers while the
+// next kernel's row goes through LDS.
+template <int M, bool DIAG, int RULE>
 __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     using P = RowPlan<M>;
     extern __shared__ float2 lds[];
-    constexpr int W = 2 * M;
+    constexpr int W = 2 * M, NK = RULE == AH_RULE_LENIA ? 1 : 2;
     const int H = a.H;
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * P::ROWS;
     const int ch = blockIdx.y;  // destination channel (world in diagonal mode)
@@ -898,58 +907,71 @@ __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     const size_t plane = (size_t)W * H * ch, cell0 = plane + (size_t)(y0 + row) * W;
     const float2* srow = reinterpret_cast<const float2*>(a.state_in + cell0);
     float2* orow = reinterpret_cast<float2*>(a.state_out + cell0);
-    float2 sv[8];  // state cells (2p, 2p+1) of positions p = t + TPR m
-    load_twist_rows<M>(lds, a, a.spec + (size_t)ch * M * H, y0);
-    // Inverse stages through LDS (spans 1, 8, ...).
-    int Ls = 1;
+    float2 sv[8];      // state cells (2p, 2p+1) of positions p = t + TPR m
+    float2 v[NK][8];   // potentials of those cells, per kernel
 #pragma unroll
-    for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
-    // The state is loaded here rather than at the start: in flight during the
-    // last stage, without holding 16 registers through the whole transform.
+    for (int k = 0; k < NK; ++k) {
+        if (k > 0) lds_sync();  // every thread has read kernel k-1's last-stage inputs
+        load_twist_rows<M>(lds, a, a.spec + (size_t)(NK == 1 ? ch : k) * M * H, y0);
+        // Inverse stages through LDS (spans 1, 8, ...).
+        int Ls = 1;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
-    // Last inverse stage (radix RL, span L): butterflies b = t + TPR i,
-    // output r of butterfly i lands on position b + L r = t + TPR m with
-    // m = i + NBL r.
-    float2 v[8];
+        for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
+        // The state is loaded here rather than at the start: in flight during
+        // the last stage, without holding 16 registers through the transform.
+        if (k == NK - 1) {
 #pragma unroll
-    for (int i = 0; i < P::NBL; ++i) {
-        float2 q[P::RL];
-        const int b = t + P::TPR * i;
+            for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
+        }
+        // Last inverse stage (radix RL, span L): butterflies b = t + TPR i,
+        // output r of butterfly i lands on position b + L r = t + TPR m with
+        // m = i + NBL r.
 #pragma unroll
-        for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
-        float2 w[P::RL];
-        twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
+        for (int i = 0; i < P::NBL; ++i) {
+            float2 q[P::RL];
+            const int b = t + P::TPR * i;
 #pragma unroll
-        for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
-        dft<P::RL, 1>(q);
+            for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
+            float2 w[P::RL];
+            twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
 #pragma unroll
-        for (int r = 0; r < P::RL; ++r) v[i + P::NBL * r] = q[r];
+            for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
+            dft<P::RL, 1>(q);
+#pragma unroll
+            for (int r = 0; r < P::RL; ++r) v[k][i + P::NBL * r] = q[r];
+        }
     }
-    // Growth, dt, clip on cells (2p, 2p+1); new state to memory.
+    // The rule on cells (2p, 2p+1); new state to memory.
     const KernelParams kp = DIAG ? a.kp_dev[ch] : a.kp[0];
     const float dt = DIAG ? a.dt_dev[ch] : a.dt;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        float u[2] = {v[m].x, v[m].y}, s2[2] = {sv[m].x, sv[m].y};
+        float u[2] = {v[0][m].x, v[0][m].y}, s2[2] = {sv[m].x, sv[m].y};
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            float d = (u[j] - kp.mu) * kp.inv_sigma;
-            float acc = 0.0f;
-            acc += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
-            s2[j] = fminf(fmaxf(s2[j] + dt * acc, 0.0f), 1.0f);
+            if constexpr (RULE == AH_RULE_LENIA) {
+                float d = (u[j] - kp.mu) * kp.inv_sigma;
+                float acc = 0.0f;
+                acc += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
+                s2[j] = fminf(fmaxf(s2[j] + dt * acc, 0.0f), 1.0f);
+            } else {
+                const float u1 = j == 0 ? v[NK - 1][m].x : v[NK - 1][m].y;
+                if constexpr (RULE == AH_RULE_SMOOTHLIFE) s2[j] = smoothlife_next(s2[j], u[j], u1, dt);
+                else s2[j] = mnca_next(s2[j], u[j], u1);
+            }
         }
-        v[m] = make_float2(s2[0], s2[1]);
-        orow[t + P::TPR * m] = v[m];
+        v[0][m] = make_float2(s2[0], s2[1]);
+        orow[t + P::TPR * m] = v[0][m];
     }
+    float2 (&vs)[8] = v[0];
     lds_sync();  // every thread has read its last-stage inputs
-    // First forward stage (radix 8, span 1) on the registers: v[m] is
+    // First forward stage (radix 8, span 1) on the registers: vs[m] is
     // position t + TPR m, its input r = m.
-    dft8<-1>(v);
+    dft8<-1>(vs);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = v[r];
+    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = vs[r];
     lds_sync();
-    Ls = 8;
+    int Ls = 8;
 #pragma unroll
     for (int s = 1; s < P::N8; ++s, Ls *= 8) stage<8, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
     if constexpr (P::RLAST == 4) stage<4, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
@@ -1052,8 +1074,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
-    const bool one_kernel = diag || (channels == 1 && n_kernels == 1 && ksrc[0] == 0);
-    if (one_kernel && H >= 64 && H <= 2048 && W >= 16) {
+    if (diag && H >= 64 && H <= 2048 && W >= 16) {
         // Block 0 (with the packed column) on the generic kernel, blocks
         // 1.. on the fused one.
         hipLaunchKernelGGL(k_col_conv<8>, dim3(4 / a.cw, channels), dim3(elems / 8), lds, s, a);
@@ -1061,7 +1082,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
         auto fused = [&](auto kernel, int threads, int cpw) {
             const size_t flds = padded((size_t)cpw * H) * sizeof(float2);
             set_lds(kernel, flds);
-            hipLaunchKernelGGL(kernel, dim3((W / 8 - 1) * (4 / cpw), channels), dim3(threads), flds, s, a);
+            hipLaunchKernelGGL(kernel, dim3((W / 8) * (4 / cpw), channels), dim3(threads), flds, s, a);
         };
         switch (H) {
             case 64: fused(k_col_fused<64>, ColPlan<64>::THREADS, ColPlan<64>::CPW); break;
@@ -1110,28 +1131,36 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
         set_lds(kernel, lds);
         hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
     };
+    const bool fusable = W >= 128 && W <= 4096 && a.rows * W == 4096 && !conv_out && a.spec_next;
+    auto fused = [&](auto diag_tag, auto rule_tag) {
+        constexpr bool D = decltype(diag_tag)::value;
+        constexpr int R = decltype(rule_tag)::value;
+        Launch L(p, s, "row_c2r_update_r2c");
+        auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), lds, s, a); };
+        switch (W) {
+            case 128: go_fused(k_row_fused<64, D, R>); break;
+            case 256: go_fused(k_row_fused<128, D, R>); break;
+            case 512: go_fused(k_row_fused<256, D, R>); break;
+            case 1024: go_fused(k_row_fused<512, D, R>); break;
+            case 2048: go_fused(k_row_fused<1024, D, R>); break;
+            default: go_fused(k_row_fused<2048, D, R>); break;
+        }
+    };
+    using Lenia = std::integral_constant<int, AH_RULE_LENIA>;
+    using Smooth = std::integral_constant<int, AH_RULE_SMOOTHLIFE>;
+    using Mnca = std::integral_constant<int, AH_RULE_MNCA>;
+    const bool two_kernels = !kp_dev && channels == 1 && n_kernels == 2;
     if (conv_out) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_conv");
     else if (!a.spec_next) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_update");
-    else if (rule == AH_RULE_SMOOTHLIFE) go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
-    else if (rule == AH_RULE_MNCA) go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
-    else if (W >= 128 && W <= 4096 && a.rows * W == 4096 &&
-             (kp_dev || (channels == 1 && n_kernels == 1 && kp[0].dst == 0))) {
-        Launch L(p, s, "row_c2r_update_r2c");
-        auto fused = [&](auto diag_tag) {
-            constexpr bool D = decltype(diag_tag)::value;
-            auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), lds, s, a); };
-            switch (W) {
-                case 128: go_fused(k_row_fused<64, D>); break;
-                case 256: go_fused(k_row_fused<128, D>); break;
-                case 512: go_fused(k_row_fused<256, D>); break;
-                case 1024: go_fused(k_row_fused<512, D>); break;
-                case 2048: go_fused(k_row_fused<1024, D>); break;
-                default: go_fused(k_row_fused<2048, D>); break;
-            }
-        };
-        if (kp_dev) fused(std::true_type{});
-        else fused(std::false_type{});
-    } else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");
+    else if (rule == AH_RULE_SMOOTHLIFE) {
+        if (fusable && two_kernels) fused(std::false_type{}, Smooth{});
+        else go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
+    } else if (rule == AH_RULE_MNCA) {
+        if (fusable && two_kernels) fused(std::false_type{}, Mnca{});
+        else go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
+    } else if (fusable && kp_dev) fused(std::true_type{}, Lenia{});
+    else if (fusable && channels == 1 && n_kernels == 1 && kp[0].dst == 0) fused(std::false_type{}, Lenia{});
+    else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");
     AH_HIP(hipGetLastError());
 }
 

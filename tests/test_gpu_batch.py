@@ -89,10 +89,13 @@ def _world(side, radius, sigma, dt, path, rings=(1.0,)):
 
 def test_batch_fft_and_auto_worlds(O):
     """FFT worlds (grouped by size, one world per channel) and AUTO worlds in
-    one batch with direct worlds.  An FFT world equals a standalone Sim of the
-    same world bitwise (the same kernels); every world is within 1e-3 of the
-    FP64 step from the same input; AUTO takes FFT for a wide, high-gain kernel
-    (3 bf16 terms, ~100 chunks) and direct for a small, low-gain one."""
+    one batch with direct worlds.  An FFT world matches a standalone Sim of
+    the same world to FP32 rounding (the batch's column pass is its own fused
+    kernel, so the summation order differs): within 1e-5, against ~1e-6 of
+    either vs FP64.  A direct world equals a standalone direct Sim bitwise.
+    Every world is within 1e-3 of the FP64 step from the same input; AUTO
+    takes FFT for a wide, high-gain kernel (3 bf16 terms, ~100 chunks) and
+    direct for a small, low-gain one."""
     worlds = W.uneven_worlds(seed=4, n=5, path="fft_fp32", sides=(128, 256))
     worlds += W.uneven_worlds(seed=5, n=5, path="direct_bf16", sides=(128, 256))
     worlds += [(_world(256, 24, 0.01, 0.2, "auto", (1.0, 0.6, 0.3)), 11),   # -> FFT
@@ -104,7 +107,7 @@ def test_batch_fft_and_auto_worlds(O):
     b.step(2)
     before = [b.read_world(i) for i in range(len(worlds))]
     b.step(1)
-    worst64 = 0.0
+    worst64 = worst_sim = 0.0
     for i, (d, seed) in enumerate(worlds):
         got = b.read_world(i)
         worst64 = max(worst64, float(np.abs(got - O.step64(d, before[i][None], "fft")[0]).max()))
@@ -114,7 +117,7 @@ def test_batch_fft_and_auto_worlds(O):
             sim = Sim(plain, seed)
             sim.write_field("state", before[i][None])
             sim.step(1)
-            assert np.array_equal(got, sim.read_field("state")[0]), i
+            worst_sim = max(worst_sim, float(np.abs(got - sim.read_field("state")[0]).max()))
         else:
             # A direct world equals a standalone direct Sim: the same MFMA
             # order per cell and the same epilogue arithmetic.
@@ -124,8 +127,9 @@ def test_batch_fft_and_auto_worlds(O):
             sim.write_field("state", before[i][None])
             sim.step(1)
             assert np.array_equal(got, sim.read_field("state")[0]), i
-    print(f"mixed batch: {info.fft_worlds} FFT worlds in {info.fft_groups} groups; worst max-abs vs FP64 {worst64:.3e}")
-    assert worst64 <= 1e-3
+    print(f"mixed batch: {info.fft_worlds} FFT worlds in {info.fft_groups} groups; worst max-abs vs FP64 {worst64:.3e}, "
+          f"FFT worlds vs standalone Sims {worst_sim:.3e}")
+    assert worst64 <= 1e-3 and worst_sim < 1e-5
     # A write into an FFT world replaces the spectra its next step uses.
     x = np.random.default_rng(1).random((worlds[0][0].height, worlds[0][0].width), dtype=np.float32)
     b.write_world(0, x)
@@ -134,7 +138,7 @@ def test_batch_fft_and_auto_worlds(O):
     sim = Sim(plain, 1)
     sim.write_field("state", x[None])
     sim.step(1)
-    assert np.array_equal(b.read_world(0), sim.read_field("state")[0])
+    assert np.abs(b.read_world(0) - sim.read_field("state")[0]).max() < 1e-5
 
 
 def test_batch_of_fft_worlds_only_and_graph():
