@@ -448,10 +448,10 @@ __device__ __forceinline__ void column_stage(float2* buf, int s, const float2 (&
 // stage, y = t + 512 r: the kernel spectrum is real and even in ky, so row y
 // takes khat[p = min(y, H - y)], stored column-major for this kernel so a
 // wave's loads are contiguous.  Not used for the packed kx=0 column.
-__device__ __forceinline__ void load_khat(float (&kf)[8], const ColArgs& a, int col) {
+__device__ __forceinline__ void load_khat(float (&kf)[8], const float* __restrict__ khat, int col) {
     const int t = threadIdx.x;
     constexpr int NP = SH / 2 + 1;
-    const float* kh = a.khat + (size_t)col * NP;
+    const float* kh = khat + (size_t)col * NP;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int y = t + r * ST;
@@ -487,12 +487,16 @@ __device__ __forceinline__ void store_column(const float2* buf, float2* __restri
     }
 }
 
-// Forward FFT, multiply, inverse FFT and write-back of one column.  After
-// the last forward stage thread t holds rows t + 512 r, which are exactly
-// the rows the first inverse stage (span 1) reads: the multiply and that
-// stage run in registers, saving one LDS round trip and two barriers.
+// Forward FFT, multiply, inverse FFT and write-back of one column, for each
+// of the NK kernels reading it (1, or 2 for SmoothLife and MNCA: the
+// transformed column stays in registers between them).  After the last
+// forward stage thread t holds rows t + 512 r, which are exactly the rows the
+// first inverse stage (span 1) reads: the multiply and that stage run in
+// registers, saving one LDS round trip and two barriers.  kf[j] holds kernel
+// j's factors for these rows.
+template <int NK>
 __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const float2 (&tw)[3],
-                                            const float (&kf)[8], int col) {
+                                            const float (&kf)[NK][8], int col) {
     column_stage<-1>(buf, 0, tw);
     column_stage<-1>(buf, 1, tw);
     column_stage<-1>(buf, 2, tw);
@@ -501,29 +505,37 @@ __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const
     lds_sync();
     twiddle_column<-1>(v, tw[2]);
     dft8<-1>(v);
-    if (col != 0) {
+    const size_t chan = (size_t)(a.W >> 1) * SH;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = make_float2(kf[r] * v[r].x, kf[r] * v[r].y);
-        dft8<1>(v);
-        column_store(buf, v, 0);
-        lds_sync();
-    } else {
-        column_store(buf, v, 3);
-        lds_sync();
-        multiply_packed_column(buf, a.kpack);
-        column_stage<1>(buf, 0, tw);
+    for (int j = 0; j < NK; ++j) {
+        if (j > 0) lds_sync();  // kernel j-1's write-back has read the buffer
+        if (col != 0) {
+            float2 u[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) u[r] = make_float2(kf[j][r] * v[r].x, kf[j][r] * v[r].y);
+            dft8<1>(u);
+            column_store(buf, u, 0);
+            lds_sync();
+        } else {
+            column_store(buf, v, 3);
+            lds_sync();
+            multiply_packed_column(buf, a.kpack + (size_t)j * (SH / 2 + 1));
+            column_stage<1>(buf, 0, tw);
+        }
+        column_stage<1>(buf, 1, tw);
+        column_stage<1>(buf, 2, tw);
+        column_stage<1>(buf, 3, tw);
+        store_column(buf, a.spec_out + j * chan, col);
     }
-    column_stage<1>(buf, 1, tw);
-    column_stage<1>(buf, 2, tw);
-    column_stage<1>(buf, 3, tw);
-    store_column(buf, a.spec_out, col);
 }
 
 // NC columns per workgroup, G = W/2/NC workgroups: column i of workgroup L
 // is L + i*G.  Fully unrolled, so each step names its buffers statically and
 // the compiler's wait tracking can tell the prefetch target from the buffer
-// being transformed.
-template <int NC>
+// being transformed.  One kernel: K-hat for column i is loaded with column
+// i's prefetch, during column i-1; two kernels: at the start of column i
+// (registers are too few to hold both a column ahead).
+template <int NC, int NK>
 __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4))) k_col_stream(ColArgs a) {
     __shared__ float2 bufA[padded(SH)];
     __shared__ float2 bufB[padded(SH)];
@@ -535,22 +547,25 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
         const int Ls = 1 << (3 * s);
         tw[s - 1] = a.twH[(threadIdx.x & (Ls - 1)) * (SH / (8 * Ls))];
     }
-    // K-hat for column i is loaded with column i's prefetch, during column
-    // i-1, so both are complete at the wait that opens column i.
-    float kf[2][8];
-    load_khat(kf[0], a, col0);
+    const size_t kstride = (size_t)(a.W >> 1) * (SH / 2 + 1);  // one kernel's column-major K-hat
+    float kf[2][NK][8];
+    if constexpr (NK == 1) load_khat(kf[0][0], a.khat, col0);
     prefetch_column(bufA, a.spec_in, col0);
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         float2* cur = (i & 1) ? bufB : bufA;
         float2* nxt = (i & 1) ? bufA : bufB;
         const int col = col0 + i * G;
+        if constexpr (NK > 1) {
+#pragma unroll
+            for (int j = 0; j < NK; ++j) load_khat(kf[i & 1][j], a.khat + j * kstride, col);
+        }
         prefetch_sync();
         if (i + 1 < NC) {
-            load_khat(kf[(i + 1) & 1], a, col + G);
+            if constexpr (NK == 1) load_khat(kf[(i + 1) & 1][0], a.khat, col + G);
             prefetch_column(nxt, a.spec_in, col + G);
         }
-        conv_column(cur, a, tw, kf[i & 1], col);
+        conv_column<NK>(cur, a, tw, kf[i & 1], col);
     }
 }
 
@@ -1003,7 +1018,8 @@ int row_rows(int W, int H) {
 }  // namespace
 
 bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
-    return channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == SH && W >= 64;
+    return channels == 1 && (n_kernels == 1 || (n_kernels == 2 && ksrc[1] == 0)) && ksrc[0] == 0 && H == SH &&
+           W >= 64;
 }
 
 // Compute units of the current device (cached per device).
@@ -1058,15 +1074,26 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     if (!diag && col_stream_ok(W, H, channels, n_kernels, ksrc)) {
         // Up to four columns per workgroup, aiming at two workgroups per CU
         // (each holds 2 x 33 KiB of LDS).
+        // Two kernels hold twice the K-hat factors and the transformed column
+        // besides: one column per workgroup keeps them in registers.
         const int ncols = W / 2, slots = 2 * cu_count();
         int nc = 1;
-        while (nc < 4 && ncols / (2 * nc) >= slots) nc *= 2;
+        while (n_kernels == 1 && nc < 4 && ncols / (2 * nc) >= slots) nc *= 2;
         const int G = ncols / nc;
         a.cw = 1;
         Launch L(p, s, "col_conv");
-        if (nc == 4) hipLaunchKernelGGL(k_col_stream<4>, dim3(G), dim3(ST), 0, s, a);
-        else if (nc == 2) hipLaunchKernelGGL(k_col_stream<2>, dim3(G), dim3(ST), 0, s, a);
-        else hipLaunchKernelGGL(k_col_stream<1>, dim3(G), dim3(ST), 0, s, a);
+        auto go_stream = [&](auto nk_tag) {
+            constexpr int NK = decltype(nk_tag)::value;
+            if constexpr (NK == 1) {
+                if (nc == 4) hipLaunchKernelGGL((k_col_stream<4, 1>), dim3(G), dim3(ST), 0, s, a);
+                else if (nc == 2) hipLaunchKernelGGL((k_col_stream<2, 1>), dim3(G), dim3(ST), 0, s, a);
+                else hipLaunchKernelGGL((k_col_stream<1, 1>), dim3(G), dim3(ST), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_col_stream<1, NK>), dim3(G), dim3(ST), 0, s, a);
+            }
+        };
+        if (n_kernels == 2) go_stream(std::integral_constant<int, 2>{});
+        else go_stream(std::integral_constant<int, 1>{});
         AH_HIP(hipGetLastError());
         return;
     }

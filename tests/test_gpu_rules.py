@@ -1,4 +1,6 @@
-"""SmoothLife and MNCA on the FFT path (GPU), against the oracle.
+"""SmoothLife and MNCA on the FFT path (GPU), against the oracle.  Height
+4096 takes the streamed column pass with both kernels, other heights the
+generic one; widths 128..4096 take the fused row pass.
 
 SmoothLife's transition is smooth, so every step is compared with the FP64
 oracle step and the reference-FP32 restatement from the same input.  MNCA's
@@ -24,7 +26,7 @@ def test_seed_bit_exact(O, rule, seed):
     assert np.array_equal(Sim(d, seed).read_field("state"), O.init_state(d, seed))
 
 
-@pytest.mark.parametrize("w,h", [(256, 256), (4096, 128)])
+@pytest.mark.parametrize("w,h", [(256, 256), (4096, 128), (64, 4096)])
 def test_smoothlife_steps(O, w, h):
     d = W.smoothlife(w, h)
     sim = Sim(d, 5)
@@ -43,7 +45,8 @@ def test_smoothlife_steps(O, w, h):
     assert 0 < s.mean() < 1
 
 
-@pytest.mark.parametrize("w,h,rings", [(256, 256, None), (4096, 128, None), (256, 256, (10, 15.5, 2, 6))])
+@pytest.mark.parametrize("w,h,rings", [(256, 256, None), (4096, 128, None), (64, 4096, None),
+                                        (256, 256, (10, 15.5, 2, 6))])
 def test_mnca_steps(O, w, h, rings):
     d = W.mnca(w, h) if rings is None else W.mnca(w, h, rings=rings)
     sim = Sim(d, 6)
@@ -70,11 +73,12 @@ def test_mnca_steps(O, w, h, rings):
     assert flips > 0
 
 
-@pytest.mark.parametrize("rule", ["smoothlife", "mnca"])
-def test_rules_resume_bitwise(rule):
+@pytest.mark.parametrize("rule,w,h", [("smoothlife", 512, 512), ("mnca", 512, 512), ("smoothlife", 128, 4096)])
+def test_rules_resume_bitwise(rule, w, h):
     """Resuming from a saved state (first step transforms the state itself)
-    equals the uninterrupted run (spectra carried by the previous step)."""
-    d = W.smoothlife(512, 512) if rule == "smoothlife" else W.mnca(512, 512)
+    equals the uninterrupted run (spectra carried by the previous step).
+    Height 4096 runs the streamed two-kernel column pass."""
+    d = W.smoothlife(w, h) if rule == "smoothlife" else W.mnca(w, h)
     a, b = Sim(d, 2), Sim(d, 2)
     a.step(3)
     b.write_field("state", a.read_field("state"))
