@@ -905,15 +905,46 @@ __device__ __forceinline__ void load_twist_rows(float2* lds, const RowArgs& a, c
     lds_sync();
 }
 
-// RULE = AH_RULE_LENIA: one kernel (NK = 1), its growth; SmoothLife and
-// MNCA: two kernels of the one channel (NK = 2), whose potentials the
-// transition combines.  Kernel k's potentials stay in registers while the
-// next kernel's row goes through LDS.
-template <int M, bool DIAG, int RULE>
+// Kernel k's row through the inverse transform into registers: the
+// twisted load, the stages through LDS (spans 1, 8, ...), and the last stage
+// (radix RL, span L) whose output r of butterfly i = t + TPR i lands on
+// position t + TPR m, m = i + NBL r, in out[m].
+template <int M>
+__device__ __forceinline__ void inverse_row(float2* lds, const RowArgs& a, const float2* __restrict__ spec, int y0,
+                                            int off, int t, float2 (&out)[8]) {
+    using P = RowPlan<M>;
+    constexpr int W = 2 * M;
+    load_twist_rows<M>(lds, a, spec, y0);
+    int Ls = 1;
+#pragma unroll
+    for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
+#pragma unroll
+    for (int i = 0; i < P::NBL; ++i) {
+        float2 q[P::RL];
+        const int b = t + P::TPR * i;
+#pragma unroll
+        for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
+        float2 w[P::RL];
+        twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
+#pragma unroll
+        for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
+        dft<P::RL, 1>(q);
+#pragma unroll
+        for (int r = 0; r < P::RL; ++r) out[i + P::NBL * r] = q[r];
+    }
+}
+
+// The rule, by mode: LENIA with one kernel (a world, or a batched world in
+// diagonal mode, DIAG); LENIA with several kernels per destination channel
+// (MULTI: each kernel ending in this channel adds its weighted growth, in
+// kernel order, as the generic kernel does); SmoothLife and MNCA (two kernels
+// of the one channel, whose potentials the transition combines).  Kernel
+// k's values stay in registers while the next kernel's row goes through LDS.
+template <int M, bool DIAG, int RULE, bool MULTI>
 __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     using P = RowPlan<M>;
     extern __shared__ float2 lds[];
-    constexpr int W = 2 * M, NK = RULE == AH_RULE_LENIA ? 1 : 2;
+    constexpr int W = 2 * M;
     const int H = a.H;
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * P::ROWS;
     const int ch = blockIdx.y;  // destination channel (world in diagonal mode)
@@ -922,69 +953,95 @@ __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     const size_t plane = (size_t)W * H * ch, cell0 = plane + (size_t)(y0 + row) * W;
     const float2* srow = reinterpret_cast<const float2*>(a.state_in + cell0);
     float2* orow = reinterpret_cast<float2*>(a.state_out + cell0);
-    float2 sv[8];      // state cells (2p, 2p+1) of positions p = t + TPR m
-    float2 v[NK][8];   // potentials of those cells, per kernel
+    const size_t kspec = (size_t)M * H;
+    float2 sv[8];  // state cells (2p, 2p+1) of positions p = t + TPR m
+    float2 ns[8];  // the new state of those cells
+    if constexpr (MULTI) {
+        float acc[8][2];
 #pragma unroll
-    for (int k = 0; k < NK; ++k) {
-        if (k > 0) lds_sync();  // every thread has read kernel k-1's last-stage inputs
-        load_twist_rows<M>(lds, a, a.spec + (size_t)(NK == 1 ? ch : k) * M * H, y0);
-        // Inverse stages through LDS (spans 1, 8, ...).
-        int Ls = 1;
+        for (int m = 0; m < 8; ++m) acc[m][0] = acc[m][1] = 0.0f;
+        bool first = true;
+        for (int k = 0; k < a.n_kernels; ++k) {
+            const KernelParams kp = a.kp[k];
+            if ((int)kp.dst != ch) continue;  // uniform over the workgroup
+            if (!first) lds_sync();           // kernel k-1's last-stage inputs are read
+            first = false;
+            float2 u[8];
+            inverse_row<M>(lds, a, a.spec + k * kspec, y0, off, t, u);
 #pragma unroll
-        for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
-        // The state is loaded here rather than at the start: in flight during
-        // the last stage, without holding 16 registers through the transform.
-        if (k == NK - 1) {
+            for (int m = 0; m < 8; ++m) {
+                const float uu[2] = {u[m].x, u[m].y};
 #pragma unroll
-            for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
-        }
-        // Last inverse stage (radix RL, span L): butterflies b = t + TPR i,
-        // output r of butterfly i lands on position b + L r = t + TPR m with
-        // m = i + NBL r.
-#pragma unroll
-        for (int i = 0; i < P::NBL; ++i) {
-            float2 q[P::RL];
-            const int b = t + P::TPR * i;
-#pragma unroll
-            for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
-            float2 w[P::RL];
-            twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
-#pragma unroll
-            for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
-            dft<P::RL, 1>(q);
-#pragma unroll
-            for (int r = 0; r < P::RL; ++r) v[k][i + P::NBL * r] = q[r];
-        }
-    }
-    // The rule on cells (2p, 2p+1); new state to memory.
-    const KernelParams kp = DIAG ? a.kp_dev[ch] : a.kp[0];
-    const float dt = DIAG ? a.dt_dev[ch] : a.dt;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        float u[2] = {v[0][m].x, v[0][m].y}, s2[2] = {sv[m].x, sv[m].y};
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if constexpr (RULE == AH_RULE_LENIA) {
-                float d = (u[j] - kp.mu) * kp.inv_sigma;
-                float acc = 0.0f;
-                acc += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
-                s2[j] = fminf(fmaxf(s2[j] + dt * acc, 0.0f), 1.0f);
-            } else {
-                const float u1 = j == 0 ? v[NK - 1][m].x : v[NK - 1][m].y;
-                if constexpr (RULE == AH_RULE_SMOOTHLIFE) s2[j] = smoothlife_next(s2[j], u[j], u1, dt);
-                else s2[j] = mnca_next(s2[j], u[j], u1);
+                for (int j = 0; j < 2; ++j) {
+                    float d = (uu[j] - kp.mu) * kp.inv_sigma;
+                    acc[m][j] += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
+                }
             }
         }
-        v[0][m] = make_float2(s2[0], s2[1]);
-        orow[t + P::TPR * m] = v[0][m];
-    }
-    float2 (&vs)[8] = v[0];
-    lds_sync();  // every thread has read its last-stage inputs
-    // First forward stage (radix 8, span 1) on the registers: vs[m] is
-    // position t + TPR m, its input r = m.
-    dft8<-1>(vs);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = vs[r];
+        for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            ns[m] = make_float2(fminf(fmaxf(sv[m].x + a.dt * acc[m][0], 0.0f), 1.0f),
+                                fminf(fmaxf(sv[m].y + a.dt * acc[m][1], 0.0f), 1.0f));
+    } else {
+        constexpr int NK = RULE == AH_RULE_LENIA ? 1 : 2;
+        float2 v[NK][8];  // potentials of the thread's cells, per kernel
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            if (k > 0) lds_sync();  // every thread has read kernel k-1's last-stage inputs
+            load_twist_rows<M>(lds, a, a.spec + (NK == 1 ? ch : k) * kspec, y0);
+            int Ls = 1;
+#pragma unroll
+            for (int s = 0; s < P::NLDS; ++s, Ls *= 8) stage<8, 8, 1>(lds, M, P::ROWS, Ls, a.twW, W);
+            if (k == NK - 1) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) sv[m] = srow[t + P::TPR * m];
+            }
+#pragma unroll
+            for (int i = 0; i < P::NBL; ++i) {
+                float2 q[P::RL];
+                const int b = t + P::TPR * i;
+#pragma unroll
+                for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
+                float2 w[P::RL];
+                twiddles<P::RL, 1>(w, a.twW, b, W / M);
+#pragma unroll
+                for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
+                dft<P::RL, 1>(q);
+#pragma unroll
+                for (int r = 0; r < P::RL; ++r) v[k][i + P::NBL * r] = q[r];
+            }
+        }
+        const KernelParams kp = DIAG ? a.kp_dev[ch] : a.kp[0];
+        const float dt = DIAG ? a.dt_dev[ch] : a.dt;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            float u[2] = {v[0][m].x, v[0][m].y}, s2[2] = {sv[m].x, sv[m].y};
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if constexpr (RULE == AH_RULE_LENIA) {
+                    float d = (u[j] - kp.mu) * kp.inv_sigma;
+                    float acc = 0.0f;
+                    acc += kp.weight * (2.0f * __expf(-0.5f * d * d) - 1.0f);
+                    s2[j] = fminf(fmaxf(s2[j] + dt * acc, 0.0f), 1.0f);
+                } else {
+                    const float u1 = j == 0 ? v[NK - 1][m].x : v[NK - 1][m].y;
+                    if constexpr (RULE == AH_RULE_SMOOTHLIFE) s2[j] = smoothlife_next(s2[j], u[j], u1, dt);
+                    else s2[j] = mnca_next(s2[j], u[j], u1);
+                }
+            }
+            ns[m] = make_float2(s2[0], s2[1]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) orow[t + P::TPR * m] = ns[m];
+    lds_sync();  // every thread has read its last-stage inputs
+    // First forward stage (radix 8, span 1) on the registers: ns[m] is
+    // position t + TPR m, its input r = m.
+    dft8<-1>(ns);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) lds[pad(off + 8 * t + r)] = ns[r];
     lds_sync();
     int Ls = 8;
 #pragma unroll
@@ -1159,20 +1216,22 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
         hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
     };
     const bool fusable = W >= 128 && W <= 4096 && a.rows * W == 4096 && !conv_out && a.spec_next;
-    auto fused = [&](auto diag_tag, auto rule_tag) {
+    auto fused = [&](auto diag_tag, auto rule_tag, auto multi_tag) {
         constexpr bool D = decltype(diag_tag)::value;
         constexpr int R = decltype(rule_tag)::value;
+        constexpr bool MU = decltype(multi_tag)::value;
         Launch L(p, s, "row_c2r_update_r2c");
         auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), lds, s, a); };
         switch (W) {
-            case 128: go_fused(k_row_fused<64, D, R>); break;
-            case 256: go_fused(k_row_fused<128, D, R>); break;
-            case 512: go_fused(k_row_fused<256, D, R>); break;
-            case 1024: go_fused(k_row_fused<512, D, R>); break;
-            case 2048: go_fused(k_row_fused<1024, D, R>); break;
-            default: go_fused(k_row_fused<2048, D, R>); break;
+            case 128: go_fused(k_row_fused<64, D, R, MU>); break;
+            case 256: go_fused(k_row_fused<128, D, R, MU>); break;
+            case 512: go_fused(k_row_fused<256, D, R, MU>); break;
+            case 1024: go_fused(k_row_fused<512, D, R, MU>); break;
+            case 2048: go_fused(k_row_fused<1024, D, R, MU>); break;
+            default: go_fused(k_row_fused<2048, D, R, MU>); break;
         }
     };
+    const std::false_type no;
     using Lenia = std::integral_constant<int, AH_RULE_LENIA>;
     using Smooth = std::integral_constant<int, AH_RULE_SMOOTHLIFE>;
     using Mnca = std::integral_constant<int, AH_RULE_MNCA>;
@@ -1180,13 +1239,14 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
     if (conv_out) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_conv");
     else if (!a.spec_next) go(k_row_c2r<8, false, AH_RULE_LENIA>, "row_c2r_update");
     else if (rule == AH_RULE_SMOOTHLIFE) {
-        if (fusable && two_kernels) fused(std::false_type{}, Smooth{});
+        if (fusable && two_kernels) fused(no, Smooth{}, no);
         else go(k_row_c2r<8, true, AH_RULE_SMOOTHLIFE>, "row_c2r_update_r2c");
     } else if (rule == AH_RULE_MNCA) {
-        if (fusable && two_kernels) fused(std::false_type{}, Mnca{});
+        if (fusable && two_kernels) fused(no, Mnca{}, no);
         else go(k_row_c2r<8, true, AH_RULE_MNCA>, "row_c2r_update_r2c");
-    } else if (fusable && kp_dev) fused(std::true_type{}, Lenia{});
-    else if (fusable && channels == 1 && n_kernels == 1 && kp[0].dst == 0) fused(std::false_type{}, Lenia{});
+    } else if (fusable && kp_dev) fused(std::true_type{}, Lenia{}, no);
+    else if (fusable && channels == 1 && n_kernels == 1 && kp[0].dst == 0) fused(no, Lenia{}, no);
+    else if (fusable) fused(no, Lenia{}, std::true_type{});
     else go(k_row_c2r<8, true, AH_RULE_LENIA>, "row_c2r_update_r2c");
     AH_HIP(hipGetLastError());
 }
