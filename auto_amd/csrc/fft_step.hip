@@ -570,9 +570,7 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
 }
 
 // Fused column pass for batched same-size worlds (diagonal mode: world c's
-// kernel reads and writes channel c), H = 64..2048, for the blocks kb >= 1
-// (the packed kx = 0 column of block 0 pairs rows p and H - p held by
-// different threads and goes through k_col_conv).  A workgroup takes CPW
+// kernel reads and writes channel c), H = 64..2048.  A workgroup takes CPW
 // columns of one block (all four for H <= 1024) with TPC = H/8 threads per
 // column.  Its large workgroups pay off only with many of them: measured on
 // single worlds (128-512 workgroups) the generic kernel is faster, on a
@@ -595,12 +593,10 @@ __global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
     using P = ColPlan<H>;
     extern __shared__ float2 lds[];
     constexpr int GPB = 4 / P::CPW;  // workgroups per block
-    // The grid covers every block, so its size stays a multiple of 8 and
-    // xcd_remap keeps the GPB workgroups of a block on one XCD (one L2);
-    // block 0's workgroups leave at once.
+    // The grid size (all blocks) stays a multiple of 8, so xcd_remap keeps
+    // the GPB workgroups of a block on one XCD (one L2).
     const int L0 = xcd_remap(blockIdx.x, gridDim.x);
     const int kb = L0 / GPB, c0 = (L0 % GPB) * P::CPW;
-    if (kb == 0) return;
     const int ch = blockIdx.y;  // world ch: channel ch, kernel ch
     const size_t chan = (size_t)(a.W >> 1) * H;
     const float2* src = a.spec_in + ch * chan;
@@ -652,9 +648,35 @@ __global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
 #pragma unroll
         for (int r = 0; r < P::RL; ++r) v[i + P::NBL * r] = q[r];
     }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = make_float2(kf[m] * v[m].x, kf[m] * v[m].y);
     lds_sync();  // every thread has read its last-stage inputs
+    if (kb == 0 && c0 == 0) {
+        // This workgroup holds the packed kx = 0 column (col 0): D[y] =
+        // A C[y] + B conj C[H - y] with (A, B) = kpack[min(y, H - y)] needs
+        // row H - y, which thread TPC - t holds (at m' = 7 - m; thread 0's
+        // own rows pair among themselves).  One exchange through LDS; the
+        // barriers are workgroup-uniform.
+        const bool packed = col == 0;
+        if (packed) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) lds[pad(t + P::TPC * m)] = v[m];
+        }
+        lds_sync();
+        if (packed) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int y = t + P::TPC * m, y2 = (H - y) & (H - 1);
+                const float2 c2 = lds[pad(y2)], ab = a.kpack[(size_t)ch * NP + min(y, y2)];
+                v[m] = cadd(make_float2(ab.x * v[m].x, ab.x * v[m].y), make_float2(ab.y * c2.x, -ab.y * c2.y));
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) v[m] = make_float2(kf[m] * v[m].x, kf[m] * v[m].y);
+        }
+        lds_sync();  // the partner reads are done
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = make_float2(kf[m] * v[m].x, kf[m] * v[m].y);
+    }
     // First inverse stage (radix 8, span 1) on the registers.
     dft8<1>(v);
 #pragma unroll
@@ -1159,10 +1181,6 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
     if (diag && H >= 64 && H <= 2048 && W >= 16) {
-        // Block 0 (with the packed column) on the generic kernel, blocks
-        // 1.. on the fused one.
-        hipLaunchKernelGGL(k_col_conv<8>, dim3(4 / a.cw, channels), dim3(elems / 8), lds, s, a);
-        AH_HIP(hipGetLastError());
         auto fused = [&](auto kernel, int threads, int cpw) {
             const size_t flds = padded((size_t)cpw * H) * sizeof(float2);
             set_lds(kernel, flds);
