@@ -305,11 +305,24 @@ void setup(ah_batch* b) {
     try {
         double* taps = (double*)tmp(sizeof(double) * box);
         double* sum = (double*)tmp(sizeof(double));
+        // Paths first.  AUTO: FFT when the world's direct cost (32-column
+        // chunks over the kernel rows, times the bf16 terms) exceeds
+        // AH_BATCH_FFT_ABOVE or its radius exceeds 31.  A direct launch has a
+        // fixed cost (~35 us at 4096-tile scale, measured) that a few small
+        // worlds do not repay, while an FFT group absorbs them almost for
+        // free: when no world asks for the direct path itself and the AUTO
+        // worlds that chose it hold under 1/20 of the cells, they join the
+        // FFT groups too.
+        std::vector<std::vector<double>> tapv(n);
+        std::vector<char> fft_path(n, 0);
+        size_t auto_direct_cells = 0, all_cells = 0;
+        bool explicit_direct = false;
         for (size_t w = 0; w < n; ++w) {
             const ah_lenia_desc& d = b->d[w];
             const int R = (int)std::ceil((double)d.kernels[0].radius), side = 2 * R + 1;
             launch_kernel_taps64(b->stream, nullptr, d.kernels[0], R, taps, sum);
-            std::vector<double> t((size_t)side * side);
+            std::vector<double>& t = tapv[w];
+            t.resize((size_t)side * side);
             AH_HIP(hipMemcpyAsync(t.data(), taps, sizeof(double) * t.size(), hipMemcpyDeviceToHost, b->stream));
             AH_HIP(hipStreamSynchronize(b->stream));
             bool use_fft = d.path == AH_PATH_FFT_FP32;
@@ -317,7 +330,6 @@ void setup(ah_batch* b) {
                 if (R > 31) {
                     use_fft = true;
                 } else {
-                    // Direct cost: 32-column chunks over the kernel rows, times the terms.
                     std::vector<int> s_tmp;
                     std::vector<uint16_t> f_tmp;
                     int nf = 0;
@@ -326,8 +338,22 @@ void setup(ah_batch* b) {
                     (void)build_fragments(t, R, terms, s_tmp, f_tmp, nf, nrow);
                     const int chunks = 3 * nrow[0] + 2 * nrow[1] + nrow[2];
                     use_fft = chunks * terms > AH_BATCH_FFT_ABOVE && d.height <= 4096;
+                    if (!use_fft) auto_direct_cells += (size_t)d.width * d.height;
                 }
+            } else if (!use_fft) {
+                explicit_direct = true;
             }
+            fft_path[w] = use_fft;
+            all_cells += (size_t)d.width * d.height;
+        }
+        if (!explicit_direct && auto_direct_cells * 20 < all_cells)
+            for (size_t w = 0; w < n; ++w)
+                if (b->d[w].path == AH_PATH_AUTO && b->d[w].height <= 4096) fft_path[w] = 1;
+        for (size_t w = 0; w < n; ++w) {
+            const ah_lenia_desc& d = b->d[w];
+            const int R = (int)std::ceil((double)d.kernels[0].radius);
+            const std::vector<double>& t = tapv[w];
+            const bool use_fft = fft_path[w];
             if (use_fft) {
                 b->fft[w] = 1;
                 int gi = -1;

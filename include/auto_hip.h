@@ -256,7 +256,10 @@ ah_status ah_sim_set_direct_tile(ah_sim* sim, const ah_direct_tile* tile);
  * their direct cost (32-column MFMA chunks summed over the kernel rows, times
  * the bf16 terms) exceeds AH_BATCH_FFT_ABOVE, or their radius exceeds 31.
  * The threshold is measured: on the 384-world benchmark batch, 30 and 0 (all
- * FFT) run at the same rate, 60 at 0.9x and 240 at 0.25x of it.
+ * FFT) run within 2% of each other, 60 at 0.9x and 240 at 0.25x of it.  When
+ * no world names a direct path itself and the AUTO worlds below the threshold
+ * hold under 1/20 of the batch's cells, they take the FFT path too: their
+ * own direct launch would cost more than they save.
  * ah_batch_step replays hipGraph-captured chunks of `graph_steps` steps and
  * runs the remainder one by one.  At creation an autotuner times every
  * admissible (tile shape, grid) pair of the direct launch on the batch itself
