@@ -152,3 +152,36 @@ def test_batch_of_fft_worlds_only_and_graph():
         b.step(1)
     for i in range(len(worlds)):
         assert np.array_equal(a.read_world(i), b.read_world(i))
+
+
+def _world_wh(w, h, radius, seed_dt=0.1):
+    return W.desc(w, h, [dict(radius=float(radius), beta=[1.0, 0.5], mu=0.2, sigma=0.03)], seed_dt, "fft_fp32",
+                  init=W.INIT_PATCH, init_size=min(w, h) // 4)
+
+
+def test_batch_fft_group_shapes(O):
+    """FFT groups at the edges of the batch kernels' ranges: heights 64 and
+    2048 (the fused column pass with four and two columns per workgroup),
+    height 4096 (the generic column pass in diagonal mode), width 64 (the
+    generic row pass) and widths 128 and 2048 (the fused row pass), two
+    worlds per group.  Every world vs a standalone Sim (within 1e-5) and vs
+    the FP64 step (within 1e-3) from the same input."""
+    shapes = [(64, 64, 8), (2048, 2048, 20), (64, 4096, 12), (2048, 64, 12), (128, 2048, 16)]
+    worlds = [(_world_wh(w, h, r), 10 * i + j) for i, (w, h, r) in enumerate(shapes) for j in range(2)]
+    b = Batch(worlds)
+    info = b.info()
+    assert info.fft_worlds == len(worlds) and info.fft_groups == len(shapes)
+    b.step(2)
+    before = [b.read_world(i) for i in range(len(worlds))]
+    b.step(1)
+    worst64 = worst_sim = 0.0
+    for i, (d, seed) in enumerate(worlds):
+        got = b.read_world(i)
+        worst64 = max(worst64, float(np.abs(got - O.step64(d, before[i][None], "fft")[0]).max()))
+        sim = Sim(d, seed)
+        sim.write_field("state", before[i][None])
+        sim.step(1)
+        worst_sim = max(worst_sim, float(np.abs(got - sim.read_field("state")[0]).max()))
+        assert got.std() > 0, i  # the patch is alive
+    print(f"group shapes: worst max-abs vs FP64 {worst64:.3e}, vs standalone Sims {worst_sim:.3e}")
+    assert worst64 <= 1e-3 and worst_sim < 1e-5
