@@ -106,7 +106,7 @@ class Stats(ctypes.Structure):
 
 
 class DirectTile(ctypes.Structure):
-    _fields_ = [("cols", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("ring", ctypes.c_uint32)]
+    _fields_ = [("cols", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("variant", ctypes.c_uint32)]
 
 
 class KernelTime(ctypes.Structure):

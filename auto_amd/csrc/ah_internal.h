@@ -165,7 +165,9 @@ struct DirectArgs {
     int halo_y, halo_x;  // max R over kernels; max R rounded up to a multiple of 8
     int tr, tc, tstride; // LDS tile rows, columns, row stride (bf16 elements)
     int cols, rows;      // output tile shape of the launch (64 or 128 each)
-    int ring;            // 1: persistent ring kernel (one channel, one kernel, one term, rows = 64)
+    int variant;         // 0: tile kernel; 1: persistent ring kernel (one channel, one kernel,
+                         // one term, rows = 64); 2: a workgroup per (tile, kernel) + sum pass
+    float* gsum;         // variant 2: weighted growth per kernel, [n_kernels][H][W]
     int n_kernels;
     DirectKernel k[AH_MAX_KERNELS];
     const int4* sched;   // per kernel, its row list

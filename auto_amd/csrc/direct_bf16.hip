@@ -333,6 +333,69 @@ __global__ void __launch_bounds__(DTHREADS) k_direct(DirectArgs a) {
     }
 }
 
+// Variant 2: workgroup (tile, kernel kn) writes kernel kn's weighted growth,
+// add_weighted(0, weight, growth(u)), to plane kn of a.gsum; k_direct_sum
+// then adds the planes of each channel's kernels in kernel order, which is
+// the sum k_direct forms in registers, rounding for rounding.  Several
+// kernels per channel then fill the chip with n_kernels times the
+// workgroups, each staging one source.
+template <int S, int NB>
+__global__ void __launch_bounds__(DTHREADS) k_direct_split(DirectArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
+    constexpr int DX = 32 * S, DY = 64 * NB;
+    const int W = a.W, H = a.H;
+    const int x0 = blockIdx.x * DX, y0 = blockIdx.y * DY;
+    const int kn = blockIdx.z;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int par = wave & 1, grp = wave >> 1;
+    const int q = lane >> 4, n = lane & 15;
+    uint16_t* tile = lds_u16;
+    uint4* frag_lds = reinterpret_cast<uint4*>(lds_u16 + (size_t)a.tr * a.tstride);
+    int4* rows_lds = reinterpret_cast<int4*>(frag_lds + (size_t)a.nfrag_max * 64);
+    const DirectKernel kk = a.k[kn];
+    int staged = -1, loaded = -1;
+    f32x4 acc[S][NB];
+    kernel_potential<S, NB>(acc, tile, frag_lds, rows_lds, staged, loaded, a.state_in + (size_t)kk.src * W * H,
+                            kk.src, W, H, x0, y0, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, a.frags, kk.frag0,
+                            kk.nfrag, kk.terms, a.sched + a.sched_base[kn], kk.nrow);
+    float* plane = a.gsum + (size_t)kn * W * H;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int xo = x0 + 16 * (par + 2 * k) + 4 * q;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int yo = y0 + grp * 16 * NB + 16 * b + n;
+            float g[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                g[r] = add_weighted(0.0f, kk.weight, growth(potential(acc[k][b][r], kk.scale), kk.mu, kk.inv_sigma));
+            *reinterpret_cast<float4*>(plane + (size_t)yo * W + xo) = make_float4(g[0], g[1], g[2], g[3]);
+        }
+    }
+}
+
+// Sum pass of variant 2: thread i takes cells 4i..4i+3 of channel
+// blockIdx.y.
+__global__ void __launch_bounds__(256) k_direct_sum(DirectArgs a) {
+    const size_t cells = (size_t)a.W * a.H, i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= cells) return;
+    const int c = blockIdx.y;
+    float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kn = 0; kn < a.n_kernels; ++kn) {
+        if (a.k[kn].dst != c) continue;
+        const float4 v = *reinterpret_cast<const float4*>(a.gsum + (size_t)kn * cells + i);
+        g[0] = add_weighted(g[0], 1.0f, v.x);
+        g[1] = add_weighted(g[1], 1.0f, v.y);
+        g[2] = add_weighted(g[2], 1.0f, v.z);
+        g[3] = add_weighted(g[3], 1.0f, v.w);
+    }
+    const size_t o = (size_t)c * cells + i;
+    const float4 sv = *reinterpret_cast<const float4*>(a.state_in + o);
+    *reinterpret_cast<float4*>(a.state_out + o) =
+        make_float4(lenia_update(sv.x, a.dt, g[0]), lenia_update(sv.y, a.dt, g[1]), lenia_update(sv.z, a.dt, g[2]),
+                    lenia_update(sv.w, a.dt, g[3]));
+}
+
 // Persistent variant for one channel, one kernel and one bf16 term.
 // Workgroup b walks a run of vertically adjacent 64-row tiles of one
 // 32S-column strip (the torus wraps a run past the last tile row).  Two
@@ -487,7 +550,7 @@ void allow_lds(K kernel, size_t bytes) {
 
 // LDS: the bf16 tile (or ring), the fragment table, the row list.
 size_t direct_lds_bytes(const DirectArgs& a) {
-    const int rows = a.ring ? a.tr + 64 : a.tr;
+    const int rows = a.variant == 1 ? a.tr + 64 : a.tr;
     return (size_t)rows * a.tstride * sizeof(uint16_t) + (size_t)a.nfrag_max * 64 * sizeof(uint4) +
            DIRECT_MAX_ROWS * sizeof(int4);
 }
@@ -531,7 +594,7 @@ void launch_ring_shape(hipStream_t s, const DirectArgs& a, size_t lds) {
 void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels) {
     const int cols = a.cols, rows = a.rows;
     Launch L(p, s, a.conv_out ? "direct_bf16_conv" : "direct_bf16_update");
-    if (a.ring && !a.conv_out) {
+    if (a.variant == 1 && !a.conv_out) {
         const size_t lds = direct_lds_bytes(a);
         if (cols == 128) launch_ring_shape<4>(s, a, lds);
         else launch_ring_shape<2>(s, a, lds);
@@ -539,8 +602,23 @@ void launch_direct(hipStream_t s, Profiler* p, const DirectArgs& a, int channels
         return;
     }
     DirectArgs plain = a;  // the potential diagnostic runs the tile kernel of the same shape
-    plain.ring = 0;
+    plain.variant = 0;
     const size_t lds = direct_lds_bytes(plain);
+    if (a.variant == 2 && !a.conv_out) {
+        auto split = [&](auto kernel, int S, int NB) {
+            allow_lds(kernel, lds);
+            hipLaunchKernelGGL(kernel, dim3(a.W / (32 * S), a.H / (64 * NB), a.n_kernels), dim3(DTHREADS), lds, s, a);
+        };
+        if (cols == 128 && rows == 128) split(k_direct_split<4, 2>, 4, 2);
+        else if (cols == 128) split(k_direct_split<4, 1>, 4, 1);
+        else if (rows == 128) split(k_direct_split<2, 2>, 2, 2);
+        else split(k_direct_split<2, 1>, 2, 1);
+        AH_HIP(hipGetLastError());
+        const size_t quads = (size_t)a.W * a.H / 4;
+        hipLaunchKernelGGL(k_direct_sum, dim3((unsigned)((quads + 255) / 256), channels), dim3(256), 0, s, a);
+        AH_HIP(hipGetLastError());
+        return;
+    }
     if (cols == 128 && rows == 128) launch_direct_shape<4, 2>(s, plain, channels, lds);
     else if (cols == 128) launch_direct_shape<4, 1>(s, plain, channels, lds);
     else if (rows == 128) launch_direct_shape<2, 2>(s, plain, channels, lds);

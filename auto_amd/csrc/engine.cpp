@@ -297,10 +297,10 @@ void validate(const ah_lenia_desc& d) {
 
 
 // LDS tile geometry of one direct-path variant: tr rows x tc bf16 columns.
-void direct_shape(DirectArgs& a, int cols, int rows, int ring) {
+void direct_shape(DirectArgs& a, int cols, int rows, int variant) {
     a.cols = cols;
     a.rows = rows;
-    a.ring = ring;
+    a.variant = variant;
     a.tr = rows + 2 * a.halo_y;
     a.tc = cols + 2 * a.halo_x + 32;
     a.tstride = direct_tile_stride(a.tc, false);
@@ -348,7 +348,9 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         nfrag_total += dk.nfrag * (dk.terms == 3 ? 2 : 1);
         a.nfrag_max = std::max(a.nfrag_max, dk.nfrag);
     }
-    // Candidates: the tile kernel at 128x128, 128x64, 64x64, and for one
+    // Candidates: the tile kernel at 128x128, 128x64, 64x64; for several
+    // kernels the same tiles one workgroup per (tile, kernel) with a sum
+    // pass (variant 2, which needs a growth plane per kernel); for one
     // channel, one kernel and one bf16 term the persistent ring kernel at
     // 64 rows (128 or 64 columns).
     using Shape = ah_direct_tile;
@@ -357,8 +359,11 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     for (auto cr : {std::make_pair(128, 128), std::make_pair(128, 64), std::make_pair(64, 64)}) {
         if (cr.first > (int)d.width || cr.second > (int)d.height) continue;
         shape(cr.first, cr.second, 0);
-        if (direct_lds_bytes(a) <= 160 * 1024) shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 0});
+        if (direct_lds_bytes(a) > 160 * 1024) continue;
+        shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 0});
+        if (K > 1) shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 2});
     }
+    if (K > 1) a.gsum = (float*)s->alloc(sizeof(float) * s->cells() * K);
     for (int cols : {128, 64}) {
         if (!ring_ok || cols > (int)d.width || d.width % cols || d.height % 64) continue;
         shape(cols, 64, 1);
@@ -379,7 +384,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
     // otherwise each admissible shape is timed on this world (zeroed states,
     // the work is data-independent) and the fastest kept.  Small worlds can
     // prefer small tiles: more workgroups than CUs.
-    shape(shapes[0].cols, shapes[0].rows, shapes[0].ring);
+    shape(shapes[0].cols, shapes[0].rows, shapes[0].variant);
     if (shapes.size() > 1) {
         const size_t bytes = sizeof(float) * s->cells() * d.channels;
         AH_HIP(hipMemsetAsync(s->state, 0, bytes, s->stream));
@@ -391,7 +396,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         Shape pick = shapes[0];
         try {
             for (const Shape& cr : shapes) {
-                shape(cr.cols, cr.rows, cr.ring);
+                shape(cr.cols, cr.rows, cr.variant);
                 DirectArgs t = a;
                 t.state_in = s->state;
                 t.state_out = s->state_alt;
@@ -414,7 +419,7 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        shape(pick.cols, pick.rows, pick.ring);
+        shape(pick.cols, pick.rows, pick.variant);
     }
 }
 
@@ -783,7 +788,7 @@ ah_status ah_sim_direct_tile(const ah_sim* s, ah_direct_tile* out) {
     return guard([&] {
         if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};
         if (s->d.path == AH_PATH_FFT_FP32) throw Error{AH_ERR_INVALID, "FFT sims have no direct-path tile"};
-        *out = ah_direct_tile{(uint32_t)s->direct.cols, (uint32_t)s->direct.rows, (uint32_t)s->direct.ring};
+        *out = ah_direct_tile{(uint32_t)s->direct.cols, (uint32_t)s->direct.rows, (uint32_t)s->direct.variant};
     });
 }
 
@@ -792,13 +797,14 @@ ah_status ah_sim_set_direct_tile(ah_sim* s, const ah_direct_tile* t) {
         if (!s || !t) throw Error{AH_ERR_INVALID, "null argument"};
         if (s->d.path == AH_PATH_FFT_FP32) throw Error{AH_ERR_INVALID, "FFT sims have no direct-path tile"};
         for (const ah_direct_tile& c : s->direct_tiles)
-            if (c.cols == t->cols && c.rows == t->rows && c.ring == t->ring) {
+            if (c.cols == t->cols && c.rows == t->rows && c.variant == t->variant) {
                 AH_HIP(hipStreamSynchronize(s->stream));
-                direct_shape(s->direct, (int)c.cols, (int)c.rows, (int)c.ring);
+                direct_shape(s->direct, (int)c.cols, (int)c.rows, (int)c.variant);
                 return;
             }
         throw Error{AH_ERR_UNSUPPORTED, "direct tile " + std::to_string(t->cols) + "x" + std::to_string(t->rows) +
-                                            (t->ring ? " (ring)" : "") + " is not admissible for this world"};
+                                            " variant " + std::to_string(t->variant) +
+                                            " is not admissible for this world"};
     });
 }
 

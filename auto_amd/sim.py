@@ -120,14 +120,15 @@ class Sim:
 
     # -- direct-path kernel variant ----------------------------------------
     def direct_tile(self) -> tuple:
-        """(cols, rows, ring) of the direct-path kernel variant in use."""
+        """(cols, rows, variant) of the direct-path kernel in use: variant 0
+        tile kernel, 1 ring kernel, 2 one workgroup per (tile, kernel)."""
         t = DirectTile()
         check(self._lib.ah_sim_direct_tile(self._h, ctypes.byref(t)))
-        return (t.cols, t.rows, t.ring)
+        return (t.cols, t.rows, t.variant)
 
-    def set_direct_tile(self, cols: int, rows: int, ring: int = 0) -> None:
+    def set_direct_tile(self, cols: int, rows: int, variant: int = 0) -> None:
         """Force a direct-path variant (raises if not admissible here)."""
-        t = DirectTile(cols, rows, ring)
+        t = DirectTile(cols, rows, variant)
         check(self._lib.ah_sim_set_direct_tile(self._h, ctypes.byref(t)))
 
     # -- kernel timing ----------------------------------------------------

@@ -229,16 +229,20 @@ ah_status ah_sim_profile_read(ah_sim* sim, ah_kernel_time* out, uint32_t capacit
 /*
  * Direct-path kernel variant.  No counterpart in the reference (its only
  * path is the FFT).  A DIRECT_BF16* sim times every admissible variant on its
- * world at create and keeps the fastest: the tile kernel with a cols x rows
- * output tile (128x128, 128x64, 64x64), or, for one channel, one kernel and
- * one bf16 term, the persistent ring kernel (ring = 1, rows = 64) that
- * carries the shared halo rows from one tile to the next.  Every variant
- * gives the same bits.  ah_sim_direct_tile reports the variant in use;
+ * world at create and keeps the fastest, for a cols x rows output tile
+ * (128x128, 128x64, 64x64): variant 0, the tile kernel (a workgroup computes
+ * every kernel ending in its channel); variant 1, for one channel, one kernel
+ * and one bf16 term, the persistent ring kernel (rows = 64) that carries the
+ * shared halo rows from one tile to the next; variant 2, for several
+ * kernels, one workgroup per (tile, kernel) writing that kernel's weighted
+ * growth to a scratch plane, and a second pass that sums them per channel
+ * in kernel order.  Every variant gives the same bits.
+ * ah_sim_direct_tile reports the variant in use;
  * ah_sim_set_direct_tile forces one (AH_ERR_UNSUPPORTED if it is not
  * admissible for this world; AH_ERR_INVALID on an FFT sim).
  */
 typedef struct {
-    uint32_t cols, rows, ring;
+    uint32_t cols, rows, variant;
 } ah_direct_tile;
 
 ah_status ah_sim_direct_tile(const ah_sim* sim, ah_direct_tile* out);

@@ -143,7 +143,8 @@ def test_direct_auto_precision_meets_gate_on_high_gain_rule(O):
     assert errs["direct_bf16"] < errs["direct_bf16_x1"]
 
 
-VARIANTS = [(128, 128, 0), (128, 64, 0), (64, 64, 0), (128, 64, 1), (64, 64, 1)]
+VARIANTS = [(128, 128, 0), (128, 64, 0), (64, 64, 0), (128, 64, 1), (64, 64, 1), (128, 128, 2), (128, 64, 2),
+            (64, 64, 2)]
 
 
 def _run_variants(d, s, steps):
@@ -175,7 +176,8 @@ def test_direct_tile_variants_agree_bitwise(O, w, h, radius):
     s = np.random.default_rng(w + 3 * h).random((1, h, w), dtype=np.float32)
     got = _run_variants(d, s, 4)
     assert (128, 64, 0) in got or (64, 64, 0) in got
-    rings = [v for v in got if v[2]]
+    assert not [v for v in got if v[2] == 2], "one kernel has no split variant"
+    rings = [v for v in got if v[2] == 1]
     assert rings, "the ring kernel is admissible for one channel, one kernel, one term"
     ref = next(iter(got.values()))
     for v, g in got.items():
@@ -210,3 +212,27 @@ def test_direct_ring_not_admissible_for_three_terms_or_channels():
     m = Sim(W.multichannel_1024(seed=4, side=256, path="direct_bf16"), 2)
     with pytest.raises(EngineError):
         m.set_direct_tile(128, 64, 1)
+
+
+@pytest.mark.parametrize("side", [256, 1024])
+def test_direct_split_variant_matches_tile_kernel_multichannel(O, side):
+    """Several kernels per channel: variant 2 (a workgroup per tile and
+    kernel, then a pass that adds each channel's growth planes in kernel
+    order) forms the same sums as the tile kernel, so the bits agree."""
+    d = W.multichannel_1024(seed=4, side=side, path="direct_bf16")
+    C = d.channels
+    s = np.random.default_rng(side).random((C, side, side), dtype=np.float32)
+    got = _run_variants(d, s, 3)
+    splits = [v for v in got if v[2] == 2]
+    assert splits and [v for v in got if v[2] == 0], sorted(got)
+    assert not [v for v in got if v[2] == 1]
+    ref = got[(128, 64, 0)] if (128, 64, 0) in got else got[(64, 64, 0)]
+    for v, g in got.items():
+        assert np.array_equal(g, ref), v
+    sim = Sim(d, 1)
+    sim.set_direct_tile(*splits[0])
+    sim.write_field("state", s)
+    sim.step(1)
+    err = np.abs(sim.read_field("state") - O.step64(d, s, "fft")).max()
+    print(f"multichannel {side}: variants {sorted(got)}; split step max-abs vs FP64 {err:.3e}")
+    assert err <= 1e-3
