@@ -203,6 +203,9 @@ struct BatchArgs {
 size_t batch_lds_bytes(const BatchArgs& a);
 void launch_direct_batch(hipStream_t s, Profiler* p, const BatchArgs& a, int cols, int rows, int grid);
 int cu_count();  // compute units of the current device
+// Lets `kernel` launch with more than 64 KiB of dynamic LDS on the current
+// device (a no-op at or below 64 KiB); thread-safe, once per device.
+void allow_dynamic_lds(const void* kernel, size_t bytes);
 
 // Row stride (bf16 elements) of a direct-path LDS tile at least tc wide:
 // the smallest multiple of 8 elements (16 bytes) that is 2 or 14 mod 16 in

@@ -539,11 +539,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_batch(BatchArgs a) {
 // stream capture.
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
-    static std::vector<const void*> done;
-    const void* key = reinterpret_cast<const void*>(kernel);
-    if (bytes <= 64 * 1024 || std::find(done.begin(), done.end(), key) != done.end()) return;
-    AH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    done.push_back(key);
+    allow_dynamic_lds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 }  // namespace

@@ -25,6 +25,8 @@
 // packs the two real bins X[0] and X[W/2] as (re, im); the column pass treats
 // that column as two real sequences sharing one complex transform.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <type_traits>
 
 #include "ah_internal.h"
@@ -1073,16 +1075,9 @@ __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     store_half_spectrum(lds, M, P::ROWS, H, y0, a.twW, a.spec_next + (size_t)ch * M * H, 256);
 }
 
-// Dynamic LDS above 64 KiB must be allowed per kernel.  Done once per kernel
-// symbol (up to the 160 KiB a gfx950 workgroup can hold), never inside a
-// stream capture.
 template <typename K>
 void set_lds(K kernel, size_t bytes) {
-    static std::vector<const void*> done;
-    const void* key = reinterpret_cast<const void*>(kernel);
-    if (bytes <= 64 * 1024 || std::find(done.begin(), done.end(), key) != done.end()) return;
-    AH_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    done.push_back(key);
+    allow_dynamic_lds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 // Rows per row-pass workgroup: one row of a wide world, more rows of a
@@ -1101,14 +1096,36 @@ bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
            W >= 64;
 }
 
-// Compute units of the current device (cached per device).
+// Compute units of the current device (cached per device; sims on several
+// host threads may ask at once).
 int cu_count() {
-    static int cached[64] = {};
+    static std::atomic<int> cached[64] = {};
     int dev = 0;
     AH_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) throw Error{AH_ERR_DEVICE, "device index out of range"};
-    if (!cached[dev]) AH_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount, dev));
-    return cached[dev];
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (!n) {
+        AH_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Dynamic LDS above 64 KiB must be allowed per kernel: once per (device,
+// kernel symbol), up to the 160 KiB a gfx950 workgroup can hold, before the
+// kernel's first launch on that device (so never first inside a stream
+// capture).  Guarded by a mutex: sims on different host threads launch
+// concurrently.
+void allow_dynamic_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::vector<std::pair<int, const void*>> done;
+    int dev = 0;
+    AH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (std::find(done.begin(), done.end(), std::make_pair(dev, kernel)) != done.end()) return;
+    AH_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done.emplace_back(dev, kernel);
 }
 
 namespace {
