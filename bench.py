@@ -32,6 +32,19 @@ HBM_PEAK_GBPS = 8000.0
 EVENT_EVERY = 16  # kernel launches per timed launch in the timed region
 BF16_PEAK_TFLOPS = 2500.0
 
+# HBM bytes per launch that rocprofv3 measured for the dominant kernels of
+# the 4096^2 three-ring world (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
+# factors a copy kernel of known size reads; summary and commands in
+# profiles/fft_fp32_rings3_4096.md and profiles/direct_bf16_rings3_4096.md).
+# A bench run cannot collect counters itself, so `roofline.traffic` reports
+# this recorded figure for that world and null for any other size.
+RECORDED_TRAFFIC = {
+    ("row_c2r_update_r2c", 4096, 4096): 274148872.6,
+    ("col_conv", 4096, 4096): 154779087.9,
+    ("direct_bf16_update", 4096, 4096): 195755733.0,
+}
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, recorded in profiles/"
+
 # FFT-path kernels priced by algorithmic HBM bytes (single channel, one
 # kernel; the half spectrum is W/2 x H complex64 = 4 B/cell).
 FFT_KERNELS = ("row_r2c", "col_conv", "row_c2r_update", "row_c2r_update_r2c")
@@ -423,8 +436,10 @@ def main() -> int:
             dom = max((n for n in kern if "bytes" in kern[n]), key=lambda n: kern[n]["avg_us"])
             step_bytes = sum(kern[n]["bytes"] for n in kern if "bytes" in kern[n])
             step_us = sum(kern[n]["avg_us"] for n in kern if "bytes" in kern[n])
+            traffic = RECORDED_TRAFFIC.get((dom, w, h))
             roof = {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBPS, "traffic": None,
+                    "unit": "GB/s", "frac": kern[dom]["GBps"] / HBM_PEAK_GBPS, "traffic": traffic,
+                    "traffic_source": TRAFFIC_SOURCE if traffic else None,
                     "bytes_per_launch": kern[dom]["bytes"],
                     "step": {"bytes": step_bytes, "kernel_us": step_us,
                              "achieved": step_bytes / (step_us * 1e-6) / 1e9,
@@ -436,8 +451,10 @@ def main() -> int:
             tflops = fpc * w * h / (kern[name]["avg_us"] * 1e-6) / 1e12
             kern[name]["flops"] = fpc * w * h
             kern[name]["TFLOPs"] = tflops
+            traffic = RECORDED_TRAFFIC.get((name, w, h))
             roof = {"bound": "mfma", "kernel": name, "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "traffic": None,
+                    "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "traffic": traffic,
+                    "traffic_source": TRAFFIC_SOURCE if traffic else None,
                     "flops_per_cell": fpc, "useful_flops_per_cell": useful_flops_per_cell(d)}
         results[path] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "kernels": kern, "roofline": roof}
         sims[path] = (d, sim)
