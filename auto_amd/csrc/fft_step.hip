@@ -489,16 +489,30 @@ __device__ __forceinline__ void store_column(const float2* buf, float2* __restri
     }
 }
 
+// The same store in two halves: rows t + 512 i into registers, and later
+// from the registers to column `col` of dst.
+__device__ __forceinline__ void read_column(const float2* buf, float2 (&v)[SH / ST]) {
+#pragma unroll
+    for (int i = 0; i < SH / ST; ++i) v[i] = buf[pad(threadIdx.x + i * ST)];
+}
+
+__device__ __forceinline__ void write_column(const float2 (&v)[SH / ST], float2* __restrict__ dst, int col) {
+    float2* d = dst + col_index(col >> 2, col & 3, 0, SH);
+#pragma unroll
+    for (int i = 0; i < SH / ST; ++i) d[(size_t)(threadIdx.x + i * ST) * BC] = v[i];
+}
+
 // Forward FFT, multiply, inverse FFT and write-back of one column, for each
 // of the NK kernels reading it (1, or 2 for SmoothLife and MNCA: the
 // transformed column stays in registers between them).  After the last
 // forward stage thread t holds rows t + 512 r, which are exactly the rows the
 // first inverse stage (span 1) reads: the multiply and that stage run in
 // registers, saving one LDS round trip and two barriers.  kf[j] holds kernel
-// j's factors for these rows.
-template <int NK>
+// j's factors for these rows.  DEFER (one kernel): the result goes to `out`
+// instead of to memory, for the caller to store later.
+template <int NK, bool DEFER>
 __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const float2 (&tw)[3],
-                                            const float (&kf)[NK][8], int col) {
+                                            const float (&kf)[NK][8], int col, float2 (&out)[SH / ST]) {
     column_stage<-1>(buf, 0, tw);
     column_stage<-1>(buf, 1, tw);
     column_stage<-1>(buf, 2, tw);
@@ -527,7 +541,8 @@ __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const
         column_stage<1>(buf, 1, tw);
         column_stage<1>(buf, 2, tw);
         column_stage<1>(buf, 3, tw);
-        store_column(buf, a.spec_out + j * chan, col);
+        if constexpr (DEFER) read_column(buf, out);
+        else store_column(buf, a.spec_out + j * chan, col);
     }
 }
 
@@ -537,6 +552,13 @@ __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const
 // being transformed.  One kernel: K-hat for column i is loaded with column
 // i's prefetch, during column i-1; two kernels: at the start of column i
 // (registers are too few to hold both a column ahead).
+//
+// One kernel: column i's result is read from LDS into registers at its end
+// and stored only after the wait at the top of column i+1.  The wait counter
+// covers loads and stores alike, so a store issued before that wait would
+// hold up the next column until it had completed; issued after it, the
+// stores drain while column i+1 is transformed, and the wait at the top of
+// column i+2 finds them done.
 template <int NC, int NK>
 __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4))) k_col_stream(ColArgs a) {
     __shared__ float2 bufA[padded(SH)];
@@ -551,6 +573,7 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
     }
     const size_t kstride = (size_t)(a.W >> 1) * (SH / 2 + 1);  // one kernel's column-major K-hat
     float kf[2][NK][8];
+    float2 done[SH / ST];  // one kernel: the previous column's result, not yet stored
     if constexpr (NK == 1) load_khat(kf[0][0], a.khat, col0);
     prefetch_column(bufA, a.spec_in, col0);
 #pragma unroll
@@ -562,13 +585,19 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
 #pragma unroll
             for (int j = 0; j < NK; ++j) load_khat(kf[i & 1][j], a.khat + j * kstride, col);
         }
+        // Also waits for this wave's LDS reads of the previous column's
+        // result in nxt, before any wave's prefetch overwrites it.
         prefetch_sync();
         if (i + 1 < NC) {
             if constexpr (NK == 1) load_khat(kf[(i + 1) & 1][0], a.khat, col + G);
             prefetch_column(nxt, a.spec_in, col + G);
         }
-        conv_column<NK>(cur, a, tw, kf[i & 1], col);
+        if constexpr (NK == 1) {
+            if (i > 0) write_column(done, a.spec_out, col - G);
+        }
+        conv_column<NK, NK == 1>(cur, a, tw, kf[i & 1], col, done);
     }
+    if constexpr (NK == 1) write_column(done, a.spec_out, col0 + (NC - 1) * G);
 }
 
 // Fused column pass for batched same-size worlds (diagonal mode: world c's
