@@ -170,12 +170,13 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
 // E = elements per thread = nseq*M/blockDim.x.
 template <int E, int SIGN>
 __device__ void fft_lds(float2* lds, int M, int nseq, const float2* __restrict__ tw, int NT) {
+    constexpr int R0 = E >= 8 ? 8 : 4;  // radix of the main stages (at most E)
     int Ls = 1;
-    while (Ls * 8 <= M) {
-        stage<8, E, SIGN>(lds, M, nseq, Ls, tw, NT);
-        Ls *= 8;
+    while (Ls * R0 <= M) {
+        stage<R0, E, SIGN>(lds, M, nseq, Ls, tw, NT);
+        Ls *= R0;
     }
-    if (Ls * 4 == M) stage<4, E, SIGN>(lds, M, nseq, Ls, tw, NT);
+    if (R0 == 8 && Ls * 4 == M) stage<4, E, SIGN>(lds, M, nseq, Ls, tw, NT);
     else if (Ls * 2 == M) stage<2, E, SIGN>(lds, M, nseq, Ls, tw, NT);
 }
 
@@ -1243,7 +1244,18 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
         AH_HIP(hipGetLastError());
         return;
     }
-    hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / 8), lds, s, a);
+    if (!diag && n_kernels > channels && elems / 4 <= FFT_THREADS && (W / 2 / a.cw) * channels < 4 * cu_count()) {
+        // Several kernels per channel on a grid of under four workgroups per
+        // CU: four elements per thread (radix-4 stages), so twice the waves
+        // hide the latency of each workgroup's longer chain.  Measured per
+        // column pass: multichannel 1024^2 37.0 -> 33.6 us, 512^2 25.4 ->
+        // 23.8 us, SmoothLife 1024^2 23.0 -> 20.0 us; SmoothLife 2048^2 (1024
+        // workgroups) 32.6 -> 38.1 us, so larger grids keep radix 8.
+        set_lds(k_col_conv<4>, lds);
+        hipLaunchKernelGGL(k_col_conv<4>, dim3(W / 2 / a.cw, channels), dim3(elems / 4), lds, s, a);
+    } else {
+        hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / 8), lds, s, a);
+    }
     AH_HIP(hipGetLastError());
 }
 
