@@ -1244,13 +1244,14 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
         AH_HIP(hipGetLastError());
         return;
     }
-    if (!diag && n_kernels > channels && elems / 4 <= FFT_THREADS && (W / 2 / a.cw) * channels < 4 * cu_count()) {
-        // Several kernels per channel on a grid of under four workgroups per
-        // CU: four elements per thread (radix-4 stages), so twice the waves
-        // hide the latency of each workgroup's longer chain.  Measured per
-        // column pass: multichannel 1024^2 37.0 -> 33.6 us, 512^2 25.4 ->
-        // 23.8 us, SmoothLife 1024^2 23.0 -> 20.0 us; SmoothLife 2048^2 (1024
-        // workgroups) 32.6 -> 38.1 us, so larger grids keep radix 8.
+    if (!diag && elems / 4 <= FFT_THREADS && (W / 2 / a.cw) * channels < 4 * cu_count()) {
+        // A grid of under four workgroups per CU: four elements per thread
+        // (radix-4 stages), so twice the waves hide the latency of each
+        // workgroup's chain.  Measured per column pass: multichannel 1024^2
+        // 37.0 -> 33.6 us, 512^2 25.4 -> 23.8 us, SmoothLife 1024^2 23.0 ->
+        // 20.0 us, one kernel at 1024^2 16.1 -> 13.9 us and 256^2 13.5 ->
+        // 11.4 us; SmoothLife 2048^2 (1024 workgroups) 32.6 -> 38.1 us, so
+        // larger grids keep radix 8.
         set_lds(k_col_conv<4>, lds);
         hipLaunchKernelGGL(k_col_conv<4>, dim3(W / 2 / a.cw, channels), dim3(elems / 4), lds, s, a);
     } else {
