@@ -3,7 +3,7 @@
 Tolerances: the state is rounded to bf16 (8-bit mantissa) on the way into
 the MFMA and the taps are rounded to bf16 and renormalised, so the potential
 carries ~1e-5 rms / ~1e-4 worst-case error; the gate for a step is the
-north-star bar of 1e-3 max-abs vs FP64 from the same input state.
+1e-3 max-abs bar vs FP64 from the same input state.
 """
 import numpy as np
 import pytest
