@@ -233,7 +233,7 @@ namespace {
 
 bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
-// Reject what the reference asserts on (spectral.rs:103-116) and what this build cannot run.
+// Reject what the reference asserts on (spectral.rs:103-116) and what this engine cannot run.
 // Largest |dx|, |dy| a kernel touches: its profile is 0 beyond.
 int kernel_reach(const ah_kernel& k) {
     if (k.shape == AH_SHAPE_SMOOTH_DISC || k.shape == AH_SHAPE_SMOOTH_RING)

@@ -407,7 +407,7 @@ def main() -> int:
     if args.torch_leg:
         return torch_leg(args.torch_leg)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}). "
+        print(f"bench.py runs on one GPU only: this engine has no multi-GPU path (asked for {args.gpus}). "
               "A batch of worlds shards by world and one large world by slab FFT, but neither is built.",
               file=sys.stderr)
         return 2

@@ -38,7 +38,7 @@ typedef enum {
     AH_ERR_ALLOC = 3,         /* device allocation failed              */
     AH_ERR_UNKNOWN_FIELD = 4, /* read_field/write_field name unknown   */
     AH_ERR_SIZE = 5,          /* host buffer size does not match       */
-    AH_ERR_UNSUPPORTED = 6,   /* valid request this build cannot serve */
+    AH_ERR_UNSUPPORTED = 6,   /* valid request this engine cannot serve */
     AH_ERR_DIVERGED = 7       /* surrogate judge exceeded its bound    */
 } ah_status;
 

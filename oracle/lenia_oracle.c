@@ -1,7 +1,7 @@
 /*
  * lenia_oracle.c -- CPU oracle for the Lenia world step.  TEST INFRASTRUCTURE.
  *
- * This file is the checker, never the product: only tests/, the smoke check
+ * Test infrastructure, never the product: only tests/, the smoke check
  * and the CPU-baseline leg of bench.py load it.  The HIP engine in auto_amd/
  * does not link, call or fall back to anything here.
  *

@@ -1,7 +1,8 @@
 """Python face of the CPU oracle.  TEST INFRASTRUCTURE.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
-this module, and only as the checker.  The engine never calls it.
+this module, and only to check the engine's results.  The engine never
+calls it.
 
 Wraps oracle/liblenia_oracle.so (the C restatement of the reference's Lenia
 passes, see lenia_oracle.c) and adds the FP64 FFT convolution, computed with

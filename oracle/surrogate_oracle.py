@@ -1,8 +1,8 @@
 """FP64 oracle of the learned k-step surrogate.  TEST INFRASTRUCTURE.
 
-Only tests/ import this module, as the checker of the GPU network's forward
-pass and gradients.  The reference has no learned part (it is new in this
-build), so the network is defined here and in auto_amd/csrc/surrogate.hip
+Only tests/ import this module, to check the GPU network's forward pass and
+gradients.  The reference has no learned part (it is new in this engine),
+so the network is defined here and in auto_amd/csrc/surrogate.hip
 from the same written specification:
 
   x      [N][H][W]   float32 state (torus), H and W even
