@@ -172,6 +172,7 @@ struct DirectArgs {
     DirectKernel k[AH_MAX_KERNELS];
     const int4* sched;   // per kernel, its row list
     int sched_base[AH_MAX_KERNELS];
+    int split_order[AH_MAX_KERNELS];  // variant 2: kernel of grid slice z, costliest first
     const uint4* frags;  // [fragment][64 lanes] x 8 bf16
     int nfrag_max;
     const float* state_in;

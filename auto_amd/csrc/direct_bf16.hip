@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_split(DirectArgs a) {
     constexpr int DX = 32 * S, DY = 64 * NB;
     const int W = a.W, H = a.H;
     const int x0 = blockIdx.x * DX, y0 = blockIdx.y * DY;
-    const int kn = blockIdx.z;
+    const int kn = a.split_order[blockIdx.z];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int par = wave & 1, grp = wave >> 1;
     const int q = lane >> 4, n = lane & 15;

@@ -363,7 +363,19 @@ void setup_direct(ah_sim* s, const std::vector<double*>& taps) {
         shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 0});
         if (K > 1) shapes.push_back({(uint32_t)cr.first, (uint32_t)cr.second, 2});
     }
-    if (K > 1) a.gsum = (float*)s->alloc(sizeof(float) * s->cells() * K);
+    if (K > 1) {
+        a.gsum = (float*)s->alloc(sizeof(float) * s->cells() * K);
+        // Slices of the costliest kernels (MFMAs per tile) go first, so the
+        // last workgroups to start are the cheap ones.
+        std::vector<int> order(K);
+        for (int k = 0; k < K; ++k) order[k] = k;
+        auto cost = [&](int k) {
+            const DirectKernel& dk = a.k[k];
+            return (3 * dk.nrow[0] + 2 * dk.nrow[1] + dk.nrow[2]) * (dk.terms == 3 ? 3 : 1);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+        for (int k = 0; k < K; ++k) a.split_order[k] = order[k];
+    }
     for (int cols : {128, 64}) {
         if (!ring_ok || cols > (int)d.width || d.width % cols || d.height % 64) continue;
         shape(cols, 64, 1);
