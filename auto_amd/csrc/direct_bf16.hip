@@ -449,6 +449,13 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
                 nv[u] = *reinterpret_cast<const float4*>(a.state_in + (size_t)gy * W + gxn);
             }
         }
+        // The FP32 state this tile's epilogue updates, loaded now so that it
+        // too lands during the MFMAs instead of after them.
+        const int yo = t * DY + grp * 16 + n;
+        float4 sv[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            sv[k] = *reinterpret_cast<const float4*>(a.state_in + (size_t)yo * W + x0 + 16 * (par + 2 * k) + 4 * q);
         // This tile's ring rows and the fragments are in LDS.  The barrier
         // does not wait for the loads just issued: they land during the MFMAs.
         lds_sync();
@@ -471,13 +478,11 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
                     make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
             }
         }
-        const int yo = t * DY + grp * 16 + n;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const int xo = x0 + 16 * (par + 2 * k) + 4 * q;
             const size_t o = (size_t)yo * W + xo;
-            float4 sv = *reinterpret_cast<const float4*>(a.state_in + o);
-            float r[4] = {sv.x, sv.y, sv.z, sv.w};
+            float r[4] = {sv[k].x, sv[k].y, sv[k].z, sv[k].w};
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 r[c] = lenia_update(r[c], a.dt,
