@@ -423,10 +423,22 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
     int4* rows_lds = reinterpret_cast<int4*>(frag_lds + (size_t)a.nfrag_max * 64);
     const DirectKernel kk = a.k[0];
     const int4* rows = a.sched + a.sched_base[0];
-    for (int i = tid; i < kk.nfrag * 64; i += DTHREADS) frag_lds[i] = a.frags[(size_t)kk.frag0 * 64 + i];
+    // The fragment table goes to LDS by DMA (16 bytes per lane, one wave
+    // instruction per 64 entries), so its loads are in flight together with
+    // the first tile's instead of one round trip ahead of them; the waits
+    // for the tile's loads (in issue order) cover it before the first
+    // barrier.
+    {
+        const uint4* src = a.frags + (size_t)kk.frag0 * 64;
+        const int w0 = __builtin_amdgcn_readfirstlane(wave);
+        for (int i0 = w0 * 64; i0 < kk.nfrag * 64; i0 += DTHREADS)
+            __builtin_amdgcn_global_load_lds(src + i0 + lane, (__attribute__((address_space(3))) void*)(frag_lds + i0),
+                                             16, 0, 0);
+    }
     for (int i = tid; i < kk.nrow[0] + kk.nrow[1] + kk.nrow[2]; i += DTHREADS) rows_lds[i] = rows[i];
     const int ybase = t0 * DY;
     stage_tile(ring, a.state_in, W, H, x0, ybase, DX, a.halo_y, a.halo_x, a.tr, a.tc, a.tstride, false);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the fragment DMA, whatever the tile's waits covered
 
     // This lane's column of the streamed rows (the tile is at most 256 wide).
     const int jn = 4 * lane;
