@@ -228,16 +228,27 @@ __device__ __forceinline__ void kernel_potential(f32x4 (&acc)[S][NB], uint16_t* 
         const int fbase = frag0 + (term == 1 ? nfrag : 0);
         __syncthreads();  // earlier reads of tile / fragments are done
         const int want = src * 2 + (lo_tile ? 1 : 0);
+        const bool new_frags = loaded != fbase;
+        if (new_frags) {
+            // Fragment table by DMA (16 bytes per lane, one wave instruction
+            // per 64 entries), in flight with the tile's loads; the barrier
+            // below waits for it.
+            const uint4* fsrc = frags + (size_t)fbase * 64;
+            const int w0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+            for (int i0 = w0 * 64; i0 < nfrag * 64; i0 += DTHREADS)
+                __builtin_amdgcn_global_load_lds(fsrc + i0 + (threadIdx.x & 63),
+                                                 (__attribute__((address_space(3))) void*)(frag_lds + i0), 16, 0, 0);
+        }
         if (staged != want) {
             stage_tile(tile, src_plane, W, H, x0, y0, 32 * S, halo_y, halo_x, tr, tc, tstride, lo_tile);
             staged = want;
         }
-        if (loaded != fbase) {
-            for (int i = threadIdx.x; i < nfrag * 64; i += DTHREADS) frag_lds[i] = frags[(size_t)fbase * 64 + i];
+        if (new_frags) {
             const int nr = nrow[0] + nrow[1] + nrow[2];
             for (int i = threadIdx.x; i < nr; i += DTHREADS) rows_lds[i] = rows[i];
             loaded = fbase;
         }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the fragment DMA
         __syncthreads();
         conv_tile<S, NB>(acc, tile, frag_lds, rows_lds, nrow, halo_y, halo_x, tstride, tr + 1, 0);
     }
