@@ -105,22 +105,21 @@ __device__ __forceinline__ void dft(float2* v) {
     else dft8<SIGN>(v);
 }
 
-// Twiddles w^r, r = 1..R-1, with w = tw[k*tstep] (conjugated for SIGN > 0),
-// from at most three table loads (w, w^2, w^4) and their products.
+// Twiddles w^r, r = 1..R-1, with w = tw[k*tstep] (conjugated for SIGN > 0):
+// one table load and products (w^2 = w w, w^4 = w^2 w^2, ...; a few ulp).
+// The lookups are scattered, so each one is an L2 request of its own, and
+// the passes are bound by L2 requests: three loads per stage instead of
+// one cost the 4096^2 row pass several microseconds.
 template <int R, int SIGN>
 __device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ tw, int k, int tstep) {
-    auto ld = [&](int m) {
-        float2 v = tw[m * k * tstep];
-        if (SIGN > 0) v.y = -v.y;
-        return v;
-    };
-    w[1] = ld(1);
+    w[1] = tw[k * tstep];
+    if (SIGN > 0) w[1].y = -w[1].y;
     if constexpr (R >= 4) {
-        w[2] = ld(2);
+        w[2] = cmul(w[1], w[1]);
         w[3] = cmul(w[1], w[2]);
     }
     if constexpr (R >= 8) {
-        w[4] = ld(4);
+        w[4] = cmul(w[2], w[2]);
         w[5] = cmul(w[1], w[4]);
         w[6] = cmul(w[2], w[4]);
         w[7] = cmul(w[3], w[4]);
