@@ -111,8 +111,13 @@ __device__ __forceinline__ void dft(float2* v) {
 // the passes are bound by L2 requests: three loads per stage instead of
 // one cost the 4096^2 row pass several microseconds.
 template <int R, int SIGN>
-__device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ tw, int k, int tstep) {
-    w[1] = tw[k * tstep];
+__device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ tw, int k, int tstep, int nt) {
+    (void)tw;
+    float x = (float)(k * tstep) / (float)nt;  // exact: a dyadic fraction of a turn
+    // Opaque, so the compiler never folds a constant angle with its own
+    // cosine: every kernel then gets the hardware's bits for every angle.
+    asm volatile("" : "+v"(x));
+    w[1] = make_float2(__builtin_amdgcn_cosf(x), -__builtin_amdgcn_sinf(x));
     if (SIGN > 0) w[1].y = -w[1].y;
     if constexpr (R >= 4) {
         w[2] = cmul(w[1], w[1]);
@@ -154,7 +159,7 @@ __device__ __forceinline__ void stage(float2* lds, int M, int nseq, int Ls, cons
     for (int i = 0; i < NB; ++i) {
         if (Ls > 1) {
             float2 w[R];
-            twiddles<R, SIGN>(w, tw, kk[i], tstep);
+            twiddles<R, SIGN>(w, tw, kk[i], tstep, NT);
 #pragma unroll
             for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], w[r]);
         }
@@ -672,7 +677,7 @@ __global__ void __launch_bounds__(ColPlan<H>::THREADS) k_col_fused(ColArgs a) {
 #pragma unroll
         for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
         float2 w[P::RL];
-        twiddles<P::RL, -1>(w, a.twH, b, 1);  // table stride H / (L * RL)
+        twiddles<P::RL, -1>(w, a.twH, b, 1, H);  // table stride H / (L * RL)
 #pragma unroll
         for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
         dft<P::RL, -1>(q);
@@ -978,7 +983,7 @@ __device__ __forceinline__ void inverse_row(float2* lds, const RowArgs& a, const
 #pragma unroll
         for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
         float2 w[P::RL];
-        twiddles<P::RL, 1>(w, a.twW, b, W / M);  // table stride W / (L * RL)
+        twiddles<P::RL, 1>(w, a.twW, b, W / M, W);  // table stride W / (L * RL)
 #pragma unroll
         for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
         dft<P::RL, 1>(q);
@@ -1058,7 +1063,7 @@ __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
 #pragma unroll
                 for (int r = 0; r < P::RL; ++r) q[r] = lds[pad(off + b + r * P::L)];
                 float2 w[P::RL];
-                twiddles<P::RL, 1>(w, a.twW, b, W / M);
+                twiddles<P::RL, 1>(w, a.twW, b, W / M, W);
 #pragma unroll
                 for (int r = 1; r < P::RL; ++r) q[r] = cmul(q[r], w[r]);
                 dft<P::RL, 1>(q);
