@@ -39,9 +39,9 @@ BF16_PEAK_TFLOPS = 2500.0
 # A bench run cannot collect counters itself, so `roofline.traffic` reports
 # this recorded figure for that world and null for any other size.
 RECORDED_TRAFFIC = {
-    ("row_c2r_update_r2c", 4096, 4096): 274150103.5,
-    ("col_conv", 4096, 4096): 154876510.9,
-    ("direct_bf16_update", 4096, 4096): 169347811.5,
+    ("row_c2r_update_r2c", 4096, 4096): 273239053.4,
+    ("col_conv", 4096, 4096): 153659249.4,
+    ("direct_bf16_update", 4096, 4096): 169424859.8,
 }
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, recorded in profiles/"
 
