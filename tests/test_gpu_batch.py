@@ -162,11 +162,13 @@ def _world_wh(w, h, radius, seed_dt=0.1):
 def test_batch_fft_group_shapes(O):
     """FFT groups at the edges of the batch kernels' ranges: heights 64 and
     2048 (the fused column pass with four and two columns per workgroup),
-    height 4096 (the generic column pass in diagonal mode), width 64 (the
-    generic row pass) and widths 128 and 2048 (the fused row pass), two
-    worlds per group.  Every world vs a standalone Sim (within 1e-5) and vs
-    the FP64 step (within 1e-3) from the same input."""
-    shapes = [(64, 64, 8), (2048, 2048, 20), (64, 4096, 12), (2048, 64, 12), (128, 2048, 16)]
+    512 and 1024 (the fused column pass at the benchmark batch's two largest
+    sides), height 4096 (the generic column pass in diagonal mode), width 64
+    (the generic row pass) and widths 128, 512, 1024 and 2048 (the fused row
+    pass), two worlds per group.  Every world vs a standalone Sim (within
+    1e-5) and vs the FP64 step (within 1e-3) from the same input."""
+    shapes = [(64, 64, 8), (2048, 2048, 20), (64, 4096, 12), (2048, 64, 12), (128, 2048, 16),
+              (512, 512, 18), (1024, 1024, 24)]
     worlds = [(_world_wh(w, h, r), 10 * i + j) for i, (w, h, r) in enumerate(shapes) for j in range(2)]
     b = Batch(worlds)
     info = b.info()
@@ -185,3 +187,61 @@ def test_batch_fft_group_shapes(O):
         assert got.std() > 0, i  # the patch is alive
     print(f"group shapes: worst max-abs vs FP64 {worst64:.3e}, vs standalone Sims {worst_sim:.3e}")
     assert worst64 <= 1e-3 and worst_sim < 1e-5
+
+
+def _bench_batch_vs_singles(O, worlds, b):
+    """Every world of a stepped batch against a standalone FFT Sim and the
+    FP64 oracle step from the same input state: (worst vs Sim, worst vs FP64)."""
+    before = [b.read_world(i) for i in range(len(worlds))]
+    b.step(1)
+    worst_sim = worst64 = 0.0
+    where = None
+    for i, (d, seed) in enumerate(worlds):
+        got = b.read_world(i)
+        plain = LeniaDesc.from_buffer_copy(d)
+        plain.path = PATH_FFT_FP32
+        sim = Sim(plain, seed)
+        sim.write_field("state", before[i][None])
+        sim.step(1)
+        e = float(np.abs(got - sim.read_field("state")[0]).max())
+        if e > worst_sim:
+            worst_sim, where = e, (i, d.width)
+        sim.close()
+        worst64 = max(worst64, float(np.abs(got - O.step64(d, before[i][None], "fft")[0]).max()))
+    return worst_sim, worst64, where
+
+
+def test_benchmark_batch_384_auto(O):
+    """The 384-world batch exactly as the benchmark builds it (seed 1, sides
+    {128, 256, 512, 1024}, path AUTO): after 2 steps, every world's next step
+    matches a standalone FFT Sim from the same input (within 1e-5) and the
+    FP64 oracle step (within 1e-3).  All four size groups are present, so
+    the fused column pass runs at heights 128..1024 and the diagonal row
+    pass at widths 128..1024."""
+    worlds = W.uneven_worlds(seed=1, n=384, path="auto")
+    b = Batch(worlds)
+    info = b.info()
+    sides = {d.width for d, _ in worlds}
+    assert sides == {128, 256, 512, 1024}
+    print(f"benchmark batch: {info.fft_worlds} FFT worlds in {info.fft_groups} groups, {info.ntiles} direct tiles")
+    b.step(2)
+    worst_sim, worst64, where = _bench_batch_vs_singles(O, worlds, b)
+    print(f"384 worlds: worst max-abs vs standalone FFT Sims {worst_sim:.3e} (world, side {where}), "
+          f"vs FP64 {worst64:.3e}")
+    assert worst_sim < 1e-5 and worst64 <= 1e-3
+
+
+def test_benchmark_batch_384_graph_replays_equal_eager():
+    """The benchmark batch stepped by graph replays (100 steps each) equals
+    the same batch stepped by single eager launches, bit for bit, after the
+    first replay and after a later one with eager work in between."""
+    worlds = W.uneven_worlds(seed=1, n=384, path="auto")
+    a, b = Batch(worlds), Batch(worlds)
+    for rnd in range(2):
+        a.step(100)
+        for _ in range(100):
+            b.step(1)
+        a.wait()
+        b.wait()
+        for i in range(len(worlds)):
+            assert np.array_equal(a.read_world(i), b.read_world(i)), (rnd, i)

@@ -34,3 +34,12 @@ def test_bench_prints_one_contract_line():
     assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
     assert 0 < r["frac"] < 1 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9
     assert r["traffic"] is None  # recorded only for the 4096^2 world
+    # Median of repeated blocks of exactly --steps steps, >= 256 event
+    # samples behind the roofline's kernel time, and the per-step kernel sum
+    # within the measured wall time per step, give or take the ~1.5 us of
+    # dispatch an event pair adds to each timed launch (2 launches per step).
+    assert b["timing"]["blocks"] >= 3 and b["timing"]["p10"] <= b["value"] * 1.0001
+    assert r["samples"] >= 256
+    assert r["step"]["kernel_us"] <= r["step"]["wall_us"] * 1.02 + 2 * 2.0
+    dr = b["paths"]["direct_bf16"]["roofline"]
+    assert dr["bound"] == "mfma" and dr["flops_per_cell"] == 2.0 * 53 ** 2 and dr["samples"] >= 256

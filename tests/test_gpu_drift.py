@@ -42,3 +42,47 @@ def test_mass_drift_1000_steps(fp64_trajectory_mass, path):
     drift = np.abs(mass - ref) / ref
     print(f"{path}: mass drift vs FP64 max {drift.max():.3e} final {drift[-1]:.3e} (mass {mass[-1]:.1f} vs {ref[-1]:.1f})")
     assert drift.max() <= 0.01
+
+
+def _one_step_errors(O, d, sim, steps, every):
+    """Every `every` steps: one step from the current state vs the FP64
+    oracle step from the same input, per channel (max-abs, live cells)."""
+    out = []
+    t = 0
+    while t < steps:
+        s = sim.read_field("state")
+        sim.step(1)
+        got = sim.read_field("state")
+        want = O.step64(d, s, "fft")
+        out.append([(float(np.abs(got[c].astype(np.float64) - want[c]).max()), int(np.count_nonzero(want[c])))
+                    for c in range(d.channels)])
+        sim.step(every - 1)
+        t += every
+    return out
+
+
+@pytest.mark.parametrize("path", ["fft_fp32", "direct_bf16"])
+def test_one_step_error_along_1000_steps(O, path):
+    """The three-ring rule at 1024^2: one-step max-abs vs FP64 sampled every
+    100 steps over 1000 steps stays <= 1e-3 on both paths, while the world
+    stays alive."""
+    d = W.rings3_4096(side=1024, path=path)
+    sim = Sim(d, 7)
+    errs = _one_step_errors(O, d, sim, 1000, 100)
+    worst = max(e[0][0] for e in errs)
+    print(f"{path}: one-step max-abs vs FP64 every 100 steps: {[f'{e[0][0]:.1e}' for e in errs]}")
+    assert worst <= 1e-3
+    assert all(e[0][1] > 0 for e in errs)
+
+
+@pytest.mark.parametrize("path", ["fft_fp32", "direct_bf16"])
+def test_multichannel_error_per_channel_along_500_steps(O, path):
+    """The 3-channel, 9-kernel rule at 1024^2: per channel, one-step max-abs
+    vs FP64 sampled every 50 steps over 500 steps stays <= 1e-3."""
+    d = W.multichannel_1024(seed=1, path=path)
+    sim = Sim(d, 1)
+    errs = _one_step_errors(O, d, sim, 500, 50)
+    per = [max(e[c][0] for e in errs) for c in range(d.channels)]
+    live = [[e[c][1] for c in range(d.channels)] for e in errs]
+    print(f"{path}: worst per-channel one-step max-abs vs FP64 {per}; live cells per sample {live}")
+    assert max(per) <= 1e-3
