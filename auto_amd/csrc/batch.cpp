@@ -137,8 +137,6 @@ void validate_world(const ah_lenia_desc& d, uint32_t i) {
     const bool direct = d.path == AH_PATH_DIRECT_BF16 || d.path == AH_PATH_DIRECT_BF16_X1 || d.path == AH_PATH_DIRECT_BF16_X3;
     if (direct && reach > 31) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": direct path radius above 31"};
     if (reach > 127) throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": radius above 127"};
-    if ((d.path == AH_PATH_FFT_FP32 || (d.path == AH_PATH_AUTO && reach > 31)) && d.height > 4096)
-        throw Error{AH_ERR_UNSUPPORTED, "world " + std::to_string(i) + ": FFT path height above 4096"};
     if (k.rings < 1 || k.rings > AH_MAX_RINGS) bad("rings must be 1..3");
     if (!(k.sigma > 0.0f) || !std::isfinite(k.mu)) bad("sigma must be positive, mu finite");
     if (!std::isfinite(d.dt) || d.dt <= 0.0f) bad("dt must be a positive finite number");
@@ -337,7 +335,7 @@ void setup(ah_batch* b) {
                     int nrow[3];
                     (void)build_fragments(t, R, terms, s_tmp, f_tmp, nf, nrow);
                     const int chunks = 3 * nrow[0] + 2 * nrow[1] + nrow[2];
-                    use_fft = chunks * terms > AH_BATCH_FFT_ABOVE && d.height <= 4096;
+                    use_fft = chunks * terms > AH_BATCH_FFT_ABOVE;
                     if (!use_fft) auto_direct_cells += (size_t)d.width * d.height;
                 }
             } else if (!use_fft) {
@@ -348,7 +346,7 @@ void setup(ah_batch* b) {
         }
         if (!explicit_direct && auto_direct_cells * 20 < all_cells)
             for (size_t w = 0; w < n; ++w)
-                if (b->d[w].path == AH_PATH_AUTO && b->d[w].height <= 4096) fft_path[w] = 1;
+                if (b->d[w].path == AH_PATH_AUTO) fft_path[w] = 1;
         for (size_t w = 0; w < n; ++w) {
             const ah_lenia_desc& d = b->d[w];
             const int R = (int)std::ceil((double)d.kernels[0].radius);

@@ -188,8 +188,9 @@ struct ah_sim {
     size_t total_bytes = 0;
     uint64_t generation = 0;
 
-    float* state = nullptr;  // [C][H][W]
-    float* state_alt = nullptr;
+    float* state = nullptr;      // [C][H][W]
+    float* state_alt = nullptr;  // the last rule step's input (steps write out of place)
+    bool stepped = false;        // a rule step has run: state_alt is its input
     KernelParams kp[AH_MAX_KERNELS]{};
     int ksrc[AH_MAX_KERNELS]{};
     int reach[AH_MAX_KERNELS]{};
@@ -283,8 +284,6 @@ void validate(const ah_lenia_desc& d) {
             throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path takes Lenia-shaped kernels"};
     if (d.path == AH_PATH_AUTO) bad("AH_PATH_AUTO chooses a path per world in a batch; a Sim takes a fixed path");
     if (d.path > AH_PATH_AUTO) bad("unknown path");
-    if (d.path == AH_PATH_FFT_FP32 && d.height > 4096)
-        throw Error{AH_ERR_UNSUPPORTED, "FFT path holds one column block in LDS: height must be <= 4096"};
     if (d.path != AH_PATH_FFT_FP32) {
         if (d.width < 64 || d.height < 64)
             throw Error{AH_ERR_UNSUPPORTED, "direct BF16 path needs width >= 64 and height >= 64"};
@@ -485,6 +484,7 @@ void setup(ah_sim* s) {
         if (d.path == AH_PATH_FFT_FP32) {
             const size_t spec_elems = (size_t)(W / 2) * H;  // complex per channel / kernel
             s->spec_src = (float2*)s->alloc(sizeof(float2) * spec_elems * C);
+            s->state_alt = (float*)s->alloc(sizeof(float) * s->cells() * C);
             bool alias = (K == 1 && C == 1);
             s->spec_k = alias ? s->spec_src : (float2*)s->alloc(sizeof(float2) * spec_elems * K);
             const size_t np = (size_t)H / 2 + 1;
@@ -540,23 +540,30 @@ void setup(ah_sim* s) {
     for (void* p : scratch) (void)hipFree(p);
 }
 
-void step_fft(ah_sim* s, float* conv_out, int conv_k) {
+// One rule step (conv_out == nullptr): reads `state`, writes `state_alt`,
+// then the two swap, so the step's input stays readable as state_alt.  With
+// conv_out set, only kernel conv_k's potential of `input` is written there
+// and the state is left alone.
+void step_fft(ah_sim* s, float* conv_out, int conv_k, const float* input) {
     const ah_lenia_desc& d = s->d;
     const int W = (int)d.width, H = (int)d.height, C = (int)d.channels, K = (int)d.n_kernels;
     // Pass 1 runs only when the state changed outside a step; otherwise the
     // previous step's pass 3 already left the spectra of the current state.
-    if (!s->spec_ready) launch_row_r2c(s->stream, &s->prof, W, H, C, s->state, s->spec_src, s->twW);
+    if (conv_out || !s->spec_ready) launch_row_r2c(s->stream, &s->prof, W, H, C, input, s->spec_src, s->twW);
     // The column pass consumes spec_src (in place when it aliases spec_k).
     s->spec_ready = false;
     launch_col_conv(s->stream, &s->prof, W, H, C, s->spec_src, s->spec_k, s->khat, s->kpack, s->twH, s->ksrc, K);
-    launch_row_c2r(s->stream, &s->prof, W, H, C, K, s->spec_k, s->state, s->state, s->kp, d.dt, s->twW, conv_out,
+    launch_row_c2r(s->stream, &s->prof, W, H, C, K, s->spec_k, input, s->state_alt, s->kp, d.dt, s->twW, conv_out,
                    conv_k, s->spec_src, (int)d.rule);
-    s->spec_ready = conv_out == nullptr;
+    if (!conv_out) {
+        std::swap(s->state, s->state_alt);
+        s->spec_ready = true;
+    }
 }
 
-void step_direct(ah_sim* s, float* conv_out, int conv_k) {
+void step_direct(ah_sim* s, float* conv_out, int conv_k, const float* input) {
     DirectArgs a = s->direct;
-    a.state_in = s->state;
+    a.state_in = input;
     a.state_out = s->state_alt;
     a.conv_out = conv_out;
     a.conv_k = conv_k;
@@ -564,9 +571,11 @@ void step_direct(ah_sim* s, float* conv_out, int conv_k) {
     if (!conv_out) std::swap(s->state, s->state_alt);
 }
 
-void step_once(ah_sim* s, float* conv_out, int conv_k) {
-    if (s->d.path == AH_PATH_FFT_FP32) step_fft(s, conv_out, conv_k);
-    else step_direct(s, conv_out, conv_k);
+void step_once(ah_sim* s, float* conv_out = nullptr, int conv_k = 0, const float* input = nullptr) {
+    if (!input) input = s->state;
+    if (s->d.path == AH_PATH_FFT_FP32) step_fft(s, conv_out, conv_k, input);
+    else step_direct(s, conv_out, conv_k, input);
+    if (!conv_out) s->stepped = true;
 }
 
 
@@ -582,7 +591,7 @@ float* sim_state(ah_sim* s) {
 hipStream_t sim_stream(ah_sim* s) { return s->stream; }
 const ah_lenia_desc& sim_desc(const ah_sim* s) { return s->d; }
 void sim_advance(ah_sim* s, uint32_t steps) {
-    for (uint32_t i = 0; i < steps; ++i) step_once(s, nullptr, 0);
+    for (uint32_t i = 0; i < steps; ++i) step_once(s);
     s->generation += steps;
 }
 void sim_add_generations(ah_sim* s, uint64_t n) { s->generation += n; }
@@ -683,7 +692,7 @@ ah_status ah_sim_step(ah_sim* s, uint32_t steps, uint64_t* generations) {
     return guard([&] {
         if (!s) throw Error{AH_ERR_INVALID, "null sim"};
         AH_HIP(hipSetDevice(s->device));
-        for (uint32_t i = 0; i < steps; ++i) step_once(s, nullptr, 0);
+        for (uint32_t i = 0; i < steps; ++i) step_once(s);
         s->generation += steps;
         if (generations) *generations = steps;
     });
@@ -707,13 +716,25 @@ ah_status ah_sim_read_field(ah_sim* s, const char* name, void* dst, size_t bytes
             AH_HIP(hipStreamSynchronize(s->stream));
             return;
         }
-        if (std::strncmp(name, "conv", 4) == 0 && name[4] >= '0' && name[4] <= '9' && name[5] == 0) {
-            int k = name[4] - '0';
+        // "conv<k>": the reference's conv_i (spectral.rs:227-234), which each
+        // step writes: kernel k's potential of that step's input, zero before
+        // the first step; a state write leaves it alone.  It is recomputed
+        // here from the last step's input, which stepping out of place keeps.
+        // "potential<k>": kernel k's potential of the current state.
+        const bool conv = std::strncmp(name, "conv", 4) == 0;
+        const bool pot = std::strncmp(name, "potential", 9) == 0;
+        const char* num = conv ? name + 4 : pot ? name + 9 : nullptr;
+        if (num && num[0] >= '0' && num[0] <= '9' && num[1] == 0) {
+            int k = num[0] - '0';
             if (k >= (int)s->d.n_kernels) throw Error{AH_ERR_UNKNOWN_FIELD, std::string("no kernel for ") + name};
-            if (bytes != plane) throw Error{AH_ERR_SIZE, "conv fields are height*width*4 bytes"};
+            if (bytes != plane) throw Error{AH_ERR_SIZE, "conv and potential fields are height*width*4 bytes"};
+            if (conv && !s->stepped) {
+                std::memset(dst, 0, bytes);
+                return;
+            }
             float* u = nullptr;
             AH_HIP(hipMallocAsync((void**)&u, plane, s->stream));
-            step_once(s, u, k);
+            step_once(s, u, k, conv ? s->state_alt : s->state);
             hipError_t e = hipMemcpyAsync(dst, u, bytes, hipMemcpyDeviceToHost, s->stream);
             (void)hipFreeAsync(u, s->stream);
             AH_HIP(e);

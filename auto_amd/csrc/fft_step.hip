@@ -1258,6 +1258,11 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
         // larger grids keep radix 8.
         set_lds(k_col_conv<4>, lds);
         hipLaunchKernelGGL(k_col_conv<4>, dim3(W / 2 / a.cw, channels), dim3(elems / 4), lds, s, a);
+    } else if (elems / 8 > FFT_THREADS) {
+        // H = 8192: one 66 KiB column per workgroup, sixteen elements per
+        // thread (two radix-8 butterflies per thread and stage).
+        set_lds(k_col_conv<16>, lds);
+        hipLaunchKernelGGL(k_col_conv<16>, dim3(W / 2 / a.cw, channels), dim3(elems / 16), lds, s, a);
     } else {
         hipLaunchKernelGGL(k_col_conv<8>, dim3(W / 2 / a.cw, channels), dim3(elems / 8), lds, s, a);
     }

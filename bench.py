@@ -374,7 +374,7 @@ def time_torch_fft(side: int, steps: int, seed: int):
     impulse = np.zeros((1, d.height, d.width), np.float32)
     impulse[0, 0, 0] = 1.0
     probe.write_field("state", impulse)
-    kern = probe.read_field("conv0").reshape(d.height, d.width)
+    kern = probe.read_field("potential0").reshape(d.height, d.width)
     del probe
     ours = Sim(d, seed)
     ours.step(20)

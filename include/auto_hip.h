@@ -178,9 +178,14 @@ ah_status ah_sim_wait(ah_sim* sim);
 /*
  * Sim::read_field / write_field (sim.rs:559-595).  Fields:
  *   "state"  float32 [channels][height][width], read/write;
- *   "conv<k>" float32 [height][width], read-only: kernel k convolved with its
- *            source channel of the current state, computed by the sim's path
- *            (the reference's conv_i field, spectral.rs:227-234).
+ *   "conv<k>" float32 [height][width], read-only: the reference's conv_i
+ *            field (spectral.rs:227-234): kernel k convolved with its source
+ *            channel of the last step's input state, i.e. the potential that
+ *            step used; zero before the first step; a state write leaves it
+ *            unchanged.  Computed on read by the sim's path (steps write the
+ *            state out of place, so their input is still held).
+ *   "potential<k>" float32 [height][width], read-only (not in the reference):
+ *            kernel k convolved with its source channel of the current state.
  * `bytes` must equal the field size exactly.
  */
 ah_status ah_sim_read_field(ah_sim* sim, const char* name, void* dst, size_t bytes);
@@ -254,7 +259,7 @@ ah_status ah_sim_set_direct_tile(ah_sim* sim, const ah_direct_tile* tile);
  * Lenia kernel, width and height >= 64, and an ORBIUM, PATCH or ZERO init
  * seeded with its own seed; worlds differ in size, rule, dt and path.
  * Direct BF16 worlds (radius <= 31) advance together in one persistent launch
- * per step, over tiles of every such world.  FFT worlds (height <= 4096) are
+ * per step, over tiles of every such world.  FFT worlds are
  * grouped by size; each group runs as one column-pass and one row-pass launch
  * per step, one world per channel.  AH_PATH_AUTO worlds take the FFT path when
  * their direct cost (32-column MFMA chunks summed over the kernel rows, times

@@ -27,3 +27,15 @@ def test_rows_and_verify(tmp_path, capsys):
         assert "gfx950" in r["adapter"] and r["batches"] >= 3
     assert [(r["id"], r["width"]) for r in rows[:2]] == [("lenia", 128), ("lenia", 256)]  # small to big
     assert bench_rows.main(["life", "--sizes", "64x64", "--out", str(out)]) == 2
+
+
+@pytest.mark.gpu
+def test_row_at_8192(tmp_path):
+    """A Lenia row at 8192^2, the largest size the reference's bench runs
+    (scripts/bench_all.sh:50-52, results/gpu.jsonl:82)."""
+    out = tmp_path / "gpu.jsonl"
+    assert bench_rows.main(["lenia", "--sizes", "8192x8192", "--seconds", "0.3", "--warmup", "2",
+                            "--out", str(out), "--tag", "test"]) == 0
+    (r,) = [json.loads(l) for l in out.read_text().splitlines()]
+    assert (r["width"], r["height"]) == (8192, 8192) and r["cells_per_second"] > 0
+    print(f"lenia 8192^2: {r['cells_per_second']:.3e} cell-updates/s")

@@ -27,7 +27,7 @@ def test_direct_conv_matches_fp64(O, w, h, radius, beta):
     sim = Sim(d, 1)
     s = np.random.default_rng(w + h).random((1, h, w), dtype=np.float32)
     sim.write_field("state", s)
-    got = sim.read_field("conv0")
+    got = sim.read_field("potential0")
     want = O.conv_fft64(d, s, 0)
     err = np.abs(got - want).max()
     rms = np.sqrt(np.mean((got - want) ** 2))
@@ -79,7 +79,7 @@ def test_direct_multichannel(O):
     print(f"direct multichannel worst per-step max-abs {worst:.3e}")
     assert worst <= 1e-3
     for k in (0, 4, 8):
-        u = sim.read_field(f"conv{k}")
+        u = sim.read_field(f"potential{k}")
         assert np.abs(u - O.conv_fft64(d, s, k)).max() < 6e-4
 
 
@@ -121,7 +121,7 @@ def test_direct_three_term_conv_is_fp32_accurate(O, w, h, radius):
     sim = Sim(d, 1)
     s = np.random.default_rng(w).random((1, h, w), dtype=np.float32)
     sim.write_field("state", s)
-    err = np.abs(sim.read_field("conv0") - O.conv_fft64(d, s, 0)).max()
+    err = np.abs(sim.read_field("potential0") - O.conv_fft64(d, s, 0)).max()
     print(f"{w}x{h} R={radius} three-term conv max-abs {err:.3e}")
     assert err < 5e-6
 

@@ -39,9 +39,9 @@ def test_reference_known_answer(O):
 
 
 @pytest.mark.parametrize("w,h", [(16, 16), (32, 64), (64, 32), (128, 128), (256, 64), (512, 1024), (2048, 256),
-                                 (64, 4096), (2048, 4096), (8192, 64), (8192, 16)])
+                                 (64, 4096), (2048, 4096), (8192, 64), (8192, 16), (64, 8192), (256, 8192)])
 def test_conv_matches_fp64(O, w, h):
-    """The convolution alone (the reference's conv0 field) vs FP64, random state."""
+    """The convolution alone (the potential of the current state) vs FP64, random state."""
     radius = min(w, h) / 2 - 1.5
     d = W.desc(w, h, [dict(radius=min(radius, 13.0), beta=[1.0, 0.5], mu=0.2, sigma=0.03)], 0.1,
                init=INIT_ZERO)
@@ -49,7 +49,7 @@ def test_conv_matches_fp64(O, w, h):
     rng = np.random.default_rng(w * 7 + h)
     s = rng.random((1, h, w), dtype=np.float32)
     sim.write_field("state", s)
-    got = sim.read_field("conv0")
+    got = sim.read_field("potential0")
     want = O.conv_fft64(d, s, 0)
     err = np.abs(got - want).max()
     print(f"{w}x{h} conv max-abs {err:.3e}")
@@ -190,6 +190,7 @@ def test_steps_split_across_calls_and_conv_reads():
     for t in range(10):
         c.step(1)
         if t % 3 == 1:
+            c.read_field("potential0")
             c.read_field("conv0")
     sa = a.read_field("state")
     assert np.array_equal(sa, b.read_field("state"))
@@ -203,6 +204,8 @@ def test_field_errors():
         sim.read_field("nope")
     with pytest.raises(EngineError):
         sim.read_field("conv3")
+    with pytest.raises(EngineError):
+        sim.read_field("potential3")
     buf = np.zeros(4, np.float32)
     assert sim._lib.ah_sim_read_field(sim._h, b"state", buf.ctypes.data, buf.nbytes) == 5  # AH_ERR_SIZE
     assert b"bytes" in sim._lib.ah_last_error()
@@ -226,7 +229,7 @@ def test_profile_counts_sampled_launches():
 
 
 @pytest.mark.parametrize("w,h", [(4096, 64), (4096, 256), (8192, 128), (2048, 64), (1024, 256), (512, 128),
-                                 (256, 512), (128, 128), (128, 16)])
+                                 (256, 512), (128, 128), (128, 16), (64, 8192), (128, 8192)])
 def test_wide_world_steps_and_resume(O, w, h):
     """Worlds 128..4096 wide run the fused row pass (2048/M rows per
     workgroup), 8192-wide ones (the widest a descriptor allows) and worlds
@@ -253,3 +256,27 @@ def test_wide_world_steps_and_resume(O, w, h):
     a.step(6)
     b.step(6)
     assert np.array_equal(a.read_field("state"), b.read_field("state"))
+
+
+@pytest.mark.parametrize("path", ["fft_fp32", "direct_bf16"])
+def test_conv_field_is_the_last_steps_potential(path):
+    """conv<k> follows the reference's conv_i (spectral.rs:227-234): zero
+    before the first step, then the potential the last step used (that of
+    its input state), unchanged by a state write; potential<k> is the
+    current state's.  Both are the same computation on the same input, so
+    they agree bit for bit."""
+    d = W.rings3_4096(side=256, path=path)
+    sim = Sim(d, 3)
+    assert not sim.read_field("conv0").any()
+    p_before = sim.read_field("potential0")
+    assert p_before.any()
+    sim.step(1)
+    assert np.array_equal(sim.read_field("conv0"), p_before)
+    s1 = sim.read_field("state")
+    p1 = sim.read_field("potential0")
+    sim.write_field("state", np.zeros_like(s1))
+    assert np.array_equal(sim.read_field("conv0"), p_before)
+    sim.write_field("state", s1)
+    sim.step(1)
+    assert np.array_equal(sim.read_field("conv0"), p1)
+    assert not np.array_equal(p1, p_before)

@@ -54,7 +54,7 @@ def test_fft_step_commutes_with_translation(dy, dx):
 def _potential(d, s):
     sim = Sim(d, 1)
     sim.write_field("state", s)
-    return sim.read_field("conv0").reshape(d.height, d.width)
+    return sim.read_field("potential0").reshape(d.height, d.width)
 
 
 def test_fft_potential_is_linear():
@@ -93,3 +93,41 @@ def test_smoothlife_4096_step_commutes_with_translation():
     print(f"smoothlife 4096^2 shift: max-abs {err:.3e}, mean {base.mean():.4f}")
     assert err <= 1e-4
     assert 0 < base.mean() < 1
+
+
+def test_registry_lenia_8192_steps_and_commutes_with_translation():
+    """automata::build("lenia", 8192, 8192) (the reference benches Lenia at
+    8192^2, results/gpu.jsonl:82): the seeded world steps, stays in [0, 1],
+    and a step commutes with a translation across the wrap to the FP32 bar."""
+    from auto_amd import build
+    d = build("lenia", 8192, 8192)
+    assert (d.width, d.height) == (8192, 8192)
+    s = _seeded(d, 5)
+    assert s.any()
+    base = _step_from(d, s, 2)
+    assert np.isfinite(base).all() and base.min() >= 0 and base.max() <= 1
+    assert not np.array_equal(base, s)
+    moved = _step_from(d, np.roll(s, (4111, -2049), axis=(1, 2)), 2)
+    err = np.abs(moved - np.roll(base, (4111, -2049), axis=(1, 2))).max()
+    print(f"lenia 8192^2 shift: max-abs {err:.3e}, mass {base.sum(dtype=np.float64):.1f}")
+    assert err <= 1e-5
+
+
+def test_fft_potential_of_impulse_is_the_kernel_8192():
+    """At 8192^2 the potential of a unit impulse is the same normalised
+    kernel wherever the impulse sits."""
+    d = W.rings3_4096(side=8192)
+    k0 = None
+    for y, x in [(0, 0), (5000, 8191), (8191, 3)]:
+        s = np.zeros((1, d.height, d.width), np.float32)
+        s[0, y, x] = 1.0
+        u = np.roll(_potential(d, s), (-y, -x), axis=(0, 1))
+        if k0 is None:
+            k0 = u
+            assert abs(float(u.sum(dtype=np.float64)) - 1.0) < 1e-4
+            # The taps lie within the radius: nothing beyond |dx|, |dy| > 26.
+            far = np.ones_like(u, bool)
+            far[:27, :27] = far[:27, -26:] = far[-26:, :27] = far[-26:, -26:] = False
+            assert np.abs(u[far]).max() <= 1e-6
+        else:
+            assert np.abs(u - k0).max() <= 1e-6

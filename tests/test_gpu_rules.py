@@ -26,7 +26,7 @@ def test_seed_bit_exact(O, rule, seed):
     assert np.array_equal(Sim(d, seed).read_field("state"), O.init_state(d, seed))
 
 
-@pytest.mark.parametrize("w,h", [(256, 256), (4096, 128), (64, 4096)])
+@pytest.mark.parametrize("w,h", [(256, 256), (4096, 128), (64, 4096), (64, 8192)])
 def test_smoothlife_steps(O, w, h):
     d = W.smoothlife(w, h)
     sim = Sim(d, 5)
@@ -45,7 +45,7 @@ def test_smoothlife_steps(O, w, h):
     assert 0 < s.mean() < 1
 
 
-@pytest.mark.parametrize("w,h,rings", [(256, 256, None), (4096, 128, None), (64, 4096, None),
+@pytest.mark.parametrize("w,h,rings", [(256, 256, None), (4096, 128, None), (64, 4096, None), (64, 8192, None),
                                         (256, 256, (10, 15.5, 2, 6))])
 def test_mnca_steps(O, w, h, rings):
     d = W.mnca(w, h) if rings is None else W.mnca(w, h, rings=rings)
@@ -54,7 +54,7 @@ def test_mnca_steps(O, w, h, rings):
     flips = 0
     for _ in range(6):
         s = sim.read_field("state")
-        u = np.stack([sim.read_field("conv0"), sim.read_field("conv1")])
+        u = np.stack([sim.read_field("potential0"), sim.read_field("potential1")])
         u64 = O.potentials64(d, s, khats)
         assert np.abs(u - u64).max() < 1e-5
         sim.step(1)
