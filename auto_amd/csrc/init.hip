@@ -16,6 +16,8 @@
 //    from the PCG hash of program.rs:140-147;
 //  - statistics: FP64 mass, min, max and non-finite count per channel, with
 //    wavefront shuffles and a fixed-order second pass (bitwise reproducible).
+#include <mutex>
+
 #include "ah_internal.h"
 
 namespace ah {
@@ -430,3 +432,83 @@ void launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n4)
     AH_HIP(hipGetLastError());
 }
 }  // namespace ah
+
+// ------------------------------------------------------------ stats without allocation
+// ah_state_stats_noalloc: the reduction of ah_sim_stats over a state the
+// caller holds in device memory, for callers that cannot allocate.  Its
+// partial results live in `scratch`, a fixed device array (one per GPU,
+// held until the process ends): per channel NOALLOC_BLOCKS FP64 masses,
+// float minima and maxima and u64 non-finite counts.  Every partial the
+// final pass reads is written by the first pass of the same call, so
+// nothing needs clearing between calls; calls that overlap on one GPU share
+// the array and race.
+namespace ah {
+namespace {
+
+constexpr int NOALLOC_BLOCKS = 1024;  // = the largest block count stats_blocks() picks
+constexpr size_t NOALLOC_SLOTS = (size_t)AH_MAX_CHANNELS * NOALLOC_BLOCKS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SLOTS * (sizeof(double) + 2 * sizeof(float) + sizeof(unsigned long long));
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of this GPU's `scratch`, looked up once per device with
+// that device current.  The array's name in host code is only a stand-in:
+// kernels and copies must use this address.
+unsigned char* scratch_address() {
+    static std::mutex mu;
+    static unsigned char* addr[64] = {};
+    int dev = 0;
+    AH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error{AH_ERR_DEVICE, "device index out of range"};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        AH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error{AH_ERR_DEVICE, "scratch array has no device address"};
+        addr[dev] = static_cast<unsigned char*>(p);
+    }
+    return addr[dev];
+}
+
+}  // namespace
+}  // namespace ah
+
+extern "C" ah_status ah_state_stats_noalloc(const float* state, uint64_t cells, uint32_t channels, void* stream,
+                                            ah_stats* out) {
+    using namespace ah;
+    return guard([&] {
+        if (!state || !out) throw Error{AH_ERR_INVALID, "null argument"};
+        if (channels < 1 || channels > AH_MAX_CHANNELS)
+            throw Error{AH_ERR_INVALID, "channels must be 1..3 (the scratch array holds partials for at most 3)"};
+        if (cells < 4 || cells % 4) throw Error{AH_ERR_INVALID, "cells per channel must be a positive multiple of 4"};
+        if (reinterpret_cast<uintptr_t>(state) % 16) throw Error{AH_ERR_INVALID, "state must be 16-byte aligned"};
+        const int nb = stats_blocks((size_t)cells);
+        if ((size_t)nb * channels > NOALLOC_SLOTS) throw Error{AH_ERR_INVALID, "partials exceed the scratch array"};
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        unsigned char* base = scratch_address();
+        double* pm = reinterpret_cast<double*>(base);
+        unsigned long long* pnf = reinterpret_cast<unsigned long long*>(base + NOALLOC_SLOTS * sizeof(double));
+        float* pmin = reinterpret_cast<float*>(base + NOALLOC_SLOTS * (sizeof(double) + sizeof(unsigned long long)));
+        float* pmax = pmin + NOALLOC_SLOTS;
+        launch_stats(st, nullptr, state, (size_t)cells, (int)channels, pm, pmin, pmax, pnf, nb);
+        ah_stats r{};
+        double mass[AH_MAX_CHANNELS];
+        float lo[AH_MAX_CHANNELS], hi[AH_MAX_CHANNELS];
+        unsigned long long nf[AH_MAX_CHANNELS];
+        for (uint32_t c = 0; c < channels; ++c) {
+            const size_t o = (size_t)c * nb;  // the final pass leaves each channel's result in slot 0
+            AH_HIP(hipMemcpyAsync(&mass[c], pm + o, sizeof(double), hipMemcpyDeviceToHost, st));
+            AH_HIP(hipMemcpyAsync(&lo[c], pmin + o, sizeof(float), hipMemcpyDeviceToHost, st));
+            AH_HIP(hipMemcpyAsync(&hi[c], pmax + o, sizeof(float), hipMemcpyDeviceToHost, st));
+            AH_HIP(hipMemcpyAsync(&nf[c], pnf + o, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        }
+        AH_HIP(hipStreamSynchronize(st));
+        for (uint32_t c = 0; c < channels; ++c) {
+            r.mass[c] = mass[c];
+            r.min[c] = lo[c];
+            r.max[c] = hi[c];
+            r.nonfinite += nf[c];
+        }
+        *out = r;
+    });
+}

@@ -204,6 +204,23 @@ ah_status ah_sim_render(ah_sim* sim, uint32_t out_w, uint32_t out_h, float cente
 /* Mass/min/max/non-finite reduction of the current state (synchronises). */
 ah_status ah_sim_stats(ah_sim* sim, ah_stats* out);
 
+/*
+ * The reduction of ah_sim_stats, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc) and hold a state in device memory of their own.  `state` is a
+ * device pointer to float32 [channels][cells], 16-byte aligned; cells is a
+ * positive multiple of 4; channels 1..3 (AH_ERR_INVALID otherwise).  The
+ * partial results go to a fixed library-owned device array of 72 KiB per
+ * GPU (at most 1024 partials per channel, 3 channels), kept until the
+ * process ends, instead of a sim's buffers; the results equal ah_sim_stats
+ * on the same state bit for bit.  `stream` is a hipStream_t (NULL: the null
+ * stream); the call synchronises it.  Calls on one GPU share the array:
+ * calls that overlap on one GPU race.  out->generation is 0.
+ */
+ah_status ah_state_stats_noalloc(const float* state, uint64_t cells, uint32_t channels, void* stream,
+                                 ah_stats* out);
+
 ah_status ah_sim_info(const ah_sim* sim, ah_info* out);
 
 void ah_sim_destroy(ah_sim* sim);
