@@ -419,7 +419,7 @@ void setup(ah_batch* b) {
                 const int R = (int)std::ceil((double)k.radius);
                 launch_kernel_taps64(b->stream, nullptr, k, R, taps, sum);
                 launch_khat64(b->stream, nullptr, g.W, g.H, R, taps, cosW, cosH, T, g.khat + khat_n * c,
-                              g.kpack + np * c, false);
+                              g.kpack + np * c, KHAT_BLOCK);
                 kp[c] = KernelParams{k.mu, 1.0f / k.sigma, 1.0f, 0u, 0u};
                 dt[c] = d.dt;
             }

@@ -488,9 +488,11 @@ void setup(ah_sim* s) {
             bool alias = (K == 1 && C == 1);
             s->spec_k = alias ? s->spec_src : (float2*)s->alloc(sizeof(float2) * spec_elems * K);
             const size_t np = (size_t)H / 2 + 1;
-            s->khat = (float*)s->alloc(sizeof(float) * (size_t)(W / 8) * np * BC * K);
+            const int klayout = khat_layout(W, H, C, K, s->ksrc);
+            const size_t kfloats = khat_floats(W, H, klayout);
+            s->khat = (float*)s->alloc(sizeof(float) * kfloats * K);
             s->kpack = (float2*)s->alloc(sizeof(float2) * np * K);
-            AH_HIP(hipMemsetAsync(s->khat, 0, sizeof(float) * (size_t)(W / 8) * np * BC * K, st));
+            AH_HIP(hipMemsetAsync(s->khat, 0, sizeof(float) * kfloats * K, st));
             s->twW = (float2*)s->alloc(sizeof(float2) * W);
             s->twH = (float2*)s->alloc(sizeof(float2) * H);
             launch_twiddles(st, &s->prof, s->twW, W);
@@ -502,9 +504,8 @@ void setup(ah_sim* s) {
             for (int k = 0; k < K; ++k) {
                 const int side = 2 * s->reach[k] + 1;
                 double* T = (double*)tmp(sizeof(double) * side * (W / 2 + 1));
-                launch_khat64(st, &s->prof, W, H, s->reach[k], taps[k], cosW, cosH, T,
-                              s->khat + (size_t)k * (W / 8) * np * BC, s->kpack + (size_t)k * np,
-                              col_stream_ok(W, H, C, K, s->ksrc));
+                launch_khat64(st, &s->prof, W, H, s->reach[k], taps[k], cosW, cosH, T, s->khat + (size_t)k * kfloats,
+                              s->kpack + (size_t)k * np, klayout);
             }
         } else {
             setup_direct(s, taps);

@@ -605,6 +605,258 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
     if constexpr (NK == 1) write_column(done, a.spec_out, col0 + (NC - 1) * G);
 }
 
+// Column pass for H = 4096 with one channel read by one kernel, one
+// workgroup per 4-column block of the half spectrum.  The block is one
+// contiguous run of 4096 rows x 32 bytes; its 16384 complex values stay in
+// registers, 16 per thread over 1024 threads, and pass through LDS only to
+// change hands between radix-16 stages.  Every global access of a wave is
+// 16 whole rows of the block, 512 contiguous bytes: each L2 request carries
+// only bytes this workgroup uses (a column-per-workgroup kernel uses 8 of
+// each 32-byte row and leaves the rest of the line to three other
+// workgroups).
+//
+// The length-4096 transform as 16 x 16 x 16: n = n2 + 256 n1 (n2 < 256),
+// k = k1 + 16 k2; X[k] = sum_n2 W256^(n2 k2) W4096^(n2 k1) sum_n1 x[n] W16^(n1 k1),
+// and the 256-point inner transforms again as n2 = m + 16 n1', k2 = k2a + 16 k2b.
+//   phase 1, thread (c, t = n2): loads x[t + 256 j], j < 16; DFT-16 over j -> k1;
+//            twiddle W4096^(t k1).  Exchange 1 through LDS (workgroup barrier).
+//   phase 2, thread (c, k1, m): n1' < 16; DFT-16 -> k2a; twiddle W256^(m k2a).
+//            Exchange 2: a 16 x 16 transpose among the 16 lanes that share
+//            (c, k1), all in one wave: wave-local LDS, no barrier.
+//   phase 3, thread (c, k1, k2a): DFT-16 over m -> k2b: the thread holds
+//            frequencies k = k1 + 16 k2a + 256 k2b, multiplies by K-hat and
+//            runs the first inverse DFT-16 in registers.
+// The inverse retraces these steps with conjugate twiddles (the transform
+// is unnormalised; 1/(W*H) is in K-hat) and stores phase 1's rows back.
+namespace c4k {
+constexpr int H = 4096;
+constexpr int KS = 272;        // LDS float2 per (c, k1): 256 + 16, so two k1 land on distinct bank halves
+constexpr int NP = H / 2 + 1;  // K-hat entries per column (real, even in ky)
+// NCOL columns of a block per workgroup, 256 threads per column.  The LDS
+// stride of a column keeps the NCOL columns of one row on distinct banks.
+template <int NCOL>
+struct Plan {
+    static constexpr int THREADS = 256 * NCOL;
+    static constexpr int CS = 16 * KS + (NCOL == 4 ? 8 : 16);
+    static constexpr size_t LDS_BYTES = (size_t)NCOL * CS * sizeof(float2);
+};
+}  // namespace c4k
+
+// X[c + 4d] = sum_a W4^(ad) W16^(ac) sum_b x[a + 4b] W4^(bc), W_n = exp(SIGN 2 pi i / n).
+template <int SIGN>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+    const float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+    float2 y[16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        float2 q[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+        dft4<SIGN>(q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[4 * a + c] = q[c];
+    }
+    // W16^(ac) for a, c in 1..3: ac = 1, 2, 3, 2, 4, 6, 3, 6, 9.
+    y[5] = cmul(y[5], make_float2(c1, SIGN * s1));     // a=1 c=1
+    y[6] = cmul(y[6], make_float2(h, SIGN * h));       // a=1 c=2
+    y[7] = cmul(y[7], make_float2(s1, SIGN * c1));     // a=1 c=3
+    y[9] = cmul(y[9], make_float2(h, SIGN * h));       // a=2 c=1
+    y[10] = mul_si<SIGN>(y[10]);                       // a=2 c=2
+    y[11] = cmul(y[11], make_float2(-h, SIGN * h));    // a=2 c=3
+    y[13] = cmul(y[13], make_float2(s1, SIGN * c1));   // a=3 c=1
+    y[14] = cmul(y[14], make_float2(-h, SIGN * h));    // a=3 c=2
+    y[15] = cmul(y[15], make_float2(-c1, -SIGN * s1)); // a=3 c=3
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float2 q[4] = {y[c], y[4 + c], y[8 + c], y[12 + c]};
+        dft4<SIGN>(q);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) v[c + 4 * d] = q[d];
+    }
+}
+
+// v[r] *= w^r, r = 1..15, w = w1 (conjugated for SIGN > 0); each power is at
+// most four products deep from w1.
+template <int SIGN>
+__device__ __forceinline__ void twiddle16(float2 (&v)[16], float2 w1) {
+    if (SIGN > 0) w1.y = -w1.y;
+    float2 w[16];
+    w[1] = w1;
+    w[2] = cmul(w1, w1);
+    w[3] = cmul(w[2], w1);
+    w[4] = cmul(w[2], w[2]);
+    w[5] = cmul(w[4], w1);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+    w[8] = cmul(w[4], w[4]);
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], w[r]);
+}
+
+__device__ __forceinline__ void wave_lds_order() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// A thread's roles: phase 1, column c1 of the workgroup's columns and row
+// t1 (with t1 + 256 j); phases 2 and 3, column c2, k1 and m (phase 2) or
+// k2a (phase 3).  `region` is (c2, k1)'s LDS, owned by 16 lanes of one wave.
+template <int NCOL>
+struct ColRoles {
+    static constexpr int LGC = NCOL == 4 ? 2 : NCOL == 2 ? 1 : 0;
+    static_assert(NCOL == 1 || NCOL == 2 || NCOL == 4, "columns per workgroup");
+    int c1, t1, c2, k1, m;
+    float2* region;
+    __device__ ColRoles(float2* lds) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        c1 = lane & (NCOL - 1);
+        t1 = (lane >> LGC) + (64 >> LGC) * wave;
+        c2 = wave >> 2;
+        k1 = ((lane >> 4) & 3) + 4 * (wave & 3);
+        m = lane & 15;
+        region = lds + c2 * c4k::Plan<NCOL>::CS + k1 * c4k::KS;
+    }
+};
+
+// Logical task L: columns cbase.. of block kb (the NCOL-column tasks of one
+// block are consecutive).
+template <int NCOL>
+__device__ __forceinline__ void col_task_id(int L, int& kb, int& cbase) {
+    kb = L >> (2 - ColRoles<NCOL>::LGC);
+    cbase = (L << ColRoles<NCOL>::LGC) & 3;
+}
+
+// A task's K-hat factors for its phase-3 frequencies k = k1 + 16 m + 256 r
+// (none for the packed column 0), and its 16 phase-1 values.
+template <int NCOL>
+__device__ __forceinline__ void col_task_load_khat(const ColArgs& a, int kb, int cbase, const ColRoles<NCOL>& q,
+                                                   float (&kf)[16]) {
+    using namespace c4k;
+    const int col = kb * BC + cbase + q.c2;
+    if (col != 0) {
+        // K-hat in digit order (KHAT_DIGITS), real and even in ky: row k
+        // takes p = min(k, H - k).  For r < 8, p = k sits at 256 r + 16 k1 + m
+        // (a wave reads 256 contiguous bytes); for r >= 8, p = H - k has the
+        // digits (16 - k1, 15 - m, 15 - r) (with borrows when k1 or m is 0),
+        // which sit at digits(H - k at r = 8) - 256 (r - 8).
+        const float* kh = a.khat + (size_t)col * NP;
+        const int lo = 16 * q.k1 + q.m;
+        const int p8 = H / 2 - q.k1 - 16 * q.m;
+        const int hi = (p8 >> 8) * 256 + (p8 & 15) * 16 + ((p8 >> 4) & 15);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) kf[r] = kh[r < 8 ? 256 * r + lo : hi - 256 * (r - 8)];
+    }
+}
+
+template <int NCOL>
+__device__ __forceinline__ void col_task_load_rows(const ColArgs& a, int kb, int cbase, const ColRoles<NCOL>& q,
+                                                   float2 (&v)[16]) {
+    const float2* src = a.spec_in + (size_t)kb * c4k::H * BC + cbase;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = src[(size_t)(q.t1 + 256 * j) * BC + q.c1];
+}
+
+// Forward transform, K-hat multiply, inverse transform and store of one task
+// whose phase-1 values are in v.  w_t1 = W4096^t1, w_m = W256^m.  `between`
+// runs once phase 1's values have gone to LDS (their registers are free).
+template <int NCOL, typename F>
+__device__ __forceinline__ void col_task_run(const ColArgs& a, float2* lds, int kb, int cbase, const ColRoles<NCOL>& q,
+                                             float2 (&v)[16], const float (&kf)[16], float2 w_t1, float2 w_m,
+                                             F&& between) {
+    using namespace c4k;
+    constexpr int CS = Plan<NCOL>::CS;
+    const int c1 = q.c1, t1 = q.t1, c2 = q.c2, k1 = q.k1, m = q.m;
+    float2* region = q.region;
+    // Phase 1.
+    dft16<-1>(v);
+    twiddle16<-1>(v, w_t1);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lds[c1 * CS + r * KS + t1] = v[r];
+    between();
+    lds_sync();
+    // Phase 2.
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = region[m + 16 * r];
+    dft16<-1>(v);
+    twiddle16<-1>(v, w_m);
+    wave_lds_order();  // this wave's phase-2 reads of `region` are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) region[17 * r + m] = v[r];
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = region[17 * m + r];
+    // Phase 3: k = k1 + 16 m + 256 r.
+    dft16<-1>(v);
+    const int col = kb * BC + cbase + c2;
+    if (kb == 0 && cbase == 0) {
+        // Workgroup-uniform: column 0 packs the real bins kx = 0 and W/2.
+        // D[k] = A C[k] + B conj C[-k] with (A, B) = kpack[min(k, H - k)];
+        // C[-k] is held by another thread, so column 0 goes through LDS in
+        // natural order (its four waves' regions hold 4352 >= 4096 entries).
+        lds_sync();  // column 0's waves have read their exchange-2 data
+        if (c2 == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) lds[k1 + 16 * m + 256 * r] = v[r];
+        }
+        lds_sync();
+        if (c2 == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int k = k1 + 16 * m + 256 * r, kn = (H - k) & (H - 1);
+                const float2 cn = lds[kn], ab = a.kpack[min(k, kn)];
+                v[r] = cadd(make_float2(ab.x * v[r].x, ab.x * v[r].y), make_float2(ab.y * cn.x, -ab.y * cn.y));
+            }
+        }
+        lds_sync();  // partner reads are done before the region is reused
+    }
+    if (col != 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = make_float2(kf[r] * v[r].x, kf[r] * v[r].y);
+    }
+    // Inverse: phase 3, exchange 2, phase 2, exchange 1, phase 1.
+    dft16<1>(v);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) region[17 * m + r] = v[r];
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = region[17 * r + m];
+    twiddle16<1>(v, w_m);
+    dft16<1>(v);
+    wave_lds_order();  // the transpose reads are done before the region is rewritten
+#pragma unroll
+    for (int r = 0; r < 16; ++r) region[m + 16 * r] = v[r];
+    lds_sync();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = lds[c1 * CS + r * KS + t1];
+    twiddle16<1>(v, w_t1);
+    dft16<1>(v);
+}
+
+// A task's results (rows t1 + 256 j of column c1, in v) back to the spectrum.
+template <int NCOL>
+__device__ __forceinline__ void col_task_store(const ColArgs& a, int kb, int cbase, const ColRoles<NCOL>& q,
+                                               const float2 (&v)[16]) {
+    float2* dst = a.spec_out + (size_t)kb * c4k::H * BC + cbase;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) dst[(size_t)(q.t1 + 256 * j) * BC + q.c1] = v[j];
+}
+
+// One task per workgroup (grid W/2/NCOL).  Workgroups of one block are
+// consecutive logical ids, kept on one XCD.
+template <int NCOL>
+__global__ void __launch_bounds__(c4k::Plan<NCOL>::THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_col_block4096(ColArgs a) {
+    extern __shared__ float2 lds[];
+    int kb, cbase;
+    col_task_id<NCOL>(xcd_remap(blockIdx.x, gridDim.x), kb, cbase);
+    const ColRoles<NCOL> q(lds);
+    float2 v[16];
+    float kf[16];
+    col_task_load_rows<NCOL>(a, kb, cbase, q, v);
+    // The K-hat loads go out once phase 1's values are in LDS, behind the
+    // rows rather than beside them (about 1 us less at 4096^2).
+    col_task_run<NCOL>(a, lds, kb, cbase, q, v, kf, a.twH[q.t1], a.twH[16 * q.m],
+                       [&] { col_task_load_khat<NCOL>(a, kb, cbase, q, kf); });
+    col_task_store<NCOL>(a, kb, cbase, q, v);
+}
+
 // Fused column pass for batched same-size worlds (diagonal mode: world c's
 // kernel reads and writes channel c), H = 64..2048.  A workgroup takes CPW
 // columns of one block (all four for H <= 1024) with TPC = H/8 threads per
@@ -1125,6 +1377,22 @@ int row_rows(int W, int H) {
 
 }  // namespace
 
+// One channel, one kernel, H = 4096: the block column kernel.
+bool col_block_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
+    return channels == 1 && n_kernels == 1 && ksrc[0] == 0 && H == c4k::H && W >= 64;
+}
+
+int khat_layout(int W, int H, int channels, int n_kernels, const int* ksrc) {
+    if (col_block_ok(W, H, channels, n_kernels, ksrc)) return KHAT_DIGITS;
+    if (col_stream_ok(W, H, channels, n_kernels, ksrc)) return KHAT_COLUMN;
+    return KHAT_BLOCK;
+}
+
+size_t khat_floats(int W, int H, int layout) {
+    (void)layout;  // every layout holds p in [0, H/2] for each of the W/2 columns
+    return (size_t)(W / 2) * (H / 2 + 1);
+}
+
 bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
     return channels == 1 && (n_kernels == 1 || (n_kernels == 2 && ksrc[1] == 0)) && ksrc[0] == 0 && H == SH &&
            W >= 64;
@@ -1201,6 +1469,16 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.diag = diag ? 1 : 0;
     a.n_kernels = diag ? channels : n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = (!diag && k < n_kernels) ? ksrc[k] : -1;
+    if (!diag && col_block_ok(W, H, channels, n_kernels, ksrc)) {
+        // Two columns per workgroup (68 KiB of LDS): two workgroups share a
+        // CU, so one's loads overlap the other's arithmetic.
+        using P = c4k::Plan<2>;
+        Launch L(p, s, "col_conv");
+        set_lds(k_col_block4096<2>, P::LDS_BYTES);
+        hipLaunchKernelGGL(k_col_block4096<2>, dim3(W / 4), dim3(P::THREADS), P::LDS_BYTES, s, a);
+        AH_HIP(hipGetLastError());
+        return;
+    }
     if (!diag && col_stream_ok(W, H, channels, n_kernels, ksrc)) {
         // Up to four columns per workgroup, aiming at two workgroups per CU
         // (each holds 2 x 33 KiB of LDS).

@@ -144,10 +144,10 @@ __global__ void k_khat_rows64(int W, int reach, const double* taps, const double
 }
 
 // khat[kb][p][c] = (sum_dy T[dy][kx] cos(2 pi p dy / H)) / (W*H), p in [0, H/2];
-// kx = 0 and kx = W/2 go to kpack[p] = ((K0+KN)/2, (K0-KN)/2).  col_major
-// stores khat[kx][p] instead, for column kernels that own one column.
+// kx = 0 and kx = W/2 go to kpack[p] = ((K0+KN)/2, (K0-KN)/2).  `layout`
+// KHAT_COLUMN stores khat[kx][p], KHAT_DIGITS khat[kx][digits(p)].
 __global__ void k_khat64(int W, int H, int reach, const double* T, const double* cosH, float* khat, float2* kpack,
-                         int col_major) {
+                         int layout) {
     const int nk = W / 2 + 1, np = H / 2 + 1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nk * np) return;
@@ -168,8 +168,12 @@ __global__ void k_khat64(int W, int H, int reach, const double* T, const double*
         kpack[p] = make_float2((float)(0.5 * (acc + accN)), (float)(0.5 * (acc - accN)));
         return;
     }
-    if (col_major) {
+    if (layout == KHAT_COLUMN) {
         khat[(size_t)kx * np + p] = (float)acc;
+        return;
+    }
+    if (layout == KHAT_DIGITS) {
+        khat[(size_t)kx * np + (p >> 8) * 256 + (p & 15) * 16 + ((p >> 4) & 15)] = (float)acc;
         return;
     }
     int kb = kx >> 2, c = kx & 3;
@@ -362,7 +366,7 @@ void launch_kernel_taps64(hipStream_t s, Profiler* p, const ah_kernel& k, int re
 }
 
 void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const double* taps, const double* cosW,
-                   const double* cosH, double* T, float* khat, float2* kpack, bool col_major) {
+                   const double* cosH, double* T, float* khat, float2* kpack, int layout) {
     const int side = 2 * reach + 1, nk = W / 2 + 1, np = H / 2 + 1;
     {
         Launch L(p, s, "khat_rows64");
@@ -374,7 +378,7 @@ void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const do
         Launch L(p, s, "khat64");
         int n = nk * np;
         hipLaunchKernelGGL(k_khat64, dim3((n + 255) / 256), dim3(256), 0, s, W, H, reach, T, cosH, khat, kpack,
-                           (int)col_major);
+                           layout);
         AH_HIP(hipGetLastError());
     }
 }

@@ -67,8 +67,10 @@ FFT_KERNELS = ("row_r2c", "col_conv", "row_c2r_update", "row_c2r_update_r2c")
 
 
 def khat_bytes(w: int, h: int) -> int:
-    """K-hat bytes one col_conv launch reads: [W/8][H/2+1][4] float + packed column."""
-    return (w // 8) * (h // 2 + 1) * 4 * 4 + (h // 2 + 1) * 8
+    """K-hat bytes one col_conv launch reads: h/2+1 floats for each of the
+    w/2 columns (the spectrum is real and even in ky), plus the packed
+    column's h/2+1 float2."""
+    return (w // 2) * (h // 2 + 1) * 4 + (h // 2 + 1) * 8
 
 
 def kernel_bytes(name: str, w: int, h: int) -> float:
