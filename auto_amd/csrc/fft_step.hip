@@ -373,14 +373,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_col_conv(ColArgs a) {
     }
 }
 
-// Streamed column pass for H = 4096 with one channel read by one kernel.  A
-// persistent workgroup of 512 threads walks columns L, L+G, L+2G, ... of the
-// half spectrum with two LDS buffers: while it transforms, multiplies and
-// writes back the column in one buffer, the next column arrives in the other
-// via global_load_lds (each lane's dword goes to LDS without a register), so
-// the load of one column overlaps the arithmetic of the previous one.
-// Everything between the waits is LDS work or loads issued before the
-// prefetch, so the prefetch is waited for only at the top of the next column.
+// Column pass for H = 4096 with one channel read by two kernels (SmoothLife,
+// MNCA): one column per workgroup of 512 threads, loaded into LDS by
+// global_load_lds (each lane's dword goes to LDS without a register), radix-8
+// stages in LDS.
 constexpr int SH = 4096, ST = SH / 8;  // column length, threads
 
 // Rows 32r..32r+31 of column (kb, c) into the padded run at 33r: one wave
@@ -494,30 +490,16 @@ __device__ __forceinline__ void store_column(const float2* buf, float2* __restri
     }
 }
 
-// The same store in two halves: rows t + 512 i into registers, and later
-// from the registers to column `col` of dst.
-__device__ __forceinline__ void read_column(const float2* buf, float2 (&v)[SH / ST]) {
-#pragma unroll
-    for (int i = 0; i < SH / ST; ++i) v[i] = buf[pad(threadIdx.x + i * ST)];
-}
-
-__device__ __forceinline__ void write_column(const float2 (&v)[SH / ST], float2* __restrict__ dst, int col) {
-    float2* d = dst + col_index(col >> 2, col & 3, 0, SH);
-#pragma unroll
-    for (int i = 0; i < SH / ST; ++i) d[(size_t)(threadIdx.x + i * ST) * BC] = v[i];
-}
-
 // Forward FFT, multiply, inverse FFT and write-back of one column, for each
-// of the NK kernels reading it (1, or 2 for SmoothLife and MNCA: the
-// transformed column stays in registers between them).  After the last
-// forward stage thread t holds rows t + 512 r, which are exactly the rows the
-// first inverse stage (span 1) reads: the multiply and that stage run in
+// of the NK kernels reading it (2 for SmoothLife and MNCA: the transformed
+// column stays in registers between them).  After the last forward stage
+// thread t holds rows t + 512 r, which are exactly the rows the first
+// inverse stage (span 1) reads: the multiply and that stage run in
 // registers, saving one LDS round trip and two barriers.  kf[j] holds kernel
-// j's factors for these rows.  DEFER (one kernel): the result goes to `out`
-// instead of to memory, for the caller to store later.
-template <int NK, bool DEFER>
+// j's factors for these rows.
+template <int NK>
 __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const float2 (&tw)[3],
-                                            const float (&kf)[NK][8], int col, float2 (&out)[SH / ST]) {
+                                            const float (&kf)[NK][8], int col) {
     column_stage<-1>(buf, 0, tw);
     column_stage<-1>(buf, 1, tw);
     column_stage<-1>(buf, 2, tw);
@@ -546,30 +528,17 @@ __device__ __forceinline__ void conv_column(float2* buf, const ColArgs& a, const
         column_stage<1>(buf, 1, tw);
         column_stage<1>(buf, 2, tw);
         column_stage<1>(buf, 3, tw);
-        if constexpr (DEFER) read_column(buf, out);
-        else store_column(buf, a.spec_out + j * chan, col);
+        store_column(buf, a.spec_out + j * chan, col);
     }
 }
 
-// NC columns per workgroup, G = W/2/NC workgroups: column i of workgroup L
-// is L + i*G.  Fully unrolled, so each step names its buffers statically and
-// the compiler's wait tracking can tell the prefetch target from the buffer
-// being transformed.  One kernel: K-hat for column i is loaded with column
-// i's prefetch, during column i-1; two kernels: at the start of column i
-// (registers are too few to hold both a column ahead).
-//
-// One kernel: column i's result is read from LDS into registers at its end
-// and stored only after the wait at the top of column i+1.  The wait counter
-// covers loads and stores alike, so a store issued before that wait would
-// hold up the next column until it had completed; issued after it, the
-// stores drain while column i+1 is transformed, and the wait at the top of
-// column i+2 finds them done.
-template <int NC, int NK>
+// One column per workgroup (G = W/2), NK kernels reading it.  The column
+// arrives in LDS by DMA (prefetch_column); both kernels' K-hat factors are
+// loaded at the start.
+template <int NK>
 __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4))) k_col_stream(ColArgs a) {
-    __shared__ float2 bufA[padded(SH)];
-    __shared__ float2 bufB[padded(SH)];
-    const int G = gridDim.x;
-    const int col0 = xcd_remap(blockIdx.x, G);
+    __shared__ float2 buf[padded(SH)];
+    const int col = xcd_remap(blockIdx.x, gridDim.x);
     float2 tw[3];
 #pragma unroll
     for (int s = 1; s < 4; ++s) {
@@ -577,32 +546,12 @@ __global__ void __launch_bounds__(ST) __attribute__((amdgpu_waves_per_eu(4, 4)))
         tw[s - 1] = a.twH[(threadIdx.x & (Ls - 1)) * (SH / (8 * Ls))];
     }
     const size_t kstride = (size_t)(a.W >> 1) * (SH / 2 + 1);  // one kernel's column-major K-hat
-    float kf[2][NK][8];
-    float2 done[SH / ST];  // one kernel: the previous column's result, not yet stored
-    if constexpr (NK == 1) load_khat(kf[0][0], a.khat, col0);
-    prefetch_column(bufA, a.spec_in, col0);
+    float kf[NK][8];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        float2* cur = (i & 1) ? bufB : bufA;
-        float2* nxt = (i & 1) ? bufA : bufB;
-        const int col = col0 + i * G;
-        if constexpr (NK > 1) {
-#pragma unroll
-            for (int j = 0; j < NK; ++j) load_khat(kf[i & 1][j], a.khat + j * kstride, col);
-        }
-        // Also waits for this wave's LDS reads of the previous column's
-        // result in nxt, before any wave's prefetch overwrites it.
-        prefetch_sync();
-        if (i + 1 < NC) {
-            if constexpr (NK == 1) load_khat(kf[(i + 1) & 1][0], a.khat, col + G);
-            prefetch_column(nxt, a.spec_in, col + G);
-        }
-        if constexpr (NK == 1) {
-            if (i > 0) write_column(done, a.spec_out, col - G);
-        }
-        conv_column<NK, NK == 1>(cur, a, tw, kf[i & 1], col, done);
-    }
-    if constexpr (NK == 1) write_column(done, a.spec_out, col0 + (NC - 1) * G);
+    for (int j = 0; j < NK; ++j) load_khat(kf[j], a.khat + j * kstride, col);
+    prefetch_column(buf, a.spec_in, col);
+    prefetch_sync();
+    conv_column<NK>(buf, a, tw, kf, col);
 }
 
 // Column pass for H = 4096 with one channel read by one kernel, one
@@ -1393,9 +1342,9 @@ size_t khat_floats(int W, int H, int layout) {
     return (size_t)(W / 2) * (H / 2 + 1);
 }
 
+// One channel read by two kernels at H = 4096: the streamed column kernel.
 bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc) {
-    return channels == 1 && (n_kernels == 1 || (n_kernels == 2 && ksrc[1] == 0)) && ksrc[0] == 0 && H == SH &&
-           W >= 64;
+    return channels == 1 && n_kernels == 2 && ksrc[0] == 0 && ksrc[1] == 0 && H == SH && W >= 64;
 }
 
 // Compute units of the current device (cached per device; sims on several
@@ -1480,28 +1429,13 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
         return;
     }
     if (!diag && col_stream_ok(W, H, channels, n_kernels, ksrc)) {
-        // Up to four columns per workgroup, aiming at two workgroups per CU
-        // (each holds 2 x 33 KiB of LDS).
-        // Two kernels hold twice the K-hat factors and the transformed column
-        // besides: one column per workgroup keeps them in registers.
-        const int ncols = W / 2, slots = 2 * cu_count();
-        int nc = 1;
-        while (n_kernels == 1 && nc < 4 && ncols / (2 * nc) >= slots) nc *= 2;
-        const int G = ncols / nc;
+        // Two kernels (SmoothLife, MNCA) at H = 4096: one column per
+        // workgroup, both kernels reading the transformed column from
+        // registers.
+        const int G = W / 2;
         a.cw = 1;
         Launch L(p, s, "col_conv");
-        auto go_stream = [&](auto nk_tag) {
-            constexpr int NK = decltype(nk_tag)::value;
-            if constexpr (NK == 1) {
-                if (nc == 4) hipLaunchKernelGGL((k_col_stream<4, 1>), dim3(G), dim3(ST), 0, s, a);
-                else if (nc == 2) hipLaunchKernelGGL((k_col_stream<2, 1>), dim3(G), dim3(ST), 0, s, a);
-                else hipLaunchKernelGGL((k_col_stream<1, 1>), dim3(G), dim3(ST), 0, s, a);
-            } else {
-                hipLaunchKernelGGL((k_col_stream<1, NK>), dim3(G), dim3(ST), 0, s, a);
-            }
-        };
-        if (n_kernels == 2) go_stream(std::integral_constant<int, 2>{});
-        else go_stream(std::integral_constant<int, 1>{});
+        hipLaunchKernelGGL(k_col_stream<2>, dim3(G), dim3(ST), 0, s, a);
         AH_HIP(hipGetLastError());
         return;
     }
