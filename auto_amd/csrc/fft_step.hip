@@ -3,23 +3,31 @@
 // The reference runs Lenia as 4*log2(N)+5 dispatches per step, one radix-2
 // Stockham stage each, ping-ponging a full complex field through memory
 // (crates/auto-core/src/automata/spectral.rs:14-64 fft_pass, :199-264 step).
-// Here a step is three passes, each holding whole transforms in LDS:
+// Here a step is two passes (a third, row_r2c, runs only after the state was
+// written from outside), each holding whole transforms on chip:
 //
 //   1. row_r2c   state rows -> real FFT along x (length-W/2 complex FFT plus
 //                the even/odd split) -> half spectrum, 4 rows per workgroup so
 //                every 32-byte spectrum block gets a 128-byte contiguous write.
-//   2. col_conv  one 4-column block of the half spectrum, all H rows:
-//                forward FFT along y, multiply by the real kernel spectrum
-//                (scaled by 1/(W*H)), inverse FFT along y, written in place.
+//   2. col_conv  columns of the half spectrum, all H rows: forward FFT along
+//                y, multiply by the real kernel spectrum (scaled by 1/(W*H)),
+//                inverse FFT along y, written in place.
 //   3. row_c2r   inverse real FFT along x, then the growth map, dt and clip
-//                (spectral.rs:236-264,337-345) written straight to the state.
+//                (spectral.rs:236-264,337-345) written to the state, and (fused
+//                variants) the next step's forward row FFT.
 //
 // Inside LDS the transforms are Stockham autosort stages of radix 8 (with a
 // final radix 2 or 4 stage), the same stage recurrence as fft_pass
 // generalised to radix R: thread j, k = j mod Ls, loads v[r] = x[j + r*M/R],
 // twiddles by w^(r*k) with w = exp(sign*2*pi*i/(Ls*R)), runs a radix-R DFT in
-// registers and stores at (j-k)*R + k + r*Ls.  Twiddles come from a table
-// generated in FP64 (init.hip), not from per-thread f32 cos/sin.
+// registers and stores at (j-k)*R + k + r*Ls.  The column kernel at H = 4096
+// runs radix-16 stages in registers instead (k_col_block4096).
+//
+// Twiddle precision: the LDS stages (twiddles()) take the hardware's f32
+// sine and cosine of the exact dyadic angle k*step/n turns; the column
+// kernels at H = 4096 and the real-FFT twist factors read FP64-built tables
+// (init.hip) and form powers by products.  One-step error vs FP64 at 4096^2
+// stays near 2e-6 either way (bar: 1e-3).
 //
 // Half-spectrum layout (complex64): spec[kb][y][c], kx = 4*kb + c.  Bin kx=0
 // packs the two real bins X[0] and X[W/2] as (re, im); the column pass treats
@@ -105,11 +113,12 @@ __device__ __forceinline__ void dft(float2* v) {
     else dft8<SIGN>(v);
 }
 
-// Twiddles w^r, r = 1..R-1, with w = tw[k*tstep] (conjugated for SIGN > 0):
-// one table load and products (w^2 = w w, w^4 = w^2 w^2, ...; a few ulp).
-// The lookups are scattered, so each one is an L2 request of its own, and
-// the passes are bound by L2 requests: three loads per stage instead of
-// one cost the 4096^2 row pass several microseconds.
+// Twiddles w^r, r = 1..R-1, w = exp(SIGN 2 pi i k tstep / nt): w from the
+// hardware sine and cosine (which take turns) of the exact angle
+// k*tstep/nt, powers by products (w^2 = w w, w^4 = w^2 w^2, ...; a few ulp).
+// A table lookup here would be a scattered L2 request per butterfly, and
+// these passes are bound by L2 requests.  `tw` is unused: the callers keep
+// one signature for table-based and computed twiddles.
 template <int R, int SIGN>
 __device__ __forceinline__ void twiddles(float2* w, const float2* __restrict__ tw, int k, int tstep, int nt) {
     (void)tw;

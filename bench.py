@@ -55,9 +55,9 @@ EVENT_SAMPLES = 256
 # A bench run cannot collect counters itself, so `roofline.traffic` reports
 # this recorded figure for that world and null for any other size.
 RECORDED_TRAFFIC = {
-    ("row_c2r_update_r2c", 4096, 4096): 273239053.4,
-    ("col_conv", 4096, 4096): 153659249.4,
-    ("direct_bf16_update", 4096, 4096): 169424859.8,
+    ("row_c2r_update_r2c", 4096, 4096): 275952558.0,
+    ("col_conv", 4096, 4096): 151500206.0,
+    ("direct_bf16_update", 4096, 4096): 168327361.0,
 }
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, recorded in profiles/"
 
