@@ -32,19 +32,26 @@ def _clean(stderr: str) -> str:
     return "\n".join(l for l in stderr.splitlines() if not any(n in l for n in NOISE))
 
 
-def _compile(src: str, obj: str, defines) -> tuple[int, str, list]:
-    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-c", "-o", obj, src]
+# Host-side AddressSanitizer (device code is never instrumented): for the
+# CPU sanitizer test, which loads such a build with the sanitizer runtime
+# preloaded.
+ASAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def _compile(src: str, obj: str, defines, extra=()) -> tuple[int, str, list]:
+    cmd = [HIPCC, *FLAGS, *extra, *[f"-D{d}" for d in defines], "-c", "-o", obj, src]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return r.returncode, _clean(r.stderr), cmd
 
 
-def build(verbose: bool = False, out: str = OUT, defines=(), obj_dir: str = OBJ_DIR) -> str:
+def build(verbose: bool = False, out: str = OUT, defines=(), obj_dir: str = OBJ_DIR, asan: bool = False) -> str:
+    extra = ASAN_HOST if asan else []
     os.makedirs(obj_dir, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     objs = [os.path.join(obj_dir, s + ".o") for s in SOURCES]
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda so: _compile(so[0], so[1], defines), zip(srcs, objs)))
+        results = list(ex.map(lambda so: _compile(so[0], so[1], defines, extra), zip(srcs, objs)))
     for (rc, err, cmd), src in zip(results, srcs):
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -53,7 +60,7 @@ def build(verbose: bool = False, out: str = OUT, defines=(), obj_dir: str = OBJ_
         if verbose and err.strip():
             print(err, file=sys.stderr)
     tmp = out + ".tmp"
-    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *extra, "-o", tmp, *objs]
     if verbose:
         print(" ".join(link), flush=True)
     r = subprocess.run(link, capture_output=True, text=True)

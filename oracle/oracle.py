@@ -19,7 +19,8 @@ import numpy as np
 from auto_amd._abi import LeniaDesc, Kernel, Stats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liblenia_oracle.so")
+# LENIA_ORACLE_LIB selects another build of the same source (the sanitizer test's).
+LIB = os.environ.get("LENIA_ORACLE_LIB", os.path.join(HERE, "liblenia_oracle.so"))
 
 _F = ctypes.POINTER(ctypes.c_float)
 _D = ctypes.POINTER(ctypes.c_double)
