@@ -50,7 +50,9 @@ struct Work {
 };
 
 constexpr int WG_CHUNKS = 64;  // pixel chunks of the weight-gradient reductions
-constexpr size_t PART_FLOATS = (size_t)WG_CHUNKS * 64 * 25 * 64;
+// Partials: the VALU reduction's chunks, or the MFMA reduction's 256
+// workgroups of one 64 x 25 x 32 weight block each.
+constexpr size_t PART_FLOATS = (size_t)256 * 64 * 25 * 32;
 
 // More pixel chunks for the small layers so the reduction still fills the chip.
 int chunks_for(int cout, int cin) {

@@ -211,20 +211,29 @@ __global__ void __launch_bounds__(256) k_loss(const float* __restrict__ y, const
 }
 
 // ds = d(loss)/ds for layer 4: ds[p][ci] = sum_t W4[t][ci] dr[p - tap_t].
-__global__ void k_out_input_grad(const float* __restrict__ dr, const float* __restrict__ W4, float* __restrict__ ds,
-                                 int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * 32;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int ci = (int)(i & 31);
-    const size_t pix = i >> 5;
+// One thread per pixel: its 25 dr values serve all 32 channels (taps in the
+// same order as before, so each channel's sum is unchanged).
+__global__ void __launch_bounds__(256) k_out_input_grad(const float* __restrict__ dr, const float* __restrict__ W4,
+                                                        float* __restrict__ ds, int N, int H, int W) {
+    __shared__ float w[25 * 32];
+    for (int i = threadIdx.x; i < 25 * 32; i += blockDim.x) w[i] = W4[i];
+    __syncthreads();
+    const size_t total = (size_t)N * H * W;
+    const size_t pix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= total) return;
     const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
-    float acc = 0.0f;
+    float acc[32];
+#pragma unroll
+    for (int ci = 0; ci < 32; ++ci) acc[ci] = 0.0f;
     for (int t = 0; t < 25; ++t) {
         const int dy = t / 5 - 2, dx = t % 5 - 2;
-        acc += W4[t * 32 + ci] * dr[((size_t)n * H + wrap(yy - dy, H)) * W + wrap(xx - dx, W)];
+        const float d = dr[((size_t)n * H + wrap(yy - dy, H)) * W + wrap(xx - dx, W)];
+#pragma unroll
+        for (int ci = 0; ci < 32; ++ci) acc[ci] += w[t * 32 + ci] * d;
     }
-    ds[i] = acc;
+    float4* o = reinterpret_cast<float4*>(ds + pix * 32);
+#pragma unroll
+    for (int g = 0; g < 8; ++g) o[g] = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
 }
 
 // dz3 = bf16(ds * gelu'(z3))
@@ -281,8 +290,9 @@ __global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ 
     const size_t p0 = per * blockIdx.y, p1 = p0 + per < npix ? p0 + per : npix;
     const int Hx = a.up ? a.H / 2 : a.H, Wx = a.up ? a.W / 2 : a.W;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // Pixel coordinates advance with the pixel index (one division per chunk).
+    int xx = (int)(p0 % a.W), yy = (int)((p0 / a.W) % a.H), n = (int)(p0 / ((size_t)a.W * a.H));
     for (size_t pix = p0; pix < p1; ++pix) {
-        const int xx = (int)(pix % a.W), yy = (int)((pix / a.W) % a.H), n = (int)(pix / ((size_t)a.W * a.H));
         int sy = wrap(yy + dy, a.H), sx = wrap(xx + dx, a.W);
         if (a.up) {
             sy >>= 1;
@@ -304,21 +314,196 @@ __global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ 
             acc[2] += __uint_as_float(g.y << 16) * xv;
             acc[3] += __uint_as_float(g.y & 0xffff0000u) * xv;
         }
+        if (++xx == a.W) {
+            xx = 0;
+            if (++yy == a.H) {
+                yy = 0;
+                ++n;
+            }
+        }
     }
     float* dst = part + (size_t)blockIdx.y * a.Cout * 25 * a.Cin;
     const int nco = a.Cout == 1 ? 1 : 4;
     for (int j = 0; j < nco; ++j) dst[((size_t)(4 * cq + j) * 25 + t) * a.Cin + ci] = acc[j];
 }
 
-// Bias gradient partials: part[chunk][c] = sum over the chunk's pixels of dZ[pix][c].
+// Weight gradients of the two 32 <-> 64 channel layers on MFMA: a
+// transposed implicit GEMM, dW[co][t][ci] = sum over pixels p of
+// dZ[p][co] X[p + tap_t][ci], with the pixels as the MFMA's K dimension.
+// v_mfma_f32_16x16x32_bf16: A = 16 output channels x 32 pixels of dZ, B = 32
+// pixels x 16 input channels of X shifted by the tap, D = 16 x 16 of dW.
+// A workgroup (8 waves) walks a fixed range of 64-pixel row segments; per
+// segment it stages dZ (64 pixels x COUT) and the X tile the 25 taps reach
+// (5 rows x 68 columns x CIN, read at half resolution when UP) in LDS, and
+// wave w accumulates the (16 co x 16 ci) block w of all 25 taps.  A lane's
+// operands are 8 pixels of one channel, read from the pixel-major LDS copies
+// 2 bytes at a time (the row strides are padded by 8 channels so the 4 pixel
+// groups of a read land on distinct banks).  Partials go to
+// part[workgroup][co][t][ci] and are summed in workgroup order afterwards:
+// the result does not depend on timing.
+namespace wg {
+constexpr int SEG = 64, ROWS = 5, COLS = SEG + 4, THREADS = 512, GROUPS = 256;
+}
+
+template <int COUT, int CIN, int UP>
+__global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* __restrict__ part) {
+    using namespace wg;
+    static_assert((COUT / 16) * (CIN / 16) == THREADS / 64, "one 16x16 block per wave");
+    constexpr int XS = CIN + 8, ZS = COUT + 8;  // LDS strides per pixel (bf16)
+    __shared__ __attribute__((aligned(16))) uint16_t xt[ROWS * COLS * XS];
+    __shared__ __attribute__((aligned(16))) uint16_t dz[SEG * ZS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int cob = wave / (CIN / 16), cib = wave % (CIN / 16);
+    const int H = a.H, W = a.W, Hx = UP ? H / 2 : H, Wx = UP ? W / 2 : W;
+    const int segs_per_row = W / SEG;
+    const long nseg = (long)a.N * H * segs_per_row;
+    const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
+    f32x4 acc[25];
+#pragma unroll
+    for (int t = 0; t < 25; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (long sg = s0; sg < s1; ++sg) {
+        const int xb = (int)(sg % segs_per_row);
+        const long ry = sg / segs_per_row;
+        const int y = (int)(ry % H), n = (int)(ry / H), x0 = xb * SEG;
+        __syncthreads();  // the previous segment's LDS reads are done
+        // dZ: 64 pixels x COUT, contiguous in global memory.
+        {
+            const uint16_t* src = a.dz_h + (((size_t)n * H + y) * W + x0) * COUT;
+            for (int v = threadIdx.x; v < SEG * COUT / 8; v += THREADS) {
+                const int px = v / (COUT / 8), part8 = v % (COUT / 8);
+                *reinterpret_cast<uint4*>(dz + px * ZS + 8 * part8) =
+                    *reinterpret_cast<const uint4*>(src + (size_t)px * COUT + 8 * part8);
+            }
+        }
+        // X tile: rows y-2..y+2, columns x0-2..x0+65, wrapped on the torus.
+        for (int v = threadIdx.x; v < ROWS * COLS * CIN / 8; v += THREADS) {
+            const int pidx = v / (CIN / 8), part8 = v % (CIN / 8);
+            const int r = pidx / COLS, c = pidx % COLS;
+            int sy = wrap(y + r - 2, H), sx = wrap(x0 + c - 2, W);
+            if (UP) {
+                sy >>= 1;
+                sx >>= 1;
+            }
+            *reinterpret_cast<uint4*>(xt + pidx * XS + 8 * part8) = *reinterpret_cast<const uint4*>(
+                a.X + (((size_t)n * Hx + sy) * Wx + sx) * a.xstride + a.xoff + 8 * part8);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kc = 0; kc < SEG / 32; ++kc) {
+            const int p0 = 32 * kc + 8 * q;  // this lane's 8 pixels: p0 .. p0 + 7
+            uint32_t ab[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                ab[j] = (uint32_t)dz[(p0 + 2 * j) * ZS + 16 * cob + l16] |
+                        ((uint32_t)dz[(p0 + 2 * j + 1) * ZS + 16 * cob + l16] << 16);
+            const bf16x8 A = __builtin_bit_cast(bf16x8, make_uint4(ab[0], ab[1], ab[2], ab[3]));
+#pragma unroll
+            for (int t = 0; t < 25; ++t) {
+                const int dy = t / 5, dx = t % 5;  // tile row and column offsets (tap + 2)
+                const uint16_t* xb0 = xt + (dy * COLS + p0 + dx) * XS + 16 * cib + l16;
+                uint32_t bb[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bb[j] = (uint32_t)xb0[(2 * j) * XS] | ((uint32_t)xb0[(2 * j + 1) * XS] << 16);
+                const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(bb[0], bb[1], bb[2], bb[3]));
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // Lane holds dW[co = 16 cob + 4q + r][t][ci = 16 cib + l16].
+    float* dst = part + (size_t)blockIdx.x * COUT * 25 * CIN;
+#pragma unroll
+    for (int t = 0; t < 25; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[((size_t)(16 * cob + 4 * q + r) * 25 + t) * CIN + 16 * cib + l16] = acc[t][r];
+}
+
+// Weight gradients of the two thin layers, out[a][t][b] = sum over pixels p
+// of U[p][a] V[p + tap_t][b], with U the fp32 output gradient and V the bf16
+// layer input: layer 1 (A = 32 output channels, B = 1: V is the input x,
+// channel 12 of its im2col) and layer 4 (A = 1, B = 32).  A workgroup walks a
+// fixed range of 64-pixel row segments, stages U and the 5 x 68 V tile the
+// taps reach in LDS as fp32, and thread (t, quad) accumulates the four
+// outputs a = 4 quad .. (A = 32) or b = 4 quad .. (B = 32) of tap t over the
+// segment's pixels in order.  Partials go to part[workgroup][a][t][b].
+template <int A, int B>
+__global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restrict__ part) {
+    using wg::SEG;
+    using wg::ROWS;
+    using wg::COLS;
+    static_assert((A == 32 && B == 1) || (A == 1 && B == 32), "thin layers");
+    __shared__ __attribute__((aligned(16))) float u[SEG * A];
+    __shared__ __attribute__((aligned(16))) float v[ROWS * COLS * B];
+    const int H = a.H, W = a.W, segs_per_row = W / SEG;
+    const long nseg = (long)a.N * H * segs_per_row;
+    const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
+    const int t = threadIdx.x / 8, quad = threadIdx.x % 8;  // t < 25 for the 200 working threads
+    const bool active = t < 25;
+    const int dy = t / 5, dx = t % 5;  // tile offsets of the tap (tap + 2)
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (long sg = s0; sg < s1; ++sg) {
+        const int xb = (int)(sg % segs_per_row);
+        const long ry = sg / segs_per_row;
+        const int y = (int)(ry % H), n = (int)(ry / H), x0 = xb * SEG;
+        __syncthreads();  // the previous segment's reads are done
+        for (int i = threadIdx.x; i < SEG * A; i += 256)
+            u[i] = a.dz_f[(((size_t)n * H + y) * W + x0) * A + i];
+        for (int i = threadIdx.x; i < ROWS * COLS * B; i += 256) {
+            const int pidx = i / B, c = i % B, r = pidx / COLS, cc = pidx % COLS;
+            const int sy = wrap(y + r - 2, H), sx = wrap(x0 + cc - 2, W);
+            v[i] = bf16f(a.X[(((size_t)n * H + sy) * W + sx) * a.xstride + a.xoff + c]);
+        }
+        __syncthreads();
+        if (active) {
+            for (int p = 0; p < SEG; ++p) {
+                const int vp = (dy * COLS + p + dx) * B;  // V at pixel p + tap
+                if constexpr (A == 32) {
+                    const float4 uu = *reinterpret_cast<const float4*>(u + p * A + 4 * quad);
+                    const float vv = v[vp];
+                    acc[0] += uu.x * vv;
+                    acc[1] += uu.y * vv;
+                    acc[2] += uu.z * vv;
+                    acc[3] += uu.w * vv;
+                } else {
+                    const float uu = u[p];
+                    const float4 vv = *reinterpret_cast<const float4*>(v + vp + 4 * quad);
+                    acc[0] += uu * vv.x;
+                    acc[1] += uu * vv.y;
+                    acc[2] += uu * vv.z;
+                    acc[3] += uu * vv.w;
+                }
+            }
+        }
+    }
+    if (active) {
+        float* dst = part + (size_t)blockIdx.x * A * 25 * B;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (A == 32) dst[(size_t)(4 * quad + k) * 25 + t] = acc[k];
+            else dst[(size_t)t * B + 4 * quad + k] = acc[k];
+        }
+    }
+}
+
+// Bias gradient partials: part[chunk][c] = sum over the chunk's pixels of
+// dZ[pix][c].  C divides 256: thread j sums channel j % C over the chunk's
+// pixels j / C, j / C + 256 / C, ...; the 256 / C lanes of a channel are then
+// added in lane order through LDS (a fixed order: bitwise reproducible).
 __global__ void __launch_bounds__(256) k_bgrad(const float* __restrict__ dz_f, const uint16_t* __restrict__ dz_h,
                                                int C, size_t npix, float* __restrict__ part) {
+    __shared__ float sh[256];
     const size_t per = (npix + gridDim.x - 1) / gridDim.x;
     const size_t p0 = per * blockIdx.x, p1 = p0 + per < npix ? p0 + per : npix;
-    for (int c = threadIdx.x; c < C; c += 256) {
-        float acc = 0.0f;
-        for (size_t pix = p0; pix < p1; ++pix) acc += dz_f ? dz_f[pix * C + c] : bf16f(dz_h[pix * C + c]);
-        part[(size_t)blockIdx.x * C + c] = acc;
+    const int c = threadIdx.x % C, stride = 256 / C;
+    float acc = 0.0f;
+    for (size_t pix = p0 + threadIdx.x / C; pix < p1; pix += stride)
+        acc += dz_f ? dz_f[pix * C + c] : bf16f(dz_h[pix * C + c]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        float t = 0.0f;
+        for (int j = 0; j < stride; ++j) t += sh[j * C + threadIdx.x];
+        part[(size_t)blockIdx.x * C + threadIdx.x] = t;
     }
 }
 
@@ -461,7 +646,7 @@ void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const floa
 
 void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W) {
     Launch L(p, s, "surr_out_input_grad");
-    hipLaunchKernelGGL(k_out_input_grad, dim3(blocks((size_t)N * H * W * 32)), dim3(256), 0, s, dr, W4, ds, N, H, W);
+    hipLaunchKernelGGL(k_out_input_grad, dim3(blocks((size_t)N * H * W)), dim3(256), 0, s, dr, W4, ds, N, H, W);
     AH_HIP(hipGetLastError());
 }
 
@@ -487,7 +672,27 @@ void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const flo
 void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out) {
     const int Cq = (a.Cout + 3) / 4;
     const int nout = Cq * 25 * a.Cin, nw = a.Cout * 25 * a.Cin;
-    {
+    const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 &&
+                      ((a.Cout == 32 && a.Cin == 64 && a.up) || (a.Cout == 64 && a.Cin == 32 && !a.up));
+    const bool thin = a.dz_f && a.W % wg::SEG == 0 && !a.up &&
+                      ((a.Cout == 32 && a.Cin == 1) || (a.Cout == 1 && a.Cin == 32));
+    if (thin) {
+        chunks = wg::GROUPS;
+        Launch L(p, s, "surr_wgrad_thin");
+        if (a.Cout == 32)
+            hipLaunchKernelGGL((k_wgrad_thin<32, 1>), dim3(chunks), dim3(256), 0, s, a, part);
+        else
+            hipLaunchKernelGGL((k_wgrad_thin<1, 32>), dim3(chunks), dim3(256), 0, s, a, part);
+        AH_HIP(hipGetLastError());
+    } else if (mfma) {
+        chunks = wg::GROUPS;
+        Launch L(p, s, "surr_wgrad_mfma");
+        if (a.Cout == 32)
+            hipLaunchKernelGGL((k_wgrad_mfma<32, 64, 1>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part);
+        else
+            hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part);
+        AH_HIP(hipGetLastError());
+    } else {
         Launch L(p, s, "surr_wgrad");
         hipLaunchKernelGGL(k_wgrad, dim3(blocks(nout), chunks), dim3(256), 0, s, a, part);
         AH_HIP(hipGetLastError());
@@ -501,6 +706,7 @@ void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, 
            int chunks, float* out) {
     {
         Launch L(p, s, "surr_bgrad");
+        if (C < 1 || 256 % C) throw Error{AH_ERR_INVALID, "bias gradient: channels must divide 256"};
         hipLaunchKernelGGL(k_bgrad, dim3(chunks), dim3(256), 0, s, dz_f, dz_h, C, npix, part);
         AH_HIP(hipGetLastError());
     }
