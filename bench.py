@@ -606,7 +606,9 @@ def main() -> int:
                     help="time the two single-world paths only (for counter runs under a profiler)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned surrogate section")
     ap.add_argument("--surrogate", action="store_true", help=argparse.SUPPRESS)  # on by default now
-    ap.add_argument("--surrogate-train-steps", type=int, default=200)
+    ap.add_argument("--surrogate-train-steps", type=int, default=1000)
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="run only the surrogate section and print it (for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 checks")
     ap.add_argument("--torch-leg", default=None, help=argparse.SUPPRESS)
@@ -620,6 +622,9 @@ def main() -> int:
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": time_surrogate(args.seed, args.surrogate_train_steps)}), flush=True)
+        return 0
     from auto_amd import _abi
     import ctypes
     lib = _abi.load()
