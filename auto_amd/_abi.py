@@ -118,11 +118,17 @@ class KernelTime(ctypes.Structure):
 
 
 BATCH_MAX_CANDIDATES = 8
+BATCH_MAX_GROUP_TUNE = 64
 
 
 class BatchTune(ctypes.Structure):
     _fields_ = [("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32), ("grid", ctypes.c_uint32),
                 ("ms_per_step", ctypes.c_double)]
+
+
+class BatchGroupTune(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("column", ctypes.c_uint32),
+                ("row", ctypes.c_uint32), ("ms_per_step", ctypes.c_double)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -140,6 +146,8 @@ class BatchInfo(ctypes.Structure):
         ("tune", BatchTune * BATCH_MAX_CANDIDATES),
         ("fft_worlds", ctypes.c_uint32),
         ("fft_groups", ctypes.c_uint32),
+        ("n_group_tune", ctypes.c_uint32),
+        ("group_tune", BatchGroupTune * BATCH_MAX_GROUP_TUNE),
     ]
 
 

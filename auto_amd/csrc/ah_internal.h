@@ -141,7 +141,10 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
                     const float2* twW);
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH, const int* ksrc,
-                     int n_kernels, bool diag = false);
+                     int n_kernels, bool diag = false, bool generic = false);
+// Whether a batch group's column (row) pass has a fused batch kernel besides the generic one.
+bool batch_col_fused_ok(int W, int H);
+bool batch_row_fused_ok(int W, int H);
 // conv_out != nullptr: write kernel conv_k's potential there instead of updating the state.
 // spec_next != nullptr (update only): also write the half spectra of the updated
 // state there, which is what launch_row_r2c would produce from it next.
@@ -151,7 +154,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
                     float* conv_out, int conv_k, float2* spec_next, int rule,
-                    const KernelParams* kp_dev = nullptr, const float* dt_dev = nullptr);
+                    const KernelParams* kp_dev = nullptr, const float* dt_dev = nullptr, bool generic = false);
 size_t fft_lds_limit();
 
 // direct_bf16.hip

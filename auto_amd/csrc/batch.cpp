@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -38,6 +39,8 @@ struct FftGroup {
     KernelParams* kp = nullptr;  // [n] growth per world
     float* dt = nullptr;         // [n]
     bool ready = false;          // spec holds the spectra of state
+    bool col_generic = false;    // tuned: the generic column kernel instead of the fused one
+    bool row_generic = false;    // tuned: the generic row kernel instead of the fused one
 };
 
 struct ah_batch {
@@ -66,6 +69,8 @@ struct ah_batch {
     int* seg = nullptr;
     ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES]{};
     uint32_t n_tune = 0;
+    ah_batch_group_tune group_tune[AH_BATCH_MAX_GROUP_TUNE]{};
+    uint32_t n_group_tune = 0;
 
     int parity = 0;
     uint64_t generation = 0;
@@ -192,14 +197,78 @@ void install_tiling(ah_batch* b, int cols, int rows, int grid) {
         }
 }
 
+// One step of one FFT group: the column pass, then the row pass (which also
+// writes the next step's spectra).
+void launch_group_step(ah_batch* b, FftGroup& g, Profiler* p, bool col_generic, bool row_generic) {
+    launch_col_conv(b->stream, p, g.W, g.H, g.n, g.spec, g.spec, g.khat, g.kpack, g.twH, nullptr, 0, true,
+                    col_generic);
+    launch_row_c2r(b->stream, p, g.W, g.H, g.n, g.n, g.spec, g.state, g.state, nullptr, 0.0f, g.twW, nullptr, 0,
+                   g.spec, AH_RULE_LENIA, g.kp, g.dt, row_generic);
+}
+
 void launch_step(ah_batch* b, Profiler* p) {
     if (b->ntiles) launch_direct_batch(b->stream, p, b->args(b->parity), b->cols, b->rows, b->grid);
     b->parity ^= 1;
-    for (FftGroup& g : b->groups) {
-        launch_col_conv(b->stream, p, g.W, g.H, g.n, g.spec, g.spec, g.khat, g.kpack, g.twH, nullptr, 0, true);
-        launch_row_c2r(b->stream, p, g.W, g.H, g.n, g.n, g.spec, g.state, g.state, nullptr, 0.0f, g.twW, nullptr, 0,
-                       g.spec, AH_RULE_LENIA, g.kp, g.dt);
+    for (FftGroup& g : b->groups) launch_group_step(b, g, p, g.col_generic, g.row_generic);
+}
+
+// Times every admissible (column kernel, row kernel) pair of each FFT group
+// on the batch (zero states at creation: the work does not depend on the
+// data), keeps the fastest per group and records every candidate.
+// AUTO_HIP_BATCH_FFT_KERNELS=fused|generic skips the timing and takes the
+// fused kernels wherever they apply, or the generic ones everywhere (tests
+// use it to run each variant on the same batch).
+void tune_groups(ah_batch* b) {
+    b->n_group_tune = 0;
+    if (b->groups.empty()) return;
+    const char* force = std::getenv("AUTO_HIP_BATCH_FFT_KERNELS");
+    if (force && *force && std::strcmp(force, "tune") != 0) {
+        const bool generic = std::strcmp(force, "generic") == 0;
+        if (!generic && std::strcmp(force, "fused") != 0)
+            throw Error{AH_ERR_INVALID, "AUTO_HIP_BATCH_FFT_KERNELS must be fused, generic or tune"};
+        for (FftGroup& g : b->groups) {
+            g.col_generic = generic || !batch_col_fused_ok(g.W, g.H);
+            g.row_generic = generic || !batch_row_fused_ok(g.W, g.H);
+        }
+        return;
     }
+    hipEvent_t e0, e1;
+    AH_HIP(hipEventCreate(&e0));
+    AH_HIP(hipEventCreate(&e1));
+    try {
+        for (FftGroup& g : b->groups) {
+            double best = 1e300;
+            for (int cg = 0; cg < 2; ++cg) {
+                if (!cg && !batch_col_fused_ok(g.W, g.H)) continue;
+                for (int rg = 0; rg < 2; ++rg) {
+                    if (!rg && !batch_row_fused_ok(g.W, g.H)) continue;
+                    launch_group_step(b, g, nullptr, cg, rg);  // warm-up
+                    const int reps = 5;
+                    AH_HIP(hipEventRecord(e0, b->stream));
+                    for (int r = 0; r < reps; ++r) launch_group_step(b, g, nullptr, cg, rg);
+                    AH_HIP(hipEventRecord(e1, b->stream));
+                    AH_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    AH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    const double per = ms / reps;
+                    if (b->n_group_tune < AH_BATCH_MAX_GROUP_TUNE)
+                        b->group_tune[b->n_group_tune++] =
+                            ah_batch_group_tune{(uint32_t)g.W, (uint32_t)g.H, (uint32_t)cg, (uint32_t)rg, per};
+                    if (per < best) {
+                        best = per;
+                        g.col_generic = cg;
+                        g.row_generic = rg;
+                    }
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 // Pass 1 for FFT groups whose state changed outside a step (seeded or
@@ -455,6 +524,7 @@ void setup(ah_batch* b) {
     // The tuner steps zero states (the work is data-independent); one eager
     // step warms the FFT launches too.  Then the worlds are seeded.
     autotune(b);
+    tune_groups(b);
     launch_step(b, nullptr);
     for (size_t w = 0; w < n; ++w) {
         float* dst = b->fft[w] ? b->groups[b->group_of[w]].state +
@@ -586,6 +656,8 @@ ah_status ah_batch_info_get(const ah_batch* b, ah_batch_info* out) {
         for (uint32_t k = 0; k < b->n_tune; ++k) i.tune[k] = b->tune[k];
         for (char f : b->fft) i.fft_worlds += f ? 1 : 0;
         i.fft_groups = (uint32_t)b->groups.size();
+        i.n_group_tune = b->n_group_tune;
+        for (uint32_t k = 0; k < b->n_group_tune; ++k) i.group_tune[k] = b->group_tune[k];
         *out = i;
     });
 }

@@ -1412,9 +1412,12 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
     AH_HIP(hipGetLastError());
 }
 
+bool batch_col_fused_ok(int W, int H) { return H >= 64 && H <= 2048 && W >= 16; }
+bool batch_row_fused_ok(int W, int H) { return W >= 128 && W <= 4096 && row_rows(W, H) * W == 4096; }
+
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH,
-                     const int* ksrc, int n_kernels, bool diag) {
+                     const int* ksrc, int n_kernels, bool diag, bool generic) {
     ColArgs a;
     a.W = W;
     a.H = H;
@@ -1452,7 +1455,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
-    if (diag && H >= 64 && H <= 2048 && W >= 16) {
+    if (diag && !generic && batch_col_fused_ok(W, H)) {
         auto fused = [&](auto kernel, int threads, int cpw) {
             const size_t flds = padded((size_t)cpw * H) * sizeof(float2);
             set_lds(kernel, flds);
@@ -1493,7 +1496,7 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int n_kernels, const float2* spec,
                     const float* state_in, float* state_out, const KernelParams* kp, float dt, const float2* twW,
                     float* conv_out, int conv_k, float2* spec_next, int rule,
-                    const KernelParams* kp_dev, const float* dt_dev) {
+                    const KernelParams* kp_dev, const float* dt_dev, bool generic) {
     RowArgs a;
     a.W = W;
     a.H = H;
@@ -1522,7 +1525,7 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
         set_lds(kernel, lds);
         hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
     };
-    const bool fusable = W >= 128 && W <= 4096 && a.rows * W == 4096 && !conv_out && a.spec_next;
+    const bool fusable = !generic && W >= 128 && W <= 4096 && a.rows * W == 4096 && !conv_out && a.spec_next;
     auto fused = [&](auto diag_tag, auto rule_tag, auto multi_tag) {
         constexpr bool D = decltype(diag_tag)::value;
         constexpr int R = decltype(rule_tag)::value;

@@ -221,6 +221,9 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
            "tile": [info.tile_cols, info.tile_rows], "grid": info.grid, "tiles": info.ntiles,
            "autotune": [{"tile": [info.tune[i].tile_cols, info.tune[i].tile_rows], "grid": info.tune[i].grid,
                          "ms_per_step": info.tune[i].ms_per_step} for i in range(info.n_tune)],
+           "fft_group_tune": [{"world": [t.width, t.height], "column": "generic" if t.column else "fused",
+                               "row": "generic" if t.row else "fused", "ms_per_step": t.ms_per_step}
+                              for t in info.group_tune[:info.n_group_tune]],
            "graph_steps": graph_steps,
            # 1 - (summed device time of the launches, from per-launch events
            # on eager steps) / (wall time of graph replays of as many steps).

@@ -288,13 +288,24 @@ ah_status ah_sim_set_direct_tile(ah_sim* sim, const ah_direct_tile* tile);
  * own direct launch would cost more than they save.
  * ah_batch_step replays hipGraph-captured chunks of `graph_steps` steps and
  * runs the remainder one by one.  At creation an autotuner times every
- * admissible (tile shape, grid) pair of the direct launch on the batch itself
- * and keeps the fastest; the candidates and their times are reported by
+ * admissible (tile shape, grid) pair of the direct launch, and for each FFT
+ * group every admissible pair of column-pass and row-pass kernels (the
+ * fused batch kernels and the generic ones), on the batch itself, and keeps
+ * the fastest; the candidates and their times are reported by
  * ah_batch_info_get.
  */
 typedef struct ah_batch ah_batch;
 
-enum { AH_BATCH_MAX_CANDIDATES = 8, AH_BATCH_FFT_ABOVE = 30 };
+enum { AH_BATCH_MAX_CANDIDATES = 8, AH_BATCH_FFT_ABOVE = 30, AH_BATCH_MAX_GROUP_TUNE = 64 };
+
+/* One timed FFT-group candidate: the group's world size, which column and
+ * row kernels ran (0: the fused batch kernel, 1: the generic kernel) and the
+ * measured time of one step of the group. */
+typedef struct {
+    uint32_t width, height;
+    uint32_t column, row;
+    double ms_per_step;
+} ah_batch_group_tune;
 
 typedef struct {
     uint32_t tile_cols;  /* output columns per tile (64 or 128)               */
@@ -313,6 +324,11 @@ typedef struct {
     ah_batch_tune tune[AH_BATCH_MAX_CANDIDATES];
     uint32_t fft_worlds;   /* worlds on the FFT path (the rest run direct BF16) */
     uint32_t fft_groups;   /* same-size FFT groups, two launches each per step  */
+    /* FFT group tuning: every timed (column kernel, row kernel) pair of every
+     * group, in group order (the first AH_BATCH_MAX_GROUP_TUNE of them); the
+     * step uses each group's fastest. */
+    uint32_t n_group_tune;
+    ah_batch_group_tune group_tune[AH_BATCH_MAX_GROUP_TUNE];
 } ah_batch_info;
 
 ah_status ah_batch_new(const ah_lenia_desc* descs, uint32_t n_worlds, const uint32_t* seeds, ah_batch** out);

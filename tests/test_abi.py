@@ -29,17 +29,22 @@ def test_struct_sizes_match_header(tmp_path):
     """ctypes layouts equal the C compiler's sizeof for every ABI struct."""
     import shutil
     import subprocess
-    want = [40, 396, 64, 56, 56]
+    names = ["ah_kernel", "ah_lenia_desc", "ah_stats", "ah_info", "ah_kernel_time", "ah_direct_tile",
+             "ah_batch_tune", "ah_batch_group_tune", "ah_batch_info", "ah_surrogate_report"]
+    types = [_abi.Kernel, _abi.LeniaDesc, _abi.Stats, _abi.Info, _abi.KernelTime, _abi.DirectTile,
+             _abi.BatchTune, _abi.BatchGroupTune, _abi.BatchInfo, _abi.SurrogateReport]
+    want = [52, 508, 64, 56, 56, 12, 24, 24, 1800, 64]  # x86-64 gcc, when no compiler is on the path
     gcc = shutil.which("gcc")
     if gcc:
         src = tmp_path / "sz.c"
-        src.write_text('#include "include/auto_hip.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu %zu",'
-                       'sizeof(ah_kernel),sizeof(ah_lenia_desc),sizeof(ah_stats),sizeof(ah_info),'
-                       'sizeof(ah_kernel_time));return 0;}')
+        fmt = " ".join(["%zu"] * len(names))
+        sizes = ",".join(f"sizeof({n})" for n in names)
+        src.write_text(f'#include "include/auto_hip.h"\n#include <stdio.h>\n'
+                       f'int main(void){{printf("{fmt}",{sizes});return 0;}}\n')
         exe = tmp_path / "sz"
         subprocess.run([gcc, f"-I{ROOT}", str(src), "-o", str(exe)], check=True)
         want = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    got = [ctypes.sizeof(t) for t in (_abi.Kernel, _abi.LeniaDesc, _abi.Stats, _abi.Info, _abi.KernelTime)]
+    got = [ctypes.sizeof(t) for t in types]
     assert got == want
 
 

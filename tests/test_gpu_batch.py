@@ -159,20 +159,37 @@ def _world_wh(w, h, radius, seed_dt=0.1):
                   init=W.INIT_PATCH, init_size=min(w, h) // 4)
 
 
-def test_batch_fft_group_shapes(O):
+@pytest.mark.parametrize("kernels", ["fused", "generic", "tune"])
+def test_batch_fft_group_shapes(O, monkeypatch, kernels):
     """FFT groups at the edges of the batch kernels' ranges: heights 64 and
     2048 (the fused column pass with four and two columns per workgroup),
     512 and 1024 (the fused column pass at the benchmark batch's two largest
     sides), height 4096 (the generic column pass in diagonal mode), width 64
     (the generic row pass) and widths 128, 512, 1024 and 2048 (the fused row
-    pass), two worlds per group.  Every world vs a standalone Sim (within
-    1e-5) and vs the FP64 step (within 1e-3) from the same input."""
+    pass), two worlds per group.  Run three ways: the fused kernels wherever
+    they apply, the generic kernels everywhere, and the tuner's choice per
+    group.  Every world vs a standalone Sim (within 1e-5) and vs the FP64
+    step (within 1e-3) from the same input."""
+    monkeypatch.setenv("AUTO_HIP_BATCH_FFT_KERNELS", kernels)
     shapes = [(64, 64, 8), (2048, 2048, 20), (64, 4096, 12), (2048, 64, 12), (128, 2048, 16),
               (512, 512, 18), (1024, 1024, 24)]
     worlds = [(_world_wh(w, h, r), 10 * i + j) for i, (w, h, r) in enumerate(shapes) for j in range(2)]
     b = Batch(worlds)
     info = b.info()
     assert info.fft_worlds == len(worlds) and info.fft_groups == len(shapes)
+    if kernels == "tune":
+        # Every group timed at least once (the generic pair always applies),
+        # and a group where both fused kernels apply times all four pairs.
+        seen = {}
+        for k in range(info.n_group_tune):
+            t = info.group_tune[k]
+            assert t.ms_per_step > 0
+            seen.setdefault((t.width, t.height), set()).add((t.column, t.row))
+        assert set(seen) == {(w, h) for w, h, _ in shapes}
+        assert all((1, 1) in v for v in seen.values())
+        assert seen[(512, 512)] == {(0, 0), (0, 1), (1, 0), (1, 1)}
+    else:
+        assert info.n_group_tune == 0
     b.step(2)
     before = [b.read_world(i) for i in range(len(worlds))]
     b.step(1)
