@@ -197,6 +197,32 @@ void install_tiling(ah_batch* b, int cols, int rows, int grid) {
         }
 }
 
+// Device bytes one FFT world holds in its group: state, spectra, kernel
+// spectra and the packed column.
+size_t fft_world_bytes(int W, int H) {
+    const size_t cells = (size_t)W * H, spec = (size_t)(W / 2) * H, np = (size_t)H / 2 + 1;
+    return sizeof(float) * cells + sizeof(float2) * spec + sizeof(float) * (size_t)(W / 8) * np * BC +
+           sizeof(float2) * np;
+}
+
+// The largest working set of one FFT group: 256 MiB, the Infinity Cache
+// (the memory-side last-level cache) of an MI355X.  Measured on 100 worlds
+// of 1024^2 (9.1 MiB each) stepped group-major: caps of 96, 192, 256, 320
+// and 512 MiB gave 0.70, 0.64, 0.62, 0.67 and 0.70 ms per step of the
+// 384-world batch.  AUTO_HIP_BATCH_GROUP_MB overrides it (0: one group per
+// size).
+size_t group_cap_bytes() {
+    const char* e = std::getenv("AUTO_HIP_BATCH_GROUP_MB");
+    if (e && *e) return (size_t)std::strtoull(e, nullptr, 10) << 20;
+    return (size_t)256 << 20;
+}
+
+// AUTO_HIP_BATCH_ORDER=step issues the launches step by step instead.
+bool group_major() {
+    const char* e = std::getenv("AUTO_HIP_BATCH_ORDER");
+    return !(e && std::strcmp(e, "step") == 0);
+}
+
 // One step of one FFT group: the column pass, then the row pass (which also
 // writes the next step's spectra).
 void launch_group_step(ah_batch* b, FftGroup& g, Profiler* p, bool col_generic, bool row_generic) {
@@ -206,15 +232,36 @@ void launch_group_step(ah_batch* b, FftGroup& g, Profiler* p, bool col_generic, 
                    g.spec, AH_RULE_LENIA, g.kp, g.dt, row_generic);
 }
 
-void launch_step(ah_batch* b, Profiler* p) {
+void launch_direct_step(ah_batch* b, Profiler* p) {
     if (b->ntiles) launch_direct_batch(b->stream, p, b->args(b->parity), b->cols, b->rows, b->grid);
     b->parity ^= 1;
+}
+
+void launch_step(ah_batch* b, Profiler* p) {
+    launch_direct_step(b, p);
     for (FftGroup& g : b->groups) launch_group_step(b, g, p, g.col_generic, g.row_generic);
 }
 
-// Times every admissible (column kernel, row kernel) pair of each FFT group
-// on the batch (zero states at creation: the work does not depend on the
-// data), keeps the fastest per group and records every candidate.
+// `steps` steps of every world.  The worlds are independent, so the order
+// of launches is free: group-major order (the direct launches, then each
+// FFT group's `steps` steps back to back) keeps a group's state, spectra and
+// kernel spectra in the 256 MiB Infinity Cache from one step to the next,
+// where step-major order streams the whole batch through HBM every step.
+// Each world's arithmetic is the same in either order (bit for bit).
+void launch_steps(ah_batch* b, uint32_t steps, Profiler* p) {
+    if (!group_major()) {
+        for (uint32_t i = 0; i < steps; ++i) launch_step(b, p);
+        return;
+    }
+    for (uint32_t i = 0; i < steps; ++i) launch_direct_step(b, p);
+    for (FftGroup& g : b->groups)
+        for (uint32_t i = 0; i < steps; ++i) launch_group_step(b, g, p, g.col_generic, g.row_generic);
+}
+
+// Times every admissible (column kernel, row kernel) pair of each size's
+// first FFT group on the batch (zero states at creation: the work does not
+// depend on the data), keeps the fastest for every group of that size and
+// records every candidate.
 // AUTO_HIP_BATCH_FFT_KERNELS=fused|generic skips the timing and takes the
 // fused kernels wherever they apply, or the generic ones everywhere (tests
 // use it to run each variant on the same batch).
@@ -236,7 +283,16 @@ void tune_groups(ah_batch* b) {
     AH_HIP(hipEventCreate(&e0));
     AH_HIP(hipEventCreate(&e1));
     try {
-        for (FftGroup& g : b->groups) {
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            FftGroup& g = b->groups[gi];
+            bool done = false;  // groups of one size take the first one's choice
+            for (size_t e = 0; e < gi && !done; ++e)
+                if (b->groups[e].W == g.W && b->groups[e].H == g.H) {
+                    g.col_generic = b->groups[e].col_generic;
+                    g.row_generic = b->groups[e].row_generic;
+                    done = true;
+                }
+            if (done) continue;
             double best = 1e300;
             for (int cg = 0; cg < 2; ++cg) {
                 if (!cg && !batch_col_fused_ok(g.W, g.H)) continue;
@@ -416,6 +472,26 @@ void setup(ah_batch* b) {
         if (!explicit_direct && auto_direct_cells * 20 < all_cells)
             for (size_t w = 0; w < n; ++w)
                 if (b->d[w].path == AH_PATH_AUTO) fft_path[w] = 1;
+        // FFT worlds of one size form groups whose working set (state,
+        // spectra, kernel spectra) stays under the group cap, split evenly.
+        std::vector<std::pair<std::pair<int, int>, int>> per_size;  // (W, H) -> worlds
+        for (size_t w = 0; w < n; ++w) {
+            if (!fft_path[w]) continue;
+            const std::pair<int, int> wh((int)b->d[w].width, (int)b->d[w].height);
+            auto it = std::find_if(per_size.begin(), per_size.end(), [&](auto& e) { return e.first == wh; });
+            if (it == per_size.end()) per_size.emplace_back(wh, 1);
+            else ++it->second;
+        }
+        const size_t cap = group_cap_bytes();
+        auto group_max = [&](int W, int H) {
+            int count = 0;
+            for (auto& e : per_size)
+                if (e.first == std::make_pair(W, H)) count = e.second;
+            const size_t ws = fft_world_bytes(W, H);
+            if (!cap) return count;
+            const int chunks = (int)std::max<size_t>(1, ((size_t)count * ws + cap - 1) / cap);
+            return (count + chunks - 1) / chunks;
+        };
         for (size_t w = 0; w < n; ++w) {
             const ah_lenia_desc& d = b->d[w];
             const int R = (int)std::ceil((double)d.kernels[0].radius);
@@ -424,8 +500,10 @@ void setup(ah_batch* b) {
             if (use_fft) {
                 b->fft[w] = 1;
                 int gi = -1;
+                const int gmax = group_max((int)d.width, (int)d.height);
                 for (size_t g = 0; g < b->groups.size(); ++g)
-                    if (b->groups[g].W == (int)d.width && b->groups[g].H == (int)d.height) gi = (int)g;
+                    if (b->groups[g].W == (int)d.width && b->groups[g].H == (int)d.height && b->groups[g].n < gmax)
+                        gi = (int)g;
                 if (gi < 0) {
                     b->groups.emplace_back();
                     gi = (int)b->groups.size() - 1;
@@ -551,7 +629,7 @@ hipGraphExec_t graph_for(ah_batch* b) {
     AH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int start = b->parity;
     try {
-        for (uint32_t i = 0; i < b->graph_steps; ++i) launch_step(b, nullptr);
+        launch_steps(b, b->graph_steps, nullptr);
     } catch (...) {
         (void)hipStreamEndCapture(b->stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -571,16 +649,13 @@ hipGraphExec_t graph_for(ah_batch* b) {
 
 void run_steps(ah_batch* b, uint32_t steps) {
     ensure_spectra(b);
-    while (steps) {
-        if (steps >= b->graph_steps) {
-            AH_HIP(hipGraphLaunch(graph_for(b), b->stream));
-            steps -= b->graph_steps;
-            b->generation += b->graph_steps;
-        } else {
-            launch_step(b, &b->prof);
-            --steps;
-            ++b->generation;
-        }
+    for (; steps >= b->graph_steps; steps -= b->graph_steps) {
+        AH_HIP(hipGraphLaunch(graph_for(b), b->stream));
+        b->generation += b->graph_steps;
+    }
+    if (steps) {
+        launch_steps(b, steps, &b->prof);
+        b->generation += steps;
     }
 }
 
@@ -674,7 +749,7 @@ ah_status ah_batch_measure(ah_batch* b, uint32_t steps, double* kernel_ms, doubl
         b->prof.reset();
         b->prof.enabled = true;
         auto t0 = std::chrono::steady_clock::now();
-        for (uint32_t i = 0; i < steps; ++i) launch_step(b, &b->prof);
+        launch_steps(b, steps, &b->prof);
         AH_HIP(hipStreamSynchronize(b->stream));
         auto t1 = std::chrono::steady_clock::now();
         double ksum = 0.0;

@@ -197,7 +197,29 @@ def time_path(path: str, side: int, steps: int, warmup: int, seed: int):
     return d, sim, r
 
 
-def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "auto", check: bool = True):
+def batch_profile_run(seed: int, steps: int, n: int = 384) -> dict:
+    """The benchmark batch (AUTO) for a profiler: 100 graph-replayed warm-up
+    steps, then one timed block of `steps` steps (rounded up to whole
+    100-step graphs)."""
+    from auto_amd import Batch, workloads as W
+    b = Batch(W.uneven_worlds(seed=seed, n=n, path="auto"))
+    info = b.info()
+    b.step(100)
+    b.wait()
+    steps = 100 * max(1, (steps + 99) // 100)
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.wait()
+    t = time.perf_counter() - t0
+    return {"worlds": n, "fft_groups": info.fft_groups, "steps": steps, "ms_per_step": 1e3 * t / steps,
+            "value": info.total_cells * steps / t, "unit": "cell-updates/s",
+            "fft_group_tune": [{"world": [g.width, g.height], "column": "generic" if g.column else "fused",
+                                "row": "generic" if g.row else "fused", "ms_per_step": g.ms_per_step}
+                               for g in info.group_tune[:info.n_group_tune]]}
+
+
+def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "auto", check: bool = True,
+               measure: bool = True):
     """The 384 uneven worlds: aggregate cell-updates/s over graph-replayed
     steps (median of 100-step blocks), autotuner choices, the launch-overhead
     share, and each world's error against the plain single-world FFT step
@@ -214,7 +236,7 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
     b.step(graph_steps)  # warm-up: captures the graph
     b.wait()
     r = timed_blocks(b.step, b.wait, graph_steps, cells)
-    m = b.measure(graph_steps)
+    m = b.measure(graph_steps) if measure else None
     out = {"worlds": n, "path": path, "fft_worlds": info.fft_worlds, "fft_groups": info.fft_groups,
            "cells_per_step": cells, "value": r["value"], "unit": "cell-updates/s", "ms_per_step": r["ms_per_step"],
            "p10": r["p10"], "p90": r["p90"], "blocks": r["blocks"],
@@ -229,8 +251,8 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
            # on eager steps) / (wall time of graph replays of as many steps).
            # Per-launch events lengthen each timed launch slightly, so the
            # share can come out at or below 0: then it is reported as 0.
-           "launch_overhead_share": max(0.0, 1.0 - m["kernel_ms"] / m["graph_wall_ms"]),
-           "eager_over_graph_wall": m["eager_wall_ms"] / m["graph_wall_ms"],
+           "launch_overhead_share": max(0.0, 1.0 - m["kernel_ms"] / m["graph_wall_ms"]) if m else None,
+           "eager_over_graph_wall": m["eager_wall_ms"] / m["graph_wall_ms"] if m else None,
            "measure": m}
     if check:
         before = [b.read_world(i) for i in range(n)]
@@ -612,6 +634,9 @@ def main() -> int:
     ap.add_argument("--surrogate-train-steps", type=int, default=1000)
     ap.add_argument("--surrogate-only", action="store_true",
                     help="run only the surrogate section and print it (for profiler runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 384-world batch: 100 warm-up steps, then --steps steps in one timed block "
+                         "(for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 checks")
     ap.add_argument("--torch-leg", default=None, help=argparse.SUPPRESS)
@@ -627,6 +652,9 @@ def main() -> int:
 
     if args.surrogate_only:
         print(json.dumps({"surrogate": time_surrogate(args.seed, args.surrogate_train_steps)}), flush=True)
+        return 0
+    if args.batch_only:
+        print(json.dumps({"batch": batch_profile_run(args.seed, args.steps)}), flush=True)
         return 0
     from auto_amd import _abi
     import ctypes

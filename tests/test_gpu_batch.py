@@ -206,6 +206,31 @@ def test_batch_fft_group_shapes(O, monkeypatch, kernels):
     assert worst64 <= 1e-3 and worst_sim < 1e-5
 
 
+def test_batch_group_split_and_order_are_bitwise_neutral(monkeypatch):
+    """FFT worlds of one size split into several groups (a 1 MiB working-set
+    cap), stepped group-major (the default) and step-major, through graph
+    replays and eager steps: every world's state is identical bit for bit
+    to the unsplit step-major batch.  The fused kernels are forced, so the
+    tuner's choice cannot differ between the batches."""
+    monkeypatch.setenv("AUTO_HIP_BATCH_FFT_KERNELS", "fused")
+    worlds = W.uneven_worlds(seed=11, n=14, path="fft_fp32", sides=(128, 256))
+    worlds += W.uneven_worlds(seed=12, n=3, path="direct_bf16", sides=(128,))
+    runs = []
+    for cap, order in (("0", "step"), ("1", "group"), ("1", "step"), ("0", "group")):
+        monkeypatch.setenv("AUTO_HIP_BATCH_GROUP_MB", cap)
+        monkeypatch.setenv("AUTO_HIP_BATCH_ORDER", order)
+        b = Batch(worlds)
+        groups = b.info().fft_groups
+        b.step(103)  # one 100-step graph replay, then three eager steps
+        runs.append(([b.read_world(i) for i in range(len(worlds))], groups))
+    sizes = {(d.width, d.height) for d, _ in worlds[:14]}
+    assert runs[0][1] == len(sizes) and runs[1][1] > len(sizes) + 2
+    for states, _ in runs[1:]:
+        for i in range(len(worlds)):
+            assert np.array_equal(states[i], runs[0][0][i]), i
+    assert any(runs[0][0][i].std() > 0 for i in range(len(worlds)))
+
+
 def _bench_batch_vs_singles(O, worlds, b):
     """Every world of a stepped batch against a standalone FFT Sim and the
     FP64 oracle step from the same input state: (worst vs Sim, worst vs FP64)."""
