@@ -141,9 +141,14 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
                     const float2* twW);
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH, const int* ksrc,
-                     int n_kernels, bool diag = false, bool generic = false);
-// Whether a batch group's column (row) pass has a fused batch kernel besides the generic one.
+                     int n_kernels, bool diag = false, int variant = 0);
+// Column-pass kernels of a batch group (diagonal mode): the fused LDS-staged
+// batch kernel, the generic kernel, or the register-resident block kernel.
+enum { COL_FUSED = 0, COL_GENERIC = 1, COL_BLOCK = 2 };
+// Whether a batch group's column pass has the fused (block) kernel, and
+// whether its row pass has a fused batch kernel besides the generic one.
 bool batch_col_fused_ok(int W, int H);
+bool batch_col_block_ok(int W, int H);
 bool batch_row_fused_ok(int W, int H);
 // conv_out != nullptr: write kernel conv_k's potential there instead of updating the state.
 // spec_next != nullptr (update only): also write the half spectra of the updated

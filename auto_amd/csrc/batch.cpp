@@ -39,7 +39,7 @@ struct FftGroup {
     KernelParams* kp = nullptr;  // [n] growth per world
     float* dt = nullptr;         // [n]
     bool ready = false;          // spec holds the spectra of state
-    bool col_generic = false;    // tuned: the generic column kernel instead of the fused one
+    int col_variant = COL_FUSED; // tuned: which column-pass kernel (COL_FUSED, COL_GENERIC, COL_BLOCK)
     bool row_generic = false;    // tuned: the generic row kernel instead of the fused one
 };
 
@@ -225,9 +225,9 @@ bool group_major() {
 
 // One step of one FFT group: the column pass, then the row pass (which also
 // writes the next step's spectra).
-void launch_group_step(ah_batch* b, FftGroup& g, Profiler* p, bool col_generic, bool row_generic) {
+void launch_group_step(ah_batch* b, FftGroup& g, Profiler* p, int col_variant, bool row_generic) {
     launch_col_conv(b->stream, p, g.W, g.H, g.n, g.spec, g.spec, g.khat, g.kpack, g.twH, nullptr, 0, true,
-                    col_generic);
+                    col_variant);
     launch_row_c2r(b->stream, p, g.W, g.H, g.n, g.n, g.spec, g.state, g.state, nullptr, 0.0f, g.twW, nullptr, 0,
                    g.spec, AH_RULE_LENIA, g.kp, g.dt, row_generic);
 }
@@ -239,7 +239,7 @@ void launch_direct_step(ah_batch* b, Profiler* p) {
 
 void launch_step(ah_batch* b, Profiler* p) {
     launch_direct_step(b, p);
-    for (FftGroup& g : b->groups) launch_group_step(b, g, p, g.col_generic, g.row_generic);
+    for (FftGroup& g : b->groups) launch_group_step(b, g, p, g.col_variant, g.row_generic);
 }
 
 // `steps` steps of every world.  The worlds are independent, so the order
@@ -255,26 +255,30 @@ void launch_steps(ah_batch* b, uint32_t steps, Profiler* p) {
     }
     for (uint32_t i = 0; i < steps; ++i) launch_direct_step(b, p);
     for (FftGroup& g : b->groups)
-        for (uint32_t i = 0; i < steps; ++i) launch_group_step(b, g, p, g.col_generic, g.row_generic);
+        for (uint32_t i = 0; i < steps; ++i) launch_group_step(b, g, p, g.col_variant, g.row_generic);
 }
 
 // Times every admissible (column kernel, row kernel) pair of each size's
 // first FFT group on the batch (zero states at creation: the work does not
 // depend on the data), keeps the fastest for every group of that size and
 // records every candidate.
-// AUTO_HIP_BATCH_FFT_KERNELS=fused|generic skips the timing and takes the
-// fused kernels wherever they apply, or the generic ones everywhere (tests
-// use it to run each variant on the same batch).
+// AUTO_HIP_BATCH_FFT_KERNELS=fused|block|generic skips the timing and takes
+// the fused (or, for the column pass, block) kernels wherever they apply, or
+// the generic ones everywhere (tests use it to run each variant on the same
+// batch).
 void tune_groups(ah_batch* b) {
     b->n_group_tune = 0;
     if (b->groups.empty()) return;
     const char* force = std::getenv("AUTO_HIP_BATCH_FFT_KERNELS");
     if (force && *force && std::strcmp(force, "tune") != 0) {
-        const bool generic = std::strcmp(force, "generic") == 0;
-        if (!generic && std::strcmp(force, "fused") != 0)
-            throw Error{AH_ERR_INVALID, "AUTO_HIP_BATCH_FFT_KERNELS must be fused, generic or tune"};
+        const bool generic = std::strcmp(force, "generic") == 0, block = std::strcmp(force, "block") == 0;
+        if (!generic && !block && std::strcmp(force, "fused") != 0)
+            throw Error{AH_ERR_INVALID, "AUTO_HIP_BATCH_FFT_KERNELS must be fused, block, generic or tune"};
         for (FftGroup& g : b->groups) {
-            g.col_generic = generic || !batch_col_fused_ok(g.W, g.H);
+            g.col_variant = generic ? COL_GENERIC
+                            : block && batch_col_block_ok(g.W, g.H) ? COL_BLOCK
+                            : batch_col_fused_ok(g.W, g.H)          ? COL_FUSED
+                                                                    : COL_GENERIC;
             g.row_generic = generic || !batch_row_fused_ok(g.W, g.H);
         }
         return;
@@ -288,18 +292,26 @@ void tune_groups(ah_batch* b) {
             bool done = false;  // groups of one size take the first one's choice
             for (size_t e = 0; e < gi && !done; ++e)
                 if (b->groups[e].W == g.W && b->groups[e].H == g.H) {
-                    g.col_generic = b->groups[e].col_generic;
+                    g.col_variant = b->groups[e].col_variant;
                     g.row_generic = b->groups[e].row_generic;
                     done = true;
                 }
             if (done) continue;
             double best = 1e300;
-            for (int cg = 0; cg < 2; ++cg) {
-                if (!cg && !batch_col_fused_ok(g.W, g.H)) continue;
+            for (int cg : {COL_FUSED, COL_GENERIC, COL_BLOCK}) {
+                if (cg == COL_FUSED && !batch_col_fused_ok(g.W, g.H)) continue;
+                if (cg == COL_BLOCK && !batch_col_block_ok(g.W, g.H)) continue;
                 for (int rg = 0; rg < 2; ++rg) {
                     if (!rg && !batch_row_fused_ok(g.W, g.H)) continue;
-                    launch_group_step(b, g, nullptr, cg, rg);  // warm-up
-                    const int reps = 5;
+                    // Warm-up, then enough steps for about 2 ms of timing.
+                    launch_group_step(b, g, nullptr, cg, rg);
+                    AH_HIP(hipEventRecord(e0, b->stream));
+                    launch_group_step(b, g, nullptr, cg, rg);
+                    AH_HIP(hipEventRecord(e1, b->stream));
+                    AH_HIP(hipEventSynchronize(e1));
+                    float est = 0.0f;
+                    AH_HIP(hipEventElapsedTime(&est, e0, e1));
+                    const int reps = std::max(5, std::min(200, (int)(2.0f / std::max(est, 1e-3f))));
                     AH_HIP(hipEventRecord(e0, b->stream));
                     for (int r = 0; r < reps; ++r) launch_group_step(b, g, nullptr, cg, rg);
                     AH_HIP(hipEventRecord(e1, b->stream));
@@ -312,7 +324,7 @@ void tune_groups(ah_batch* b) {
                             ah_batch_group_tune{(uint32_t)g.W, (uint32_t)g.H, (uint32_t)cg, (uint32_t)rg, per};
                     if (per < best) {
                         best = per;
-                        g.col_generic = cg;
+                        g.col_variant = cg;
                         g.row_generic = rg;
                     }
                 }

@@ -815,6 +815,160 @@ k_col_block4096(ColArgs a) {
     col_task_store<NCOL>(a, kb, cbase, q, v);
 }
 
+// Column pass for H = 256..2048 in the same register-resident form as the
+// H = 4096 kernel above, for one kernel per channel (batched same-size
+// worlds in diagonal mode, or a single world with one kernel).  H = 16 S,
+// S = 16 Q (Q = 1, 2, 4, 8): n = t + S j, k = k1 + 16 k2;
+//   phase 1, thread (c, t): rows t + S j, j < 16; DFT-16 over j -> k1;
+//            twiddle W_H^(t k1).  Exchange 1 through LDS (barrier).
+//   phase 2, thread (c, k1, m), m < Q: t = m + Q n, n < 16; DFT-16 over n
+//            -> k2a; twiddle W_S^(m k2a).  Exchange 2 among the Q lanes of
+//            (c, k1), one wave, no barrier: thread m' takes k2a = m' 16/Q + i.
+//   phase 3: DFT-Q over m -> k2b for each i: the thread holds
+//            k = k1 + 16 (m' 16/Q + i) + 256 k2b, multiplies by K-hat and
+//            runs the inverse back through the same steps.
+// One workgroup takes NCOL columns of a 4-column block: its loads and
+// stores are whole 32-byte rows of the block when NCOL = 4.
+namespace cblk {
+template <int H>
+struct Dims {
+    static constexpr int S = H / 16, Q = S / 16, P16 = 16 / Q;
+    // LDS float2 per (c, k1) region and the exchange-2 index a Q + m + a/P16:
+    // both chosen so every phase's LDS access is free of bank conflicts.
+    static constexpr int KS = H == 256 ? 17 : H == 512 ? 34 : H == 1024 ? 76 : 152;
+    static constexpr int NP = H / 2 + 1;
+    static_assert(H >= 256 && H <= 2048 && (H & (H - 1)) == 0, "column length");
+};
+template <int H, int NCOL>
+struct Plan {
+    static constexpr int THREADS = Dims<H>::S * NCOL;
+    static constexpr int CS = 16 * Dims<H>::KS + (NCOL == 4 ? 4 : 8);
+    static constexpr size_t LDS_BYTES = (size_t)NCOL * CS * sizeof(float2);
+};
+__device__ __forceinline__ constexpr int x2(int a, int m, int Q, int P16) { return a * Q + m + a / P16; }
+}  // namespace cblk
+
+template <int Q, int SIGN>
+__device__ __forceinline__ void dft_q(float2* v) {
+    if constexpr (Q == 2) dft2<SIGN>(v);
+    else if constexpr (Q == 4) dft4<SIGN>(v);
+    else if constexpr (Q == 8) dft8<SIGN>(v);
+}
+
+template <int H, int NCOL>
+__global__ void __launch_bounds__((cblk::Plan<H, NCOL>::THREADS)) k_col_blk(ColArgs a) {
+    using D = cblk::Dims<H>;
+    using P = cblk::Plan<H, NCOL>;
+    constexpr int S = D::S, Q = D::Q, P16 = D::P16, KS = D::KS, NP = D::NP, CS = P::CS;
+    constexpr int LGC = NCOL == 4 ? 2 : NCOL == 2 ? 1 : 0;
+    extern __shared__ float2 lds[];
+    int kb, cbase;
+    col_task_id<NCOL>(xcd_remap(blockIdx.x, gridDim.x), kb, cbase);
+    const int world = blockIdx.y;
+    const size_t chan = (size_t)(a.W >> 1) * H;
+    const int tid = threadIdx.x;
+    const int c1 = tid & (NCOL - 1), t1 = tid >> LGC;            // phase 1
+    const int c2 = tid / S, u = tid & (S - 1), k1 = u / Q, m = u & (Q - 1);  // phases 2, 3
+    float2* region = lds + c2 * CS + k1 * KS;
+    const float2* src = a.spec_in + world * chan + (size_t)kb * H * BC + cbase;
+    float2 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = src[(size_t)(t1 + S * j) * BC + c1];
+    // Phase 1.
+    dft16<-1>(v);
+    twiddle16<-1>(v, a.twH[t1]);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lds[c1 * CS + r * KS + t1] = v[r];
+    // K-hat of this thread's phase-3 frequencies (real, even in ky: row k
+    // takes p = min(k, H - k)), behind the rows.
+    const int col = kb * BC + cbase + c2;
+    float kf[16];
+    if (col != 0) {
+        const float* kh = a.khat + ((size_t)world * (a.W >> 3) + kb) * (size_t)NP * BC + cbase + c2;
+#pragma unroll
+        for (int i = 0; i < P16; ++i)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                const int k = k1 + 16 * (m * P16 + i) + 256 * b;
+                kf[i * Q + b] = kh[(size_t)min(k, H - k) * BC];
+            }
+    }
+    lds_sync();
+    // Phase 2.
+#pragma unroll
+    for (int n = 0; n < 16; ++n) v[n] = region[m + Q * n];
+    dft16<-1>(v);
+    if constexpr (Q > 1) {
+        twiddle16<-1>(v, a.twH[16 * m]);
+        wave_lds_order();  // this wave's phase-2 reads of `region` are done
+#pragma unroll
+        for (int r = 0; r < 16; ++r) region[cblk::x2(r, m, Q, P16)] = v[r];
+        wave_lds_order();
+#pragma unroll
+        for (int i = 0; i < P16; ++i)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) v[i * Q + b] = region[cblk::x2(m * P16 + i, b, Q, P16)];
+        // Phase 3.
+#pragma unroll
+        for (int i = 0; i < P16; ++i) dft_q<Q, -1>(&v[i * Q]);
+    }
+    if (kb == 0 && cbase == 0) {
+        // Workgroup-uniform: column 0 packs the real bins kx = 0 and W/2.
+        // D[k] = A C[k] + B conj C[-k] with (A, B) = kpack[min(k, H - k)];
+        // C[-k] is held by another thread: column 0 goes through LDS in
+        // natural order (its regions hold 16 KS >= H entries).
+        lds_sync();  // column 0's exchange-2 reads are done
+        if (c2 == 0) {
+#pragma unroll
+            for (int i = 0; i < P16; ++i)
+#pragma unroll
+                for (int b = 0; b < Q; ++b) lds[k1 + 16 * (m * P16 + i) + 256 * b] = v[i * Q + b];
+        }
+        lds_sync();
+        if (c2 == 0) {
+            const float2* kp = a.kpack + (size_t)world * NP;
+#pragma unroll
+            for (int i = 0; i < P16; ++i)
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    const int k = k1 + 16 * (m * P16 + i) + 256 * b, kn = (H - k) & (H - 1);
+                    const float2 cn = lds[kn], ab = kp[min(k, kn)], x = v[i * Q + b];
+                    v[i * Q + b] = cadd(make_float2(ab.x * x.x, ab.x * x.y), make_float2(ab.y * cn.x, -ab.y * cn.y));
+                }
+        }
+        lds_sync();  // partner reads are done before the region is reused
+    }
+    if (col != 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = make_float2(kf[r] * v[r].x, kf[r] * v[r].y);
+    }
+    // Inverse: phase 3, exchange 2, phase 2, exchange 1, phase 1.
+    if constexpr (Q > 1) {
+#pragma unroll
+        for (int i = 0; i < P16; ++i) dft_q<Q, 1>(&v[i * Q]);
+#pragma unroll
+        for (int i = 0; i < P16; ++i)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) region[cblk::x2(m * P16 + i, b, Q, P16)] = v[i * Q + b];
+        wave_lds_order();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = region[cblk::x2(r, m, Q, P16)];
+        twiddle16<1>(v, a.twH[16 * m]);
+    }
+    dft16<1>(v);
+    wave_lds_order();  // the exchange reads are done before the region is rewritten
+#pragma unroll
+    for (int n = 0; n < 16; ++n) region[m + Q * n] = v[n];
+    lds_sync();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = lds[c1 * CS + r * KS + t1];
+    twiddle16<1>(v, a.twH[t1]);
+    dft16<1>(v);
+    float2* dst = a.spec_out + world * chan + (size_t)kb * H * BC + cbase;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) dst[(size_t)(t1 + S * j) * BC + c1] = v[j];
+}
+
 // Fused column pass for batched same-size worlds (diagonal mode: world c's
 // kernel reads and writes channel c), H = 64..2048.  A workgroup takes CPW
 // columns of one block (all four for H <= 1024) with TPC = H/8 threads per
@@ -1413,11 +1567,12 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
 }
 
 bool batch_col_fused_ok(int W, int H) { return H >= 64 && H <= 2048 && W >= 16; }
+bool batch_col_block_ok(int W, int H) { return H >= 256 && H <= 2048 && W >= 16; }
 bool batch_row_fused_ok(int W, int H) { return W >= 128 && W <= 4096 && row_rows(W, H) * W == 4096; }
 
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH,
-                     const int* ksrc, int n_kernels, bool diag, bool generic) {
+                     const int* ksrc, int n_kernels, bool diag, int variant) {
     ColArgs a;
     a.W = W;
     a.H = H;
@@ -1455,7 +1610,22 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
-    if (diag && !generic && batch_col_fused_ok(W, H)) {
+    if (diag && variant == COL_BLOCK && batch_col_block_ok(W, H)) {
+        auto block = [&](auto kernel, int threads, size_t bytes, int ncol) {
+            set_lds(kernel, bytes);
+            hipLaunchKernelGGL(kernel, dim3((W / 8) * (4 / ncol), channels), dim3(threads), bytes, s, a);
+        };
+        using namespace cblk;
+        switch (H) {
+            case 256: block(k_col_blk<256, 4>, Plan<256, 4>::THREADS, Plan<256, 4>::LDS_BYTES, 4); break;
+            case 512: block(k_col_blk<512, 4>, Plan<512, 4>::THREADS, Plan<512, 4>::LDS_BYTES, 4); break;
+            case 1024: block(k_col_blk<1024, 4>, Plan<1024, 4>::THREADS, Plan<1024, 4>::LDS_BYTES, 4); break;
+            default: block(k_col_blk<2048, 2>, Plan<2048, 2>::THREADS, Plan<2048, 2>::LDS_BYTES, 2); break;
+        }
+        AH_HIP(hipGetLastError());
+        return;
+    }
+    if (diag && variant == COL_FUSED && batch_col_fused_ok(W, H)) {
         auto fused = [&](auto kernel, int threads, int cpw) {
             const size_t flds = padded((size_t)cpw * H) * sizeof(float2);
             set_lds(kernel, flds);

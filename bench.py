@@ -213,7 +213,7 @@ def batch_profile_run(seed: int, steps: int, n: int = 384) -> dict:
     t = time.perf_counter() - t0
     return {"worlds": n, "fft_groups": info.fft_groups, "steps": steps, "ms_per_step": 1e3 * t / steps,
             "value": info.total_cells * steps / t, "unit": "cell-updates/s",
-            "fft_group_tune": [{"world": [g.width, g.height], "column": "generic" if g.column else "fused",
+            "fft_group_tune": [{"world": [g.width, g.height], "column": ("fused", "generic", "block")[g.column],
                                 "row": "generic" if g.row else "fused", "ms_per_step": g.ms_per_step}
                                for g in info.group_tune[:info.n_group_tune]]}
 
@@ -243,7 +243,7 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
            "tile": [info.tile_cols, info.tile_rows], "grid": info.grid, "tiles": info.ntiles,
            "autotune": [{"tile": [info.tune[i].tile_cols, info.tune[i].tile_rows], "grid": info.tune[i].grid,
                          "ms_per_step": info.tune[i].ms_per_step} for i in range(info.n_tune)],
-           "fft_group_tune": [{"world": [t.width, t.height], "column": "generic" if t.column else "fused",
+           "fft_group_tune": [{"world": [t.width, t.height], "column": ("fused", "generic", "block")[t.column],
                                "row": "generic" if t.row else "fused", "ms_per_step": t.ms_per_step}
                               for t in info.group_tune[:info.n_group_tune]],
            "graph_steps": graph_steps,

@@ -159,27 +159,29 @@ def _world_wh(w, h, radius, seed_dt=0.1):
                   init=W.INIT_PATCH, init_size=min(w, h) // 4)
 
 
-@pytest.mark.parametrize("kernels", ["fused", "generic", "tune"])
+@pytest.mark.parametrize("kernels", ["fused", "block", "generic", "tune"])
 def test_batch_fft_group_shapes(O, monkeypatch, kernels):
     """FFT groups at the edges of the batch kernels' ranges: heights 64 and
     2048 (the fused column pass with four and two columns per workgroup),
-    512 and 1024 (the fused column pass at the benchmark batch's two largest
-    sides), height 4096 (the generic column pass in diagonal mode), width 64
-    (the generic row pass) and widths 128, 512, 1024 and 2048 (the fused row
-    pass), two worlds per group.  Run three ways: the fused kernels wherever
-    they apply, the generic kernels everywhere, and the tuner's choice per
-    group.  Every world vs a standalone Sim (within 1e-5) and vs the FP64
-    step (within 1e-3) from the same input."""
+    256, 512, 1024 and 2048 (the block column pass, Q = 1, 2, 4, 8), height
+    4096 (the generic column pass in diagonal mode), width 64 (the generic
+    row pass) and widths 128, 256, 512, 1024 and 2048 (the fused row pass),
+    two worlds per group.  Run four ways: the fused kernels wherever they
+    apply, the block column kernel wherever it applies, the generic kernels
+    everywhere, and the tuner's choice per size.  Every world vs a
+    standalone Sim (within 1e-5) and vs the FP64 step (within 1e-3) from the
+    same input."""
     monkeypatch.setenv("AUTO_HIP_BATCH_FFT_KERNELS", kernels)
     shapes = [(64, 64, 8), (2048, 2048, 20), (64, 4096, 12), (2048, 64, 12), (128, 2048, 16),
-              (512, 512, 18), (1024, 1024, 24)]
+              (512, 512, 18), (1024, 1024, 24), (256, 256, 14), (64, 512, 10)]
     worlds = [(_world_wh(w, h, r), 10 * i + j) for i, (w, h, r) in enumerate(shapes) for j in range(2)]
     b = Batch(worlds)
     info = b.info()
     assert info.fft_worlds == len(worlds) and info.fft_groups == len(shapes)
     if kernels == "tune":
         # Every group timed at least once (the generic pair always applies),
-        # and a group where both fused kernels apply times all four pairs.
+        # and a group where every batch kernel applies times all six pairs
+        # (column: 0 fused, 1 generic, 2 block; row: 0 fused, 1 generic).
         seen = {}
         for k in range(info.n_group_tune):
             t = info.group_tune[k]
@@ -187,7 +189,7 @@ def test_batch_fft_group_shapes(O, monkeypatch, kernels):
             seen.setdefault((t.width, t.height), set()).add((t.column, t.row))
         assert set(seen) == {(w, h) for w, h, _ in shapes}
         assert all((1, 1) in v for v in seen.values())
-        assert seen[(512, 512)] == {(0, 0), (0, 1), (1, 0), (1, 1)}
+        assert seen[(512, 512)] == {(c, r) for c in (0, 1, 2) for r in (0, 1)}
     else:
         assert info.n_group_tune == 0
     b.step(2)
@@ -273,10 +275,14 @@ def test_benchmark_batch_384_auto(O):
     assert worst_sim < 1e-5 and worst64 <= 1e-3
 
 
-def test_benchmark_batch_384_graph_replays_equal_eager():
+@pytest.mark.parametrize("kernels", ["fused", "block"])
+def test_benchmark_batch_384_graph_replays_equal_eager(monkeypatch, kernels):
     """The benchmark batch stepped by graph replays (100 steps each) equals
     the same batch stepped by single eager launches, bit for bit, after the
-    first replay and after a later one with eager work in between."""
+    first replay and after a later one with eager work in between.  The
+    column kernels are forced (two batches tuned apart may time their
+    candidates differently and pick different kernels)."""
+    monkeypatch.setenv("AUTO_HIP_BATCH_FFT_KERNELS", kernels)
     worlds = W.uneven_worlds(seed=1, n=384, path="auto")
     a, b = Batch(worlds), Batch(worlds)
     for rnd in range(2):
