@@ -145,10 +145,12 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
 // Column-pass kernels of a batch group (diagonal mode): the fused LDS-staged
 // batch kernel, the generic kernel, or the register-resident block kernel.
 enum { COL_FUSED = 0, COL_GENERIC = 1, COL_BLOCK = 2 };
-// Whether a batch group's column pass has the fused (block) kernel, and
-// whether its row pass has a fused batch kernel besides the generic one.
+// Whether a batch group's column pass has the fused kernel; whether the
+// block column kernel applies (every single world of that height, and batch
+// groups that choose it); whether a batch group's row pass has a fused
+// kernel besides the generic one.
 bool batch_col_fused_ok(int W, int H);
-bool batch_col_block_ok(int W, int H);
+bool col_blk_ok(int W, int H);
 bool batch_row_fused_ok(int W, int H);
 // conv_out != nullptr: write kernel conv_k's potential there instead of updating the state.
 // spec_next != nullptr (update only): also write the half spectra of the updated

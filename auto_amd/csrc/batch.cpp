@@ -276,7 +276,7 @@ void tune_groups(ah_batch* b) {
             throw Error{AH_ERR_INVALID, "AUTO_HIP_BATCH_FFT_KERNELS must be fused, block, generic or tune"};
         for (FftGroup& g : b->groups) {
             g.col_variant = generic ? COL_GENERIC
-                            : block && batch_col_block_ok(g.W, g.H) ? COL_BLOCK
+                            : block && col_blk_ok(g.W, g.H) ? COL_BLOCK
                             : batch_col_fused_ok(g.W, g.H)          ? COL_FUSED
                                                                     : COL_GENERIC;
             g.row_generic = generic || !batch_row_fused_ok(g.W, g.H);
@@ -300,7 +300,7 @@ void tune_groups(ah_batch* b) {
             double best = 1e300;
             for (int cg : {COL_FUSED, COL_GENERIC, COL_BLOCK}) {
                 if (cg == COL_FUSED && !batch_col_fused_ok(g.W, g.H)) continue;
-                if (cg == COL_BLOCK && !batch_col_block_ok(g.W, g.H)) continue;
+                if (cg == COL_BLOCK && !col_blk_ok(g.W, g.H)) continue;
                 for (int rg = 0; rg < 2; ++rg) {
                     if (!rg && !batch_row_fused_ok(g.W, g.H)) continue;
                     // Warm-up, then enough steps for about 2 ms of timing.

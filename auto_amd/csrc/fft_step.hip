@@ -20,13 +20,14 @@
 // final radix 2 or 4 stage), the same stage recurrence as fft_pass
 // generalised to radix R: thread j, k = j mod Ls, loads v[r] = x[j + r*M/R],
 // twiddles by w^(r*k) with w = exp(sign*2*pi*i/(Ls*R)), runs a radix-R DFT in
-// registers and stores at (j-k)*R + k + r*Ls.  The column kernel at H = 4096
-// runs radix-16 stages in registers instead (k_col_block4096).
+// registers and stores at (j-k)*R + k + r*Ls.  The column kernels at
+// H = 256..4096 run radix-16 stages in registers instead (k_col_blk,
+// k_col_block4096), with LDS only to change hands between them.
 //
 // Twiddle precision: the LDS stages (twiddles()) take the hardware's f32
-// sine and cosine of the exact dyadic angle k*step/n turns; the column
-// kernels at H = 4096 and the real-FFT twist factors read FP64-built tables
-// (init.hip) and form powers by products.  One-step error vs FP64 at 4096^2
+// sine and cosine of the exact dyadic angle k*step/n turns; the register-
+// resident column kernels and the real-FFT twist factors read FP64-built
+// tables (init.hip) and form powers by products.  One-step error vs FP64 at 4096^2
 // stays near 2e-6 either way (bar: 1e-3).
 //
 // Half-spectrum layout (complex64): spec[kb][y][c], kx = 4*kb + c.  Bin kx=0
@@ -828,7 +829,10 @@ k_col_block4096(ColArgs a) {
 //            k = k1 + 16 (m' 16/Q + i) + 256 k2b, multiplies by K-hat and
 //            runs the inverse back through the same steps.
 // One workgroup takes NCOL columns of a 4-column block: its loads and
-// stores are whole 32-byte rows of the block when NCOL = 4.
+// stores are whole 32-byte rows of the block when NCOL = 4.  Grid y is the
+// world (diagonal mode) or the kernel, which reads channel ksrc[kernel]
+// (several kernels of one channel each transform it: no workgroup holds
+// more than one column's values).
 namespace cblk {
 template <int H>
 struct Dims {
@@ -864,13 +868,14 @@ __global__ void __launch_bounds__((cblk::Plan<H, NCOL>::THREADS)) k_col_blk(ColA
     extern __shared__ float2 lds[];
     int kb, cbase;
     col_task_id<NCOL>(xcd_remap(blockIdx.x, gridDim.x), kb, cbase);
-    const int world = blockIdx.y;
+    const int world = blockIdx.y;  // world, or kernel: K-hat, kpack and output
+    const int src_ch = a.diag ? world : a.ksrc[world];
     const size_t chan = (size_t)(a.W >> 1) * H;
     const int tid = threadIdx.x;
     const int c1 = tid & (NCOL - 1), t1 = tid >> LGC;            // phase 1
     const int c2 = tid / S, u = tid & (S - 1), k1 = u / Q, m = u & (Q - 1);  // phases 2, 3
     float2* region = lds + c2 * CS + k1 * KS;
-    const float2* src = a.spec_in + world * chan + (size_t)kb * H * BC + cbase;
+    const float2* src = a.spec_in + src_ch * chan + (size_t)kb * H * BC + cbase;
     float2 v[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = src[(size_t)(t1 + S * j) * BC + c1];
@@ -1567,7 +1572,7 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
 }
 
 bool batch_col_fused_ok(int W, int H) { return H >= 64 && H <= 2048 && W >= 16; }
-bool batch_col_block_ok(int W, int H) { return H >= 256 && H <= 2048 && W >= 16; }
+bool col_blk_ok(int W, int H) { return H >= 256 && H <= 2048 && W >= 16; }
 bool batch_row_fused_ok(int W, int H) { return W >= 128 && W <= 4096 && row_rows(W, H) * W == 4096; }
 
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
@@ -1610,17 +1615,24 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     const size_t lds = padded(elems) * sizeof(float2);
     Launch L(p, s, "col_conv");
     set_lds(k_col_conv<8>, lds);
-    if (diag && variant == COL_BLOCK && batch_col_block_ok(W, H)) {
-        auto block = [&](auto kernel, int threads, size_t bytes, int ncol) {
+    if (col_blk_ok(W, H) && (!diag || variant == COL_BLOCK)) {
+        // Four columns (one block) per workgroup; grid y is the world in
+        // diagonal mode, else the kernel.  Measured per column pass against
+        // the kernels below: multichannel 1024^2 (9 kernels) 28.3 -> 18.4 us,
+        // one kernel at 256^2 7.6 -> 5.4, 1024^2 10.8 -> 8.4, 2048^2
+        // 19.3 -> 13.5 us, SmoothLife 2048^2 28.5 -> 18.7 us; two columns per
+        // workgroup came within 0.3 us, one column per workgroup behind.
+        const int gy = diag ? channels : n_kernels;
+        auto block = [&](auto kernel, int threads, size_t bytes) {
             set_lds(kernel, bytes);
-            hipLaunchKernelGGL(kernel, dim3((W / 8) * (4 / ncol), channels), dim3(threads), bytes, s, a);
+            hipLaunchKernelGGL(kernel, dim3(W / 8, gy), dim3(threads), bytes, s, a);
         };
         using namespace cblk;
         switch (H) {
-            case 256: block(k_col_blk<256, 4>, Plan<256, 4>::THREADS, Plan<256, 4>::LDS_BYTES, 4); break;
-            case 512: block(k_col_blk<512, 4>, Plan<512, 4>::THREADS, Plan<512, 4>::LDS_BYTES, 4); break;
-            case 1024: block(k_col_blk<1024, 4>, Plan<1024, 4>::THREADS, Plan<1024, 4>::LDS_BYTES, 4); break;
-            default: block(k_col_blk<2048, 2>, Plan<2048, 2>::THREADS, Plan<2048, 2>::LDS_BYTES, 2); break;
+            case 256: block(k_col_blk<256, 4>, Plan<256, 4>::THREADS, Plan<256, 4>::LDS_BYTES); break;
+            case 512: block(k_col_blk<512, 4>, Plan<512, 4>::THREADS, Plan<512, 4>::LDS_BYTES); break;
+            case 1024: block(k_col_blk<1024, 4>, Plan<1024, 4>::THREADS, Plan<1024, 4>::LDS_BYTES); break;
+            default: block(k_col_blk<2048, 4>, Plan<2048, 4>::THREADS, Plan<2048, 4>::LDS_BYTES); break;
         }
         AH_HIP(hipGetLastError());
         return;

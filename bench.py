@@ -331,7 +331,8 @@ def time_rule(rule: str, side: int, steps: int, seed: int):
     r = timed_blocks(sim.step, sim.wait, steps, d.width * d.height)
     return {"value": r["value"], "unit": "cell-updates/s", "ms_per_step": r["ms_per_step"],
             "p10": r["p10"], "p90": r["p90"],
-            "workload": f"{rule} {d.width}x{d.height}, reference parameters, blocks of {steps} steps"}
+            "workload": f"{rule} {d.width}x{d.height}, reference parameters, blocks of {steps} steps",
+            "kernels": event_window(sim)}
 
 
 def time_multichannel(path: str, steps: int, seed: int):
@@ -344,7 +345,7 @@ def time_multichannel(path: str, steps: int, seed: int):
     r = timed_blocks(sim.step, sim.wait, steps, cells)
     return d, sim, {"value": r["value"], "unit": "cell-updates/s (3 channels per cell)",
                     "channel_cell_updates_per_s": 3 * r["value"], "ms_per_step": r["ms_per_step"],
-                    "p10": r["p10"], "p90": r["p90"]}
+                    "p10": r["p10"], "p90": r["p90"], "kernels": event_window(sim)}
 
 
 def multichannel_errors(seed: int, steps: int = 500, every: int = 50):
