@@ -1276,8 +1276,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
 // Pass 3 fused with the next pass 1 for the Lenia rule with one kernel per
 // destination channel (a single world, or batched same-size worlds in
 // diagonal mode): k_row_c2r<8, true, LENIA> with the middle in registers.
-// Rows of M complex (W = 2M, 128 <= W <= 4096), 2048/M rows per workgroup
-// of 256 threads, TPR = M/8 threads per row.  The inverse FFT's last stage
+// Rows of M complex (W = 2M, 128 <= W <= 8192), 2048/M rows per workgroup
+// of 256 threads (one row of 512 threads at W = 8192), TPR = M/8 threads
+// per row.  The inverse FFT's last stage
 // (radix RL = 8, 4 or 2, span L = M/RL) leaves thread t of a row holding
 // positions t + TPR m, m = 0..7; complex position p is cells 2p and 2p+1,
 // so the thread applies the growth to exactly those cells; and the forward
@@ -1285,12 +1286,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_row_c2r(RowArgs a) {
 // set.  The LDS round trips of the epilogue and of that stage disappear.
 template <int M>
 struct RowPlan {
-    static constexpr int TPR = M / 8, ROWS = 2048 / M;
-    static constexpr int N8 = M >= 512 ? 3 : 2;                   // radix-8 stages with 8^N8 <= M
-    static constexpr int RLAST = M / (N8 == 3 ? 512 : 64);        // 1, 2 or 4
+    static constexpr int TPR = M / 8, ROWS = M >= 2048 ? 1 : 2048 / M, THREADS = TPR * ROWS;
+    static constexpr int N8 = M >= 4096 ? 4 : M >= 512 ? 3 : 2;   // radix-8 stages with 8^N8 <= M
+    static constexpr int RLAST = M / (N8 == 4 ? 4096 : N8 == 3 ? 512 : 64);  // 1, 2 or 4
     static constexpr int RL = RLAST == 1 ? 8 : RLAST, L = M / RL, NBL = 8 / RL;
     static constexpr int NLDS = RLAST == 1 ? N8 - 1 : N8;         // inverse stages through LDS
-    static_assert(M >= 64 && M <= 2048 && (M & (M - 1)) == 0, "row length");
+    static_assert(M >= 64 && M <= 4096 && (M & (M - 1)) == 0, "row length");
 };
 
 // Pass-3 input of the workgroup's rows, twisted on the way in: thread t of
@@ -1368,7 +1369,7 @@ __device__ __forceinline__ void inverse_row(float2* lds, const RowArgs& a, const
 // of the one channel, whose potentials the transition combines).  Kernel
 // k's values stay in registers while the next kernel's row goes through LDS.
 template <int M, bool DIAG, int RULE, bool MULTI>
-__global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
+__global__ void __launch_bounds__(RowPlan<M>::THREADS) k_row_fused(RowArgs a) {
     using P = RowPlan<M>;
     extern __shared__ float2 lds[];
     constexpr int W = 2 * M;
@@ -1475,7 +1476,7 @@ __global__ void __launch_bounds__(256) k_row_fused(RowArgs a) {
     for (int s = 1; s < P::N8; ++s, Ls *= 8) stage<8, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
     if constexpr (P::RLAST == 4) stage<4, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
     if constexpr (P::RLAST == 2) stage<2, 8, -1>(lds, M, P::ROWS, Ls, a.twW, W);
-    store_half_spectrum(lds, M, P::ROWS, H, y0, a.twW, a.spec_next + (size_t)ch * M * H, 256);
+    store_half_spectrum(lds, M, P::ROWS, H, y0, a.twW, a.spec_next + (size_t)ch * M * H, P::THREADS);
 }
 
 template <typename K>
@@ -1573,7 +1574,9 @@ void launch_row_r2c(hipStream_t s, Profiler* p, int W, int H, int channels, cons
 
 bool batch_col_fused_ok(int W, int H) { return H >= 64 && H <= 2048 && W >= 16; }
 bool col_blk_ok(int W, int H) { return H >= 256 && H <= 2048 && W >= 16; }
-bool batch_row_fused_ok(int W, int H) { return W >= 128 && W <= 4096 && row_rows(W, H) * W == 4096; }
+bool batch_row_fused_ok(int W, int H) {
+    return W >= 128 && W <= 8192 && row_rows(W, H) * W == std::max(4096, W);
+}
 
 void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, const float2* spec_in,
                      float2* spec_out, const float* khat, const float2* kpack, const float2* twH,
@@ -1707,20 +1710,21 @@ void launch_row_c2r(hipStream_t s, Profiler* p, int W, int H, int channels, int 
         set_lds(kernel, lds);
         hipLaunchKernelGGL(kernel, grid, block, lds, s, a);
     };
-    const bool fusable = !generic && W >= 128 && W <= 4096 && a.rows * W == 4096 && !conv_out && a.spec_next;
+    const bool fusable = !generic && batch_row_fused_ok(W, H) && !conv_out && a.spec_next;
     auto fused = [&](auto diag_tag, auto rule_tag, auto multi_tag) {
         constexpr bool D = decltype(diag_tag)::value;
         constexpr int R = decltype(rule_tag)::value;
         constexpr bool MU = decltype(multi_tag)::value;
         Launch L(p, s, "row_c2r_update_r2c");
-        auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(256), lds, s, a); };
+        auto go_fused = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, dim3(elems / 8), lds, s, a); };
         switch (W) {
             case 128: go_fused(k_row_fused<64, D, R, MU>); break;
             case 256: go_fused(k_row_fused<128, D, R, MU>); break;
             case 512: go_fused(k_row_fused<256, D, R, MU>); break;
             case 1024: go_fused(k_row_fused<512, D, R, MU>); break;
             case 2048: go_fused(k_row_fused<1024, D, R, MU>); break;
-            default: go_fused(k_row_fused<2048, D, R, MU>); break;
+            case 4096: go_fused(k_row_fused<2048, D, R, MU>); break;
+            default: go_fused(k_row_fused<4096, D, R, MU>); break;
         }
     };
     const std::false_type no;
