@@ -114,14 +114,13 @@ void launch_twiddles(hipStream_t s, Profiler* p, float2* tw, int n);
 // Normalised FP64 taps of one kernel in a (2*reach+1)^2 box; writes taps and raw sum.
 void launch_kernel_taps64(hipStream_t s, Profiler* p, const ah_kernel& k, int reach, double* taps, double* sum);
 // How the column pass of a world reads its real kernel spectrum:
-//   KHAT_BLOCK   khat[kb][p][c], p in [0, H/2] (the generic column kernels);
-//   KHAT_COLUMN  khat[kx][p] (the streamed one-column kernel);
+//   KHAT_BLOCK   khat[kb][p][c], p in [0, H/2] (every column kernel but one);
 //   KHAT_DIGITS  khat[kx][digits(p)], p in [0, H/2], digits(p) = (p >> 8) * 256
 //                + (p & 15) * 16 + ((p >> 4) & 15) (H = 4096: the block column
 //                kernel, whose thread (k1, m) holds k = k1 + 16 m + 256 r; the
 //                swapped digits make a wave's loads contiguous, for p = k and
 //                for p = H - k alike).  kx = 0 and W/2 always go to kpack.
-enum { KHAT_BLOCK = 0, KHAT_COLUMN = 1, KHAT_DIGITS = 2 };
+enum { KHAT_BLOCK = 0, KHAT_DIGITS = 1 };
 // khat/kpack for the FFT path from FP64 taps (separable cosine transform); T is scratch.
 void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const double* taps, const double* cosW,
                    const double* cosH, double* T, float* khat, float2* kpack, int layout);
@@ -129,7 +128,6 @@ void launch_khat64(hipStream_t s, Profiler* p, int W, int H, int reach, const do
 // khat per kernel in it.
 int khat_layout(int W, int H, int channels, int n_kernels, const int* ksrc);
 size_t khat_floats(int W, int H, int layout);
-bool col_stream_ok(int W, int H, int channels, int n_kernels, const int* ksrc);
 void launch_seed(hipStream_t s, Profiler* p, const ah_lenia_desc& d, uint32_t seed, const uint32_t* corners,
                  float* state);
 int stats_blocks(size_t cells);

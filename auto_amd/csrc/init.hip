@@ -145,7 +145,7 @@ __global__ void k_khat_rows64(int W, int reach, const double* taps, const double
 
 // khat[kb][p][c] = (sum_dy T[dy][kx] cos(2 pi p dy / H)) / (W*H), p in [0, H/2];
 // kx = 0 and kx = W/2 go to kpack[p] = ((K0+KN)/2, (K0-KN)/2).  `layout`
-// KHAT_COLUMN stores khat[kx][p], KHAT_DIGITS khat[kx][digits(p)].
+// KHAT_DIGITS stores khat[kx][digits(p)] instead.
 __global__ void k_khat64(int W, int H, int reach, const double* T, const double* cosH, float* khat, float2* kpack,
                          int layout) {
     const int nk = W / 2 + 1, np = H / 2 + 1;
@@ -166,10 +166,6 @@ __global__ void k_khat64(int W, int H, int reach, const double* T, const double*
                     cosH[(p * dy) & (H - 1)];
         accN *= scale;
         kpack[p] = make_float2((float)(0.5 * (acc + accN)), (float)(0.5 * (acc - accN)));
-        return;
-    }
-    if (layout == KHAT_COLUMN) {
-        khat[(size_t)kx * np + p] = (float)acc;
         return;
     }
     if (layout == KHAT_DIGITS) {
