@@ -643,6 +643,224 @@ k_col_block4096(ColArgs in) {
     col_task_store<NCOL>(a, kb, cbase, q, v);
 }
 
+// Column pass for H = 8192: the H = 4096 kernel's form with a radix-32
+// first phase.  n = t + 256 j (t < 256, j < 32), k = k1 + 32 k2;
+//   phase 1, thread (c, t): rows t + 256 j; DFT-32 over j -> k1; twiddle
+//            W_8192^(t k1).  Exchange 1 through LDS (barrier).
+//   phases 2 and 3, thread (c, k1lo, m) for the two values k1 = k1lo and
+//            k1lo + 16: the 256-point transform over t exactly as at
+//            H = 4096 (DFT-16, twiddle W_256^(m k2a), 16 x 16 transpose in
+//            one wave, DFT-16), holding k = k1 + 32 m + 512 r.
+// 256 threads and 70 KiB of LDS per column (NCOL columns per workgroup);
+// K-hat in digit order
+// (KHAT_DIGITS at H = 8192: digits(p) = (p >> 9) 512 + (p & 31) 16 +
+// ((p >> 5) & 15), so a wave's K-hat loads are contiguous).
+namespace c8k {
+constexpr int H = 8192;
+constexpr int KS = 272;        // LDS float2 per (c, k1) region
+constexpr int NP = H / 2 + 1;
+template <int NCOL>
+struct Plan {
+    static constexpr int THREADS = 256 * NCOL;
+    static constexpr int CS = 32 * KS + 16;
+    static constexpr size_t LDS_BYTES = (size_t)NCOL * CS * sizeof(float2);
+};
+}  // namespace c8k
+
+// X[k] = E[k mod 16] + W32^k O[k mod 16] with E, O the DFT-16 of the even
+// and odd inputs, W32 = exp(SIGN 2 pi i / 32).
+template <int SIGN>
+__device__ __forceinline__ void dft32(float2 (&v)[32]) {
+    constexpr float C[16] = {1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+                             0.70710678118654752f, 0.55557023301960218f, 0.38268343236508977f, 0.19509032201612827f,
+                             0.0f, -0.19509032201612827f, -0.38268343236508977f, -0.55557023301960218f,
+                             -0.70710678118654752f, -0.83146961230254524f, -0.92387953251128674f,
+                             -0.98078528040323043f};
+    float2 e[16], o[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        e[b] = v[2 * b];
+        o[b] = v[2 * b + 1];
+    }
+    dft16<SIGN>(e);
+    dft16<SIGN>(o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        // W32^k = (cos(2 pi k / 32), SIGN sin(2 pi k / 32)); sin(2 pi k / 32) = C[8 - k] for k <= 8,
+        // C[k - 8] for k > 8.
+        float2 t;
+        if (k == 0) t = o[0];
+        else if (k == 8) t = mul_si<SIGN>(o[8]);
+        else t = cmul(o[k], make_float2(C[k], SIGN * (k < 8 ? C[8 - k] : C[k - 8])));
+        v[k] = cadd(e[k], t);
+        v[k + 16] = csub(e[k], t);
+    }
+}
+
+// v[r] *= w^r, r = 1..31, w = w1 (conjugated for SIGN > 0).
+template <int SIGN>
+__device__ __forceinline__ void twiddle32(float2 (&v)[32], float2 w1) {
+    if (SIGN > 0) w1.y = -w1.y;
+    float2 w[16];
+    w[1] = w1;
+    w[2] = cmul(w1, w1);
+    w[3] = cmul(w[2], w1);
+    w[4] = cmul(w[2], w[2]);
+    w[5] = cmul(w[4], w1);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+    w[8] = cmul(w[4], w[4]);
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+    const float2 w16 = cmul(w[8], w[8]);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], w[r]);
+    v[16] = cmul(v[16], w16);
+#pragma unroll
+    for (int r = 17; r < 32; ++r) v[r] = cmul(v[r], cmul(w16, w[r - 16]));
+}
+
+template <int NCOL>
+__global__ void __launch_bounds__(256 * NCOL) k_col_block8192(ColArgs in) {
+    using namespace c8k;
+    constexpr int CS = Plan<NCOL>::CS, LGC = NCOL == 2 ? 1 : 0;
+    static_assert(NCOL == 1 || NCOL == 2, "columns per workgroup");
+    extern __shared__ float2 lds[];
+    // Grid y: the kernel, which reads channel ksrc[kernel].
+    const int kern = blockIdx.y;
+    const size_t chan = (size_t)(in.W >> 1) * H;
+    const float2* spec_in = in.spec_in + in.ksrc[kern] * chan;
+    float2* spec_out = in.spec_out + kern * chan;
+    const float* khat = in.khat + kern * (size_t)(in.W >> 1) * NP;
+    const float2* kpack = in.kpack + kern * NP;
+    const int col0 = xcd_remap(blockIdx.x, gridDim.x) * NCOL, kb = col0 >> 2, cbase = col0 & 3;
+    const int tid = threadIdx.x;
+    const int c1 = tid & (NCOL - 1), t1 = tid >> LGC;                       // phase 1
+    const int c2 = tid >> 8, k1lo = (tid & 255) >> 4, m = tid & 15;         // phases 2, 3
+    float2* ra = lds + c2 * CS + k1lo * KS;                                 // region of k1 = k1lo
+    float2* rb = ra + 16 * KS;                                              // and of k1lo + 16
+    const float2* src = spec_in + (size_t)kb * H * BC + cbase;
+    {
+        float2 v[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) v[j] = src[(size_t)(t1 + 256 * j) * BC + c1];
+        dft32<-1>(v);
+        twiddle32<-1>(v, in.twH[t1]);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) lds[c1 * CS + r * KS + t1] = v[r];
+    }
+    // K-hat of the phase-3 frequencies k = k1 + 32 m + 512 r, p = min(k, H - k):
+    // p = k for r < 8; for r >= 8, p = H - k = p8 - 512 (r - 8), which keeps
+    // the lower digits of p8 = H/2 - k1 - 32 m.
+    const int col = kb * BC + cbase + c2;
+    float kf[2][16];
+    if (col != 0) {
+        const float* kh = khat + (size_t)col * NP;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int k1 = k1lo + 16 * q;
+            const int lo = 16 * k1 + m, p8 = H / 2 - k1 - 32 * m;
+            const int hi = (p8 >> 9) * 512 + (p8 & 31) * 16 + ((p8 >> 5) & 15);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) kf[q][r] = kh[r < 8 ? 512 * r + lo : hi - 512 * (r - 8)];
+        }
+    }
+    lds_sync();
+    float2 va[16], vb[16];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        va[n] = ra[m + 16 * n];
+        vb[n] = rb[m + 16 * n];
+    }
+    const float2 wm = in.twH[32 * m];
+    dft16<-1>(va);
+    dft16<-1>(vb);
+    twiddle16<-1>(va, wm);
+    twiddle16<-1>(vb, wm);
+    wave_lds_order();  // this wave's phase-2 reads of its regions are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        ra[17 * r + m] = va[r];
+        rb[17 * r + m] = vb[r];
+    }
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        va[r] = ra[17 * m + r];
+        vb[r] = rb[17 * m + r];
+    }
+    dft16<-1>(va);
+    dft16<-1>(vb);
+    if (kb == 0 && cbase == 0) {
+        // Workgroup-uniform: column 0 packs the real bins kx = 0 and W/2;
+        // D[k] = A C[k] + B conj C[-k], (A, B) = kpack[min(k, H - k)], with
+        // C[-k] from LDS in natural order (column 0's regions hold
+        // 32 KS >= H entries).
+        lds_sync();
+        if (c2 == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                lds[k1lo + 32 * m + 512 * r] = va[r];
+                lds[k1lo + 16 + 32 * m + 512 * r] = vb[r];
+            }
+        }
+        lds_sync();
+        if (c2 == 0) {
+            auto packed = [&](float2 x, int k) {
+                const int kn = (H - k) & (H - 1);
+                const float2 cn = lds[kn], ab = kpack[min(k, kn)];
+                return cadd(make_float2(ab.x * x.x, ab.x * x.y), make_float2(ab.y * cn.x, -ab.y * cn.y));
+            };
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                va[r] = packed(va[r], k1lo + 32 * m + 512 * r);
+                vb[r] = packed(vb[r], k1lo + 16 + 32 * m + 512 * r);
+            }
+        }
+        lds_sync();  // partner reads are done before the regions are reused
+    }
+    if (col != 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            va[r] = make_float2(kf[0][r] * va[r].x, kf[0][r] * va[r].y);
+            vb[r] = make_float2(kf[1][r] * vb[r].x, kf[1][r] * vb[r].y);
+        }
+    }
+    // Inverse: phase 3, exchange 2, phase 2, exchange 1, phase 1.
+    dft16<1>(va);
+    dft16<1>(vb);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        ra[17 * m + r] = va[r];
+        rb[17 * m + r] = vb[r];
+    }
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        va[r] = ra[17 * r + m];
+        vb[r] = rb[17 * r + m];
+    }
+    twiddle16<1>(va, wm);
+    twiddle16<1>(vb, wm);
+    dft16<1>(va);
+    dft16<1>(vb);
+    wave_lds_order();  // the transpose reads are done before the regions are rewritten
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        ra[m + 16 * n] = va[n];
+        rb[m + 16 * n] = vb[n];
+    }
+    lds_sync();
+    float2 v[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r] = lds[c1 * CS + r * KS + t1];
+    twiddle32<1>(v, in.twH[t1]);
+    dft32<1>(v);
+    float2* dst = spec_out + (size_t)kb * H * BC + cbase;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) dst[(size_t)(t1 + 256 * j) * BC + c1] = v[j];
+}
+
 // Column pass for H = 256..2048 in the same register-resident form as the
 // H = 4096 kernel above, for one kernel per channel (batched same-size
 // worlds in diagonal mode, or a single world with one kernel).  H = 16 S,
@@ -1322,8 +1540,9 @@ int row_rows(int W, int H) {
 
 }  // namespace
 
-// H = 4096 outside batches: the block column kernel, any channels and kernels.
-bool col_block_ok(int W, int H) { return H == c4k::H && W >= 64; }
+// H = 4096 or 8192 outside batches: the block column kernels, any channels
+// and kernels.
+bool col_block_ok(int W, int H) { return (H == c4k::H || H == c8k::H) && W >= 64; }
 
 int khat_layout(int W, int H, int channels, int n_kernels, const int* ksrc) {
     (void)channels;
@@ -1414,6 +1633,17 @@ void launch_col_conv(hipStream_t s, Profiler* p, int W, int H, int channels, con
     a.diag = diag ? 1 : 0;
     a.n_kernels = diag ? channels : n_kernels;
     for (int k = 0; k < AH_MAX_KERNELS; ++k) a.ksrc[k] = (!diag && k < n_kernels) ? ksrc[k] : -1;
+    if (!diag && col_block_ok(W, H) && H == c8k::H) {
+        // Two columns per workgroup (139 KiB of LDS, one workgroup per CU):
+        // at 8192^2, 183 us against 213 us with one column (70 KiB, two
+        // workgroups per CU) and 346 us for the LDS-staged k_col_conv<16>.
+        using P = c8k::Plan<2>;
+        Launch L(p, s, "col_conv");
+        set_lds(k_col_block8192<2>, P::LDS_BYTES);
+        hipLaunchKernelGGL(k_col_block8192<2>, dim3(W / 4, n_kernels), dim3(P::THREADS), P::LDS_BYTES, s, a);
+        AH_HIP(hipGetLastError());
+        return;
+    }
     if (!diag && col_block_ok(W, H)) {
         // Two columns per workgroup (68 KiB of LDS): two workgroups share a
         // CU, so one's loads overlap the other's arithmetic.  Grid y is the

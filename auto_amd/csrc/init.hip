@@ -145,7 +145,8 @@ __global__ void k_khat_rows64(int W, int reach, const double* taps, const double
 
 // khat[kb][p][c] = (sum_dy T[dy][kx] cos(2 pi p dy / H)) / (W*H), p in [0, H/2];
 // kx = 0 and kx = W/2 go to kpack[p] = ((K0+KN)/2, (K0-KN)/2).  `layout`
-// KHAT_DIGITS stores khat[kx][digits(p)] instead.
+// KHAT_DIGITS stores khat[kx][digits(p)] instead (the block column kernels'
+// digit order: H = 4096 and H = 8192).
 __global__ void k_khat64(int W, int H, int reach, const double* T, const double* cosH, float* khat, float2* kpack,
                          int layout) {
     const int nk = W / 2 + 1, np = H / 2 + 1;
@@ -169,7 +170,9 @@ __global__ void k_khat64(int W, int H, int reach, const double* T, const double*
         return;
     }
     if (layout == KHAT_DIGITS) {
-        khat[(size_t)kx * np + (p >> 8) * 256 + (p & 15) * 16 + ((p >> 4) & 15)] = (float)acc;
+        const int d = H == 8192 ? (p >> 9) * 512 + (p & 31) * 16 + ((p >> 5) & 15)
+                                : (p >> 8) * 256 + (p & 15) * 16 + ((p >> 4) & 15);
+        khat[(size_t)kx * np + d] = (float)acc;
         return;
     }
     int kb = kx >> 2, c = kx & 3;
