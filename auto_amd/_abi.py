@@ -180,6 +180,7 @@ _SIGNATURES = {
     "ah_sim_read_field": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
     "ah_sim_write_field": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
     "ah_sim_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "ah_sim_check": (ctypes.c_int, [_P]),
     "ah_state_stats_noalloc": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, _P, ctypes.POINTER(Stats)]),
     "ah_sim_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
     "ah_sim_destroy": (None, [_P]),

@@ -803,6 +803,17 @@ ah_status ah_sim_stats(ah_sim* s, ah_stats* out) {
     });
 }
 
+ah_status ah_sim_check(ah_sim* s) {
+    ah_stats st{};
+    const ah_status e = ah_sim_stats(s, &st);
+    if (e != AH_OK) return e;
+    return guard([&] {
+        if (st.nonfinite)
+            throw Error{AH_ERR_DIVERGED, "state holds " + std::to_string(st.nonfinite) + " non-finite cells at generation " +
+                                             std::to_string(st.generation)};
+    });
+}
+
 ah_status ah_sim_info(const ah_sim* s, ah_info* out) {
     return guard([&] {
         if (!s || !out) throw Error{AH_ERR_INVALID, "null argument"};

@@ -104,6 +104,10 @@ class Sim:
         check(self._lib.ah_sim_stats(self._h, ctypes.byref(s)))
         return s
 
+    def check(self) -> None:
+        """Raise EngineError (AH_ERR_DIVERGED) if any cell is NaN or infinite."""
+        check(self._lib.ah_sim_check(self._h))
+
     # -- view -------------------------------------------------------------
     def render(self, out_w: int, out_h: int, center=None, scale: float | None = None) -> np.ndarray:
         """rgba8 [out_h][out_w][4] view of channel 0 (Sim::render, sim.rs:452-514).

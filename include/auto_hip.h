@@ -205,6 +205,17 @@ ah_status ah_sim_render(ah_sim* sim, uint32_t out_w, uint32_t out_h, float cente
 ah_status ah_sim_stats(ah_sim* sim, ah_stats* out);
 
 /*
+ * Failure check: the same reduction, AH_ERR_DIVERGED when any cell of the
+ * current state is NaN or infinite (the message gives the count and the
+ * generation), AH_OK otherwise.  No counterpart in the reference, whose
+ * Lenia has no failure detection.  A step maps a non-finite input to 0
+ * (the clip's fmax/fmin return the non-NaN operand, and a NaN reaches every
+ * potential through the transform), so the check belongs after a state
+ * write and before stepping it.
+ */
+ah_status ah_sim_check(ah_sim* sim);
+
+/*
  * The reduction of ah_sim_stats, allocating nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -161,6 +161,32 @@ def test_stats_count_nonfinite():
     assert st.mass[0] == pytest.approx(float(np.nansum(np.where(np.isfinite(s), s, 0), dtype=np.float64)))
 
 
+def test_check_reports_a_diverged_state():
+    """ah_sim_check: AH_OK on a finite state, AH_ERR_DIVERGED naming the
+    count once a NaN is written in.  A step from that state spreads the NaN
+    through the FFT to every potential, and the clip (fmax/fmin return the
+    non-NaN operand) maps every cell to 0: the step never leaves a non-finite
+    state behind, so the check belongs before stepping a state from outside."""
+    from auto_amd._abi import EngineError, ERR_DIVERGED
+    d = W.single_ring_256()
+    sim = Sim(d, 1)
+    sim.step(3)
+    sim.check()
+    s = sim.read_field("state")
+    clean = s.copy()
+    s[0, 100, 100] = np.nan
+    sim.write_field("state", s)
+    with pytest.raises(EngineError) as e:
+        sim.check()
+    assert e.value.code == ERR_DIVERGED and "1 non-finite" in str(e.value)
+    sim.step(1)
+    sim.check()
+    after = sim.read_field("state")
+    assert np.isfinite(after).all() and after.max() == 0.0
+    sim.write_field("state", clean)
+    sim.check()
+
+
 def test_deterministic_and_resumable():
     """Same seed -> bitwise-equal states; save at t and continue == uninterrupted run."""
     d = W.rings3_4096(side=512)
