@@ -68,8 +68,42 @@ __global__ void k_im2col1(const float* __restrict__ x, uint16_t* __restrict__ X1
 }
 
 // ---------------------------------------------------------------- MFMA conv
+// Bias, z, GELU (+ skip) and the bf16 activation of one wave's outputs: lane
+// holds channels 16m + 4q .. +3 of pixel x0 + 16b + l16 of row y.
+template <int MT, int NB>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NB], int n, int y, int x0,
+                                              int q, int l16) {
+    const int H = a.H, W = a.W, Cout = a.Cout;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int co = 16 * m + 4 * q;
+            const size_t o = (((size_t)n * H + y) * W + x0 + 16 * b + l16) * Cout + co;
+            float z[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z[r] = acc[m][b][r] + (a.bias ? a.bias[co + r] : 0.0f);
+            if (a.z) *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
+            if (a.mode == CONV_GELU || a.mode == CONV_GELU_ADD) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = gelu(z[r]);
+                if (a.mode == CONV_GELU_ADD) {
+                    const uint2 ad = *reinterpret_cast<const uint2*>(a.add + o);
+                    v[0] += __uint_as_float(ad.x << 16);
+                    v[1] += __uint_as_float(ad.x & 0xffff0000u);
+                    v[2] += __uint_as_float(ad.y << 16);
+                    v[3] += __uint_as_float(ad.y & 0xffff0000u);
+                }
+                *reinterpret_cast<uint2*>(a.a + o) =
+                    make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+            }
+        }
+}
+
 // Wave: one output row, 64 consecutive pixels (4 blocks of 16), MT blocks of
 // 16 output channels.  Workgroup: 4 waves = 4 rows.  Grid (W/64, H/4, N).
+// Operands straight from global memory (the 1x1 first layer).
 template <int MT>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -108,33 +142,69 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
             }
         }
     }
-    // Lane holds channels 16m + 4q .. +3 of pixel x0 + 16b + l16.
-    const int Cout = a.Cout;
+    conv_epilogue<MT, 4>(a, acc, n, y, x0, q, l16);
+}
+
+// The 5x5 convolutions: the same waves and MFMAs as k_conv, with the input
+// staged in LDS.  Per 32-channel slice the workgroup loads its 8 x 68-pixel
+// input window (4 output rows plus the 2-row, 2-column halo) once, and every
+// tap's B fragments come from LDS: k_conv read them from the L1 cache for
+// every tap, 25 times per input.  An upsampled layer (UP) stages the 4 x 34
+// half-resolution pixels under that window and reads window pixel (r, c)
+// at (r/2, c/2).  Pixel stride 40 bf16 (80 bytes): a wave's 16-pixel
+// fragment reads fall on distinct banks.
+namespace cl {
+constexpr int TR = 8, PS = 40;  // window rows, bf16 per pixel (32 + 8)
+}
+
+// NB blocks of 16 pixels per wave (a window of 16 NB + 4 columns).
+template <int MT, bool UP, int NB>
+__global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
+    using namespace cl;
+    constexpr int TC = 16 * NB + 4;
+    constexpr int SR = UP ? TR / 2 : TR, SC = UP ? TC / 2 : TC;  // staged rows, columns
+    __shared__ __align__(16) uint16_t tile[SR * SC * PS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int n = blockIdx.z, y0 = blockIdx.y * 4, y = y0 + wave, x0 = blockIdx.x * (16 * NB);
+    const int H = a.H, W = a.W, Cin = a.Cin;
+    const int Hx = UP ? H / 2 : H, Wx = UP ? W / 2 : W;
+    const uint16_t* X = a.X + (size_t)n * Hx * Wx * Cin;
+    // First staged input row and column: the window starts at (y0 - 2, x0 - 2),
+    // at (y0/2 - 1, x0/2 - 1) in the half-resolution input (y0, x0 even).
+    const int ys = UP ? y0 / 2 - 1 : y0 - 2, xs = UP ? x0 / 2 - 1 : x0 - 2;
+    f32x4 acc[MT][NB];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int co = 16 * m + 4 * q;
-            const size_t o = (((size_t)n * H + y) * W + x0 + 16 * b + l16) * Cout + co;
-            float z[4];
+        for (int b = 0; b < NB; ++b) acc[m][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < Cin; c0 += 32) {
+        if (c0) __syncthreads();  // every wave has read the previous slice
+        for (int i = threadIdx.x; i < SR * SC * 4; i += 256) {
+            const int pix = i >> 2, part = i & 3, r = pix / SC, c = pix - r * SC;
+            const int yy = wrap(ys + r, Hx), xx = wrap(xs + c, Wx);
+            const uint4 v = *reinterpret_cast<const uint4*>(X + ((size_t)yy * Wx + xx) * Cin + c0 + 8 * part);
+            *reinterpret_cast<uint4*>(tile + pix * PS + 8 * part) = v;
+        }
+        __syncthreads();
+        for (int t = 0; t < 25; ++t) {
+            const int dy = t / 5 - 2, dx = t % 5 - 2;
+            bf16x8 A[MT];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) z[r] = acc[m][b][r] + (a.bias ? a.bias[co + r] : 0.0f);
-            if (a.z) *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
-            if (a.mode == CONV_GELU || a.mode == CONV_GELU_ADD) {
-                float v[4];
+            for (int m = 0; m < MT; ++m)
+                A[m] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(a.Wt + ((size_t)(16 * m + l16) * 25 + t) * Cin + c0 + 8 * q));
+            const int r = wave + 2 + dy, c = 2 + dx + l16;  // window pixel of block 0
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu(z[r]);
-                if (a.mode == CONV_GELU_ADD) {
-                    const uint2 ad = *reinterpret_cast<const uint2*>(a.add + o);
-                    v[0] += __uint_as_float(ad.x << 16);
-                    v[1] += __uint_as_float(ad.x & 0xffff0000u);
-                    v[2] += __uint_as_float(ad.y << 16);
-                    v[3] += __uint_as_float(ad.y & 0xffff0000u);
-                }
-                *reinterpret_cast<uint2*>(a.a + o) =
-                    make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+            for (int b = 0; b < NB; ++b) {
+                const int pix = UP ? (r >> 1) * SC + ((c + 16 * b) >> 1) : r * SC + c + 16 * b;
+                const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(tile + pix * PS + 8 * q));
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][b], 0, 0, 0);
             }
         }
+    }
+    conv_epilogue<MT, NB>(a, acc, n, y, x0, q, l16);
 }
 
 // ---------------------------------------------------------------- pooling
@@ -522,6 +592,25 @@ __global__ void __launch_bounds__(256) k_bgrad(const float* __restrict__ dz_f, c
     }
 }
 
+// Bias gradient: out[c] = sum over chunks of part[chunk][c], one workgroup
+// per channel: thread t adds chunks t, t + 256, ... in order, then a fixed
+// pairwise tree over the 256 threads (bitwise reproducible).
+__global__ void __launch_bounds__(256) k_reduce_bias(const float* __restrict__ part, int chunks, int C,
+                                                     float* __restrict__ out) {
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
+    float acc = 0.0f;
+    for (int k = threadIdx.x; k < chunks; k += 256) acc += part[(size_t)k * C + c];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
 // out[i] = sum over chunks of part[chunk][i], in chunk order.
 __global__ void k_reduce_parts(const float* __restrict__ part, int chunks, int n, float* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -628,12 +717,26 @@ void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, in
 }
 
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
-    if (a.W % 64 || a.H % 4 || a.Cin % 32 || (a.Cout != 32 && a.Cout != 64))
+    if (a.W % 64 || a.H % 4 || a.Cin % 32 || (a.Cout != 32 && a.Cout != 64) || (a.ks != 1 && a.ks != 5))
         throw Error{AH_ERR_INVALID, "surrogate conv: unsupported shape"};
     Launch L(p, s, name);
     dim3 grid(a.W / 64, a.H / 4, a.N);
-    if (a.Cout == 64) hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, a);
+    if (a.ks == 5) {
+        // 32 output channels: 128 pixels per wave when the width allows, so
+        // each weight fragment feeds 16 MFMAs.
+        const dim3 grid8(a.W / 128, a.H / 4, a.N);
+        const bool wide = a.W % 128 == 0;
+        if (a.Cout == 64 && a.up) hipLaunchKernelGGL((k_conv_lds<4, true, 4>), grid, dim3(256), 0, s, a);
+        else if (a.Cout == 64) hipLaunchKernelGGL((k_conv_lds<4, false, 4>), grid, dim3(256), 0, s, a);
+        else if (a.up && wide) hipLaunchKernelGGL((k_conv_lds<2, true, 8>), grid8, dim3(256), 0, s, a);
+        else if (a.up) hipLaunchKernelGGL((k_conv_lds<2, true, 4>), grid, dim3(256), 0, s, a);
+        else if (wide) hipLaunchKernelGGL((k_conv_lds<2, false, 8>), grid8, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_conv_lds<2, false, 4>), grid, dim3(256), 0, s, a);
+    } else if (a.Cout == 64) {
+        hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, a);
+    }
     AH_HIP(hipGetLastError());
 }
 
@@ -725,8 +828,10 @@ void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, 
         hipLaunchKernelGGL(k_bgrad, dim3(chunks), dim3(256), 0, s, dz_f, dz_h, C, npix, part);
         AH_HIP(hipGetLastError());
     }
-    Launch L(p, s, "surr_reduce");
-    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(C)), dim3(256), 0, s, part, chunks, C, out);
+    // One workgroup per channel: a lone thread per channel walking all the
+    // chunks in turn took 183 us per call on its chain of dependent loads.
+    Launch L(p, s, "surr_reduce_bias");
+    hipLaunchKernelGGL(k_reduce_bias, dim3(C), dim3(256), 0, s, part, chunks, C, out);
     AH_HIP(hipGetLastError());
 }
 
