@@ -413,6 +413,9 @@ __global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ 
 // the result does not depend on timing.
 namespace wg {
 constexpr int SEG = 64, ROWS = 5, COLS = SEG + 4, THREADS = 512, GROUPS = 256;
+// Workgroups of the thin-layer kernels: 800 outputs each, so many more
+// workgroups than the MFMA kernels' (256 left 4 waves per CU).
+constexpr int THIN_GROUPS = 2048;
 }
 
 template <int COUT, int CIN, int UP>
@@ -592,11 +595,12 @@ __global__ void __launch_bounds__(256) k_bgrad(const float* __restrict__ dz_f, c
     }
 }
 
-// Bias gradient: out[c] = sum over chunks of part[chunk][c], one workgroup
-// per channel: thread t adds chunks t, t + 256, ... in order, then a fixed
-// pairwise tree over the 256 threads (bitwise reproducible).
-__global__ void __launch_bounds__(256) k_reduce_bias(const float* __restrict__ part, int chunks, int C,
-                                                     float* __restrict__ out) {
+// Few outputs over many chunks (biases, the thin layers' weights):
+// out[c] = sum over chunks of part[chunk][c], one workgroup per output:
+// thread t adds chunks t, t + 256, ... in order, then a fixed pairwise tree
+// over the 256 threads (bitwise reproducible).
+__global__ void __launch_bounds__(256) k_reduce_few(const float* __restrict__ part, int chunks, int C,
+                                                    float* __restrict__ out) {
     __shared__ float sh[256];
     const int c = blockIdx.x;
     float acc = 0.0f;
@@ -722,15 +726,15 @@ void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
     Launch L(p, s, name);
     dim3 grid(a.W / 64, a.H / 4, a.N);
     if (a.ks == 5) {
-        // 32 output channels: 128 pixels per wave when the width allows, so
-        // each weight fragment feeds 16 MFMAs.
+        // The upsampled 32-channel layer: 128 pixels per wave when the width
+        // allows, so each weight fragment feeds 16 MFMAs (1008 -> 803 us per
+        // call at batch 64 x 256^2; the half-resolution 32-channel input
+        // gradient lost the same change, 300 -> 394 us).
         const dim3 grid8(a.W / 128, a.H / 4, a.N);
-        const bool wide = a.W % 128 == 0;
         if (a.Cout == 64 && a.up) hipLaunchKernelGGL((k_conv_lds<4, true, 4>), grid, dim3(256), 0, s, a);
         else if (a.Cout == 64) hipLaunchKernelGGL((k_conv_lds<4, false, 4>), grid, dim3(256), 0, s, a);
-        else if (a.up && wide) hipLaunchKernelGGL((k_conv_lds<2, true, 8>), grid8, dim3(256), 0, s, a);
+        else if (a.up && a.W % 128 == 0) hipLaunchKernelGGL((k_conv_lds<2, true, 8>), grid8, dim3(256), 0, s, a);
         else if (a.up) hipLaunchKernelGGL((k_conv_lds<2, true, 4>), grid, dim3(256), 0, s, a);
-        else if (wide) hipLaunchKernelGGL((k_conv_lds<2, false, 8>), grid8, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_conv_lds<2, false, 4>), grid, dim3(256), 0, s, a);
     } else if (a.Cout == 64) {
         hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a);
@@ -795,7 +799,7 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
     const bool thin = a.dz_f && a.W % wg::SEG == 0 && !a.up &&
                       ((a.Cout == 32 && a.Cin == 1) || (a.Cout == 1 && a.Cin == 32));
     if (thin) {
-        chunks = wg::GROUPS;
+        chunks = wg::THIN_GROUPS;
         Launch L(p, s, "surr_wgrad_thin");
         if (a.Cout == 32)
             hipLaunchKernelGGL((k_wgrad_thin<32, 1>), dim3(chunks), dim3(256), 0, s, a, part);
@@ -816,7 +820,8 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         AH_HIP(hipGetLastError());
     }
     Launch L(p, s, "surr_reduce");
-    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
+    if (thin) hipLaunchKernelGGL(k_reduce_few, dim3(nw), dim3(256), 0, s, part, chunks, nw, out);
+    else hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
     AH_HIP(hipGetLastError());
 }
 
@@ -831,7 +836,7 @@ void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, 
     // One workgroup per channel: a lone thread per channel walking all the
     // chunks in turn took 183 us per call on its chain of dependent loads.
     Launch L(p, s, "surr_reduce_bias");
-    hipLaunchKernelGGL(k_reduce_bias, dim3(C), dim3(256), 0, s, part, chunks, C, out);
+    hipLaunchKernelGGL(k_reduce_few, dim3(C), dim3(256), 0, s, part, chunks, C, out);
     AH_HIP(hipGetLastError());
 }
 
