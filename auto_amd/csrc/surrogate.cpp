@@ -33,6 +33,7 @@ struct Work {
     float *dr = nullptr, *ds = nullptr, *du = nullptr, *dp = nullptr, *dz1 = nullptr;
     uint16_t *dz3 = nullptr, *dz2 = nullptr;
     float* part = nullptr;
+    float* bpart = nullptr;  // bias-gradient partials, 2048 workgroups x 64 channels
     double* lpart = nullptr;
     void release() {
         for (void* b : bufs) (void)hipFree(b);
@@ -136,6 +137,7 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
         w.dp = w.get<float>(Ph * C1);
         w.dz1 = w.get<float>(P * C1);
         w.part = w.get<float>(PART_FLOATS);
+        w.bpart = w.get<float>(2048 * 64);
     }
 }
 
@@ -201,7 +203,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
     hipStream_t st = S->stream;
     Profiler* pf = &S->prof;
     const int N = w.N, H = w.H, W = w.W;
-    const size_t P = (size_t)N * H * W, Ph = P / 4;
+    const size_t P = (size_t)N * H * W;
     const int lb = loss_blocks(P);
     surr::loss(st, pf, w.y, target, x, w.r, grad ? w.dr : nullptr, w.lpart, P);
     if (grad) {
@@ -216,8 +218,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W;
         a.Cin = C1;
         a.Cout = 1;
-        wgrad(st, pf, a, w.part, chunks_for(1, C1), g + OFF_W4);
-        bgrad(st, pf, w.dr, nullptr, 1, P, w.part, 1024, g + OFF_B4);
+        wgrad(st, pf, a, w.part, chunks_for(1, C1), g + OFF_W4, w.bpart, g + OFF_B4);
         out_input_grad(st, pf, w.dr, S->pw.W4, w.ds, N, H, W);
         // layer 3
         dz3(st, pf, w.ds, w.z3, w.dz3, P * C1);
@@ -231,8 +232,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.Cin = C2;
         a.Cout = C1;
         a.up = 1;
-        wgrad(st, pf, a, w.part, chunks_for(C1, C2), g + OFF_W3);
-        bgrad(st, pf, nullptr, w.dz3, C1, P, w.part, 1024, g + OFF_B3);
+        wgrad(st, pf, a, w.part, chunks_for(C1, C2), g + OFF_W3, w.bpart, g + OFF_B3);
         ConvArgs c{};
         c.X = w.dz3;
         c.N = N;
@@ -256,8 +256,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W / 2;
         a.Cin = C1;
         a.Cout = C2;
-        wgrad(st, pf, a, w.part, chunks_for(C2, C1), g + OFF_W2);
-        bgrad(st, pf, nullptr, w.dz2, C2, Ph, w.part, 1024, g + OFF_B2);
+        wgrad(st, pf, a, w.part, chunks_for(C2, C1), g + OFF_W2, w.bpart, g + OFF_B2);
         c = ConvArgs{};
         c.X = w.dz2;
         c.N = N;
@@ -282,8 +281,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W;
         a.Cin = 1;
         a.Cout = C1;
-        wgrad(st, pf, a, w.part, chunks_for(C1, 1), g + OFF_W1);
-        bgrad(st, pf, w.dz1, nullptr, C1, P, w.part, 1024, g + OFF_B1);
+        wgrad(st, pf, a, w.part, chunks_for(C1, 1), g + OFF_W1, w.bpart, g + OFF_B1);
     }
     std::vector<double> part(lb);
     AH_HIP(hipMemcpyAsync(part.data(), w.lpart, sizeof(double) * lb, hipMemcpyDeviceToHost, st));

@@ -64,7 +64,10 @@ void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t*
 void dz2(hipStream_t s, Profiler* p, const float* du, const float* z2, uint16_t* out, int N, int H, int W, int C);
 void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
          int C);
-void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out);
+// Weight gradient `out` and bias gradient `bout` of one layer; `part` and
+// `bpart` (at least 2048 * 64 floats) hold per-workgroup partials.
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,
+           float* bout);
 void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
            int chunks, float* out);
 void adam(hipStream_t s, Profiler* p, float* theta, float* m, float* v, const float* g, int n, float lr, float b1,

@@ -419,7 +419,8 @@ constexpr int THIN_GROUPS = 2048;
 }
 
 template <int COUT, int CIN, int UP>
-__global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* __restrict__ part) {
+__global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* __restrict__ part,
+                                                            float* __restrict__ bpart) {
     using namespace wg;
     static_assert((COUT / 16) * (CIN / 16) == THREADS / 64, "one 16x16 block per wave");
     constexpr int XS = CIN + 8, ZS = COUT + 8;  // LDS strides per pixel (bf16)
@@ -435,6 +436,7 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
     f32x4 acc[25];
 #pragma unroll
     for (int t = 0; t < 25; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;  // bias gradient of channel threadIdx.x (< COUT) over this workgroup's pixels
     for (long sg = s0; sg < s1; ++sg) {
         const int xb = (int)(sg % segs_per_row);
         const long ry = sg / segs_per_row;
@@ -462,6 +464,8 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
                 a.X + (((size_t)n * Hx + sy) * Wx + sx) * a.xstride + a.xoff + 8 * part8);
         }
         __syncthreads();
+        if ((int)threadIdx.x < COUT)
+            for (int px = 0; px < SEG; ++px) bacc += bf16f(dz[px * ZS + threadIdx.x]);
 #pragma unroll
         for (int kc = 0; kc < SEG / 32; ++kc) {
             const int p0 = 32 * kc + 8 * q;  // this lane's 8 pixels: p0 .. p0 + 7
@@ -483,6 +487,7 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
             }
         }
     }
+    if ((int)threadIdx.x < COUT) bpart[(size_t)blockIdx.x * COUT + threadIdx.x] = bacc;
     // Lane holds dW[co = 16 cob + 4q + r][t][ci = 16 cib + l16].
     float* dst = part + (size_t)blockIdx.x * COUT * 25 * CIN;
 #pragma unroll
@@ -500,7 +505,7 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
 // outputs a = 4 quad .. (A = 32) or b = 4 quad .. (B = 32) of tap t over the
 // segment's pixels in order.  Partials go to part[workgroup][a][t][b].
 template <int A, int B>
-__global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restrict__ part, float* __restrict__ bpart) {
     using wg::SEG;
     using wg::ROWS;
     using wg::COLS;
@@ -514,6 +519,7 @@ __global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restri
     const bool active = t < 25;
     const int dy = t / 5, dx = t % 5;  // tile offsets of the tap (tap + 2)
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;  // bias gradient of output channel threadIdx.x (< A) over this workgroup's pixels
     for (long sg = s0; sg < s1; ++sg) {
         const int xb = (int)(sg % segs_per_row);
         const long ry = sg / segs_per_row;
@@ -542,6 +548,8 @@ __global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restri
             }
         }
         __syncthreads();
+        if ((int)threadIdx.x < A)
+            for (int p = 0; p < SEG; ++p) bacc += u[p * A + threadIdx.x];
         if (active) {
             for (int p = 0; p < SEG; ++p) {
                 const int vp = (dy * COLS + p + dx) * B;  // V at pixel p + tap
@@ -563,6 +571,7 @@ __global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restri
             }
         }
     }
+    if ((int)threadIdx.x < A) bpart[(size_t)blockIdx.x * A + threadIdx.x] = bacc;
     if (active) {
         float* dst = part + (size_t)blockIdx.x * A * 25 * B;
 #pragma unroll
@@ -791,7 +800,8 @@ void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const flo
     AH_HIP(hipGetLastError());
 }
 
-void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out) {
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,
+           float* bout) {
     const int Cq = (a.Cout + 3) / 4;
     const int nout = Cq * 25 * a.Cin, nw = a.Cout * 25 * a.Cin;
     const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 &&
@@ -802,27 +812,39 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         chunks = wg::THIN_GROUPS;
         Launch L(p, s, "surr_wgrad_thin");
         if (a.Cout == 32)
-            hipLaunchKernelGGL((k_wgrad_thin<32, 1>), dim3(chunks), dim3(256), 0, s, a, part);
+            hipLaunchKernelGGL((k_wgrad_thin<32, 1>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
         else
-            hipLaunchKernelGGL((k_wgrad_thin<1, 32>), dim3(chunks), dim3(256), 0, s, a, part);
+            hipLaunchKernelGGL((k_wgrad_thin<1, 32>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else if (mfma) {
         chunks = wg::GROUPS;
         Launch L(p, s, "surr_wgrad_mfma");
         if (a.Cout == 32)
-            hipLaunchKernelGGL((k_wgrad_mfma<32, 64, 1>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part);
+            hipLaunchKernelGGL((k_wgrad_mfma<32, 64, 1>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
         else
-            hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part);
+            hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else {
         Launch L(p, s, "surr_wgrad");
         hipLaunchKernelGGL(k_wgrad, dim3(blocks(nout), chunks), dim3(256), 0, s, a, part);
         AH_HIP(hipGetLastError());
     }
-    Launch L(p, s, "surr_reduce");
-    if (thin) hipLaunchKernelGGL(k_reduce_few, dim3(nw), dim3(256), 0, s, part, chunks, nw, out);
-    else hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
-    AH_HIP(hipGetLastError());
+    {
+        Launch L(p, s, "surr_reduce");
+        if (thin) hipLaunchKernelGGL(k_reduce_few, dim3(nw), dim3(256), 0, s, part, chunks, nw, out);
+        else hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
+        AH_HIP(hipGetLastError());
+    }
+    // The bias gradient (the sum of dZ over the pixels): the thin and MFMA
+    // kernels add it up from the dZ they stage (per-workgroup partials in
+    // bpart); the generic kernel leaves it to a pass of its own.
+    if (thin || mfma) {
+        Launch L(p, s, "surr_reduce_bias");
+        hipLaunchKernelGGL(k_reduce_few, dim3(a.Cout), dim3(256), 0, s, bpart, chunks, a.Cout, bout);
+        AH_HIP(hipGetLastError());
+    } else {
+        bgrad(s, p, a.dz_f, a.dz_h, a.Cout, (size_t)a.N * a.H * a.W, part, 1024, bout);
+    }
 }
 
 void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
