@@ -30,7 +30,7 @@ struct Work {
     std::vector<void*> bufs;
     uint16_t *X1 = nullptr, *a1 = nullptr, *p = nullptr, *a2 = nullptr, *s = nullptr;
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *r = nullptr, *y = nullptr;
-    float *dr = nullptr, *ds = nullptr, *du = nullptr, *dp = nullptr, *dz1 = nullptr;
+    float *dr = nullptr, *ds = nullptr, *dp = nullptr, *dz1 = nullptr;
     uint16_t *dz3 = nullptr, *dz2 = nullptr;
     float* part = nullptr;
     float* bpart = nullptr;  // bias-gradient partials, 2048 workgroups x 64 channels
@@ -132,7 +132,6 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
         w.dr = w.get<float>(P);
         w.ds = w.get<float>(P * C1);
         w.dz3 = w.get<uint16_t>(P * C1);
-        w.du = w.get<float>(P * C2);
         w.dz2 = w.get<uint16_t>(Ph * C2);
         w.dp = w.get<float>(Ph * C1);
         w.dz1 = w.get<float>(P * C1);
@@ -242,11 +241,13 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         c.ks = 5;
         c.Wt = S->pw.W3T;
         c.Cout = C2;
-        c.mode = CONV_LINEAR;
-        c.z = w.du;
+        // dZ2 = (the input gradient summed over 2 x 2 blocks) * gelu'(z2),
+        // formed in the convolution's epilogue.
+        c.mode = CONV_POOL_DGELU;
+        c.zp = w.z2;
+        c.dpool = w.dz2;
         conv(st, pf, c, "surr_conv3_dgrad");
         // layer 2
-        dz2(st, pf, w.du, w.z2, w.dz2, N, H, W, C2);
         a = WGradArgs{};
         a.dz_h = w.dz2;
         a.X = w.p;

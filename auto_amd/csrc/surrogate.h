@@ -19,7 +19,10 @@ constexpr int OFF_W4 = OFF_B3 + C1;
 constexpr int OFF_B4 = OFF_W4 + 25 * C1;
 constexpr int N_PARAMS = OFF_B4 + 1;  // 104129
 
-enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2 };
+// CONV_POOL_DGELU (the layer-3 input gradient): the output, summed over
+// 2 x 2 pixel blocks (the adjoint of the nearest upsampling) and multiplied
+// by gelu'(zp), goes to dpool at half resolution; nothing else is written.
+enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2, CONV_POOL_DGELU = 3 };
 
 struct ConvArgs {
     const uint16_t* X;   // bf16 NHWC input, Cin channels (half resolution when up)
@@ -32,6 +35,8 @@ struct ConvArgs {
     float* z;            // fp32 [N][H][W][Cout] or null
     uint16_t* a;         // bf16 [N][H][W][Cout]: gelu(z) (+ add)
     const uint16_t* add; // bf16 [N][H][W][Cout] for CONV_GELU_ADD
+    const float* zp;     // fp32 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
+    uint16_t* dpool;     // bf16 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
 };
 
 struct WGradArgs {
@@ -61,7 +66,6 @@ void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const floa
           double* part, size_t total);
 void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W);
 void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total);
-void dz2(hipStream_t s, Profiler* p, const float* du, const float* z2, uint16_t* out, int N, int H, int W, int C);
 void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
          int C);
 // Weight gradient `out` and bias gradient `bout` of one layer; `part` and

@@ -204,6 +204,47 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             }
         }
     }
+    constexpr bool can_pool = !UP && MT * NB * 64 * 4 * 2 * sizeof(float) <= sizeof(tile);
+    if (can_pool && a.mode == CONV_POOL_DGELU) {
+        // Rows y0 + 2j and y0 + 2j + 1 (waves 2j, 2j + 1) meet in LDS, then
+        // lanes l16 and l16 ^ 1 (pixels x, x + 1) meet by a lane exchange.
+        float* sc = reinterpret_cast<float*>(tile);
+        __syncthreads();  // every wave has read the last slice
+        if (wave & 1) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        sc[(((wave >> 1) * MT + m) * NB + b) * 256 + r * 64 + lane] = acc[m][b][r];
+        }
+        __syncthreads();
+        if (!(wave & 1)) {
+            const int Hh = H / 2, Wh = W / 2, yh = y >> 1;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float col = acc[m][b][r] + sc[(((wave >> 1) * MT + m) * NB + b) * 256 + r * 64 + lane];
+                        v[r] = col + __shfl_xor(col, 1);
+                    }
+                    if (!(l16 & 1)) {
+                        const int xh = (x0 + 16 * b + l16) >> 1, co = 16 * m + 4 * q;
+                        const size_t o = (((size_t)n * Hh + yh) * Wh + xh) * a.Cout + co;
+                        const float4 zz = *reinterpret_cast<const float4*>(a.zp + o);
+                        const float g[4] = {v[0] * gelu_grad(zz.x), v[1] * gelu_grad(zz.y), v[2] * gelu_grad(zz.z),
+                                            v[3] * gelu_grad(zz.w)};
+                        *reinterpret_cast<uint2*>(a.dpool + o) = make_uint2(bf16_bits(g[0]) | (bf16_bits(g[1]) << 16),
+                                                                            bf16_bits(g[2]) | (bf16_bits(g[3]) << 16));
+                    }
+                }
+        }
+        return;
+    }
     conv_epilogue<MT, NB>(a, acc, n, y, x0, q, l16);
 }
 
@@ -314,19 +355,6 @@ __global__ void k_dz3(const float* __restrict__ ds, const float* __restrict__ z3
 }
 
 // dz2 = bf16((sum of du over the 2x2 children) * gelu'(z2)), half resolution, C = 64.
-__global__ void k_dz2(const float* __restrict__ du, const float* __restrict__ z2, uint16_t* __restrict__ dz2, int N,
-                      int H, int W, int C) {
-    const int Hh = H / 2, Wh = W / 2;
-    const size_t total = (size_t)N * Hh * Wh * C;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t pix = i / C;
-    const int xh = (int)(pix % Wh), yh = (int)((pix / Wh) % Hh), n = (int)(pix / ((size_t)Wh * Hh));
-    const size_t b = (((size_t)n * H + 2 * yh) * W + 2 * xh) * C + c;
-    const float s = du[b] + du[b + C] + du[b + (size_t)W * C] + du[b + (size_t)W * C + C];
-    dz2[i] = (uint16_t)bf16_bits(s * gelu_grad(z2[i]));
-}
 
 // dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, C = 32.
 __global__ void k_dz1(const float* __restrict__ ds, const float* __restrict__ dp, const float* __restrict__ z1,
@@ -732,6 +760,8 @@ void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, in
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
     if (a.W % 64 || a.H % 4 || a.Cin % 32 || (a.Cout != 32 && a.Cout != 64) || (a.ks != 1 && a.ks != 5))
         throw Error{AH_ERR_INVALID, "surrogate conv: unsupported shape"};
+    if (a.mode == CONV_POOL_DGELU && (a.ks != 5 || a.up))
+        throw Error{AH_ERR_INVALID, "surrogate conv: the pooled epilogue needs a 5x5 full-resolution input"};
     Launch L(p, s, name);
     dim3 grid(a.W / 64, a.H / 4, a.N);
     if (a.ks == 5) {
@@ -784,12 +814,6 @@ void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4
 void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total) {
     Launch L(p, s, "surr_dz3");
     hipLaunchKernelGGL(k_dz3, dim3(blocks(total)), dim3(256), 0, s, ds, z3, out, total);
-    AH_HIP(hipGetLastError());
-}
-
-void dz2(hipStream_t s, Profiler* p, const float* du, const float* z2, uint16_t* out, int N, int H, int W, int C) {
-    Launch L(p, s, "surr_dz2");
-    hipLaunchKernelGGL(k_dz2, dim3(blocks((size_t)N * H * W * C / 4)), dim3(256), 0, s, du, z2, out, N, H, W, C);
     AH_HIP(hipGetLastError());
 }
 
