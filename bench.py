@@ -279,6 +279,44 @@ def time_batch(seed: int, n: int = 384, graph_steps: int = 100, path: str = "aut
     return out
 
 
+def time_plain_worlds(seed: int, n: int = 384, steps: int = 20):
+    """The batch's worlds without the batch: every world a standalone FFT Sim
+    (the plain single-world step, the way a caller of Sim::step advances one
+    world at a time, crates/auto-core/src/sim.rs:297-309), each block queueing
+    `steps` steps on every world's own stream before waiting for all of them,
+    so worlds on different streams may overlap on the GPU.  Median of blocks
+    as timed_blocks does."""
+    from auto_amd import Sim, workloads as W
+    from auto_amd._abi import LeniaDesc, PATH_FFT_FP32
+    sims, cells = [], 0
+    t0 = time.perf_counter()
+    for d, s in W.uneven_worlds(seed=seed, n=n):
+        plain = LeniaDesc.from_buffer_copy(d)
+        plain.path = PATH_FFT_FP32
+        sims.append(Sim(plain, s))
+        cells += d.width * d.height
+    setup_s = time.perf_counter() - t0
+
+    def step(k):
+        for sim in sims:
+            sim.step(k)
+
+    def wait():
+        for sim in sims:
+            sim.wait()
+
+    step(2)
+    wait()
+    r = timed_blocks(step, wait, steps, cells)
+    for sim in sims:
+        sim.close()
+    return {"worlds": n, "cells_per_step": cells, "value": r["value"], "unit": "cell-updates/s",
+            "ms_per_step": r["ms_per_step"], "p10": r["p10"], "p90": r["p90"], "blocks": r["blocks"],
+            "block_steps": steps, "setup_s": setup_s,
+            "method": f"{n} standalone FFT Sims, {steps} steps queued on each world's stream per block, "
+                      "then a wait on every stream; median block"}
+
+
 def run_4096(seed: int, steps: int = 1000, every: int = 100):
     """Config-2 error protocol along the run: the 4096^2 three-ring world on
     both paths for `steps` steps; every `every` steps, one step from the
@@ -714,6 +752,11 @@ def main() -> int:
         log(f"batch (per-world path, {out['batch']['fft_worlds']} FFT worlds): {out['batch']['value']:.3e} "
             f"cell-updates/s; worst per-world max-abs vs the plain step "
             f"{out['batch']['per_world_vs_plain_step']['worst_max_abs']:.2e}")
+        pw = time_plain_worlds(args.seed)
+        out["batch"]["plain_per_world"] = pw
+        out["batch"]["over_plain_per_world"] = out["batch"]["value"] / pw["value"]
+        log(f"the same worlds as {pw['worlds']} standalone FFT Sims: {pw['value']:.3e} cell-updates/s; "
+            f"batch / plain per-world stepping = {out['batch']['over_plain_per_world']:.2f}x")
         out["batch_all_direct"] = time_batch(args.seed, path="direct_bf16", check=False)
         log(f"batch (all direct): {out['batch_all_direct']['value']:.3e} cell-updates/s")
     mc = {}
