@@ -28,9 +28,10 @@ struct Work {
     int N = 0, H = 0, W = 0;
     bool train = false;
     std::vector<void*> bufs;
-    uint16_t *X1 = nullptr, *a1 = nullptr, *p = nullptr, *a2 = nullptr, *s = nullptr;
-    float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *r = nullptr, *y = nullptr;
-    float *dr = nullptr, *ds = nullptr, *dp = nullptr, *dz1 = nullptr;
+    uint16_t *xb = nullptr, *a1 = nullptr, *p = nullptr, *a2 = nullptr, *s = nullptr;
+    float *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;  // gelu' of each layer's pre-activation (training)
+    float *r = nullptr, *y = nullptr;
+    float *dr = nullptr, *dp = nullptr, *dz1 = nullptr;
     uint16_t *dz3 = nullptr, *dz2 = nullptr;
     float* part = nullptr;
     float* bpart = nullptr;  // bias-gradient partials, 2048 workgroups x 64 channels
@@ -117,7 +118,7 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
     w.H = H;
     w.W = W;
     w.train = train;
-    w.X1 = w.get<uint16_t>(P * 32);
+    w.xb = w.get<uint16_t>(P);
     w.a1 = w.get<uint16_t>(P * C1);
     w.p = w.get<uint16_t>(Ph * C1);
     w.a2 = w.get<uint16_t>(Ph * C2);
@@ -126,11 +127,10 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
     w.y = w.get<float>(P);
     w.lpart = w.get<double>(1024 * 2);
     if (train) {
-        w.z1 = w.get<float>(P * C1);
-        w.z2 = w.get<float>(Ph * C2);
-        w.z3 = w.get<float>(P * C1);
+        w.g1 = w.get<float>(P * C1);
+        w.g2 = w.get<float>(Ph * C2);
+        w.g3 = w.get<float>(P * C1);
         w.dr = w.get<float>(P);
-        w.ds = w.get<float>(P * C1);
         w.dz3 = w.get<uint16_t>(P * C1);
         w.dz2 = w.get<uint16_t>(Ph * C2);
         w.dp = w.get<float>(Ph * C1);
@@ -140,28 +140,25 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
     }
 }
 
-// x (device, [N][H][W]) -> work.y; pre-activations kept when training.
+// x (device, [N][H][W]) -> work.y; gelu' of every pre-activation kept when training.
 void forward(ah_surrogate* S, const float* x, bool train) {
     Work& w = S->work;
     hipStream_t st = S->stream;
     Profiler* pf = &S->prof;
     const int N = w.N, H = w.H, W = w.W;
-    im2col1(st, pf, x, w.X1, N, H, W);
     ConvArgs c{};
-    c.X = w.X1;
     c.N = N;
     c.H = H;
     c.W = W;
     c.Cin = 32;
-    c.ks = 1;
+    c.ks = 5;
     c.Wt = S->pw.W1x;
     c.Cout = C1;
     c.bias = S->theta + OFF_B1;
     c.mode = CONV_GELU;
-    c.z = train ? w.z1 : nullptr;
+    c.dgelu = train ? w.g1 : nullptr;
     c.a = w.a1;
-    conv(st, pf, c, "surr_conv1");
-    pool2(st, pf, w.a1, w.p, N, H, W, C1);
+    conv1(st, pf, c, x, w.xb, w.p);
     c = ConvArgs{};
     c.X = w.p;
     c.N = N;
@@ -173,7 +170,7 @@ void forward(ah_surrogate* S, const float* x, bool train) {
     c.Cout = C2;
     c.bias = S->theta + OFF_B2;
     c.mode = CONV_GELU;
-    c.z = train ? w.z2 : nullptr;
+    c.dgelu = train ? w.g2 : nullptr;
     c.a = w.a2;
     conv(st, pf, c, "surr_conv2");
     c = ConvArgs{};
@@ -188,11 +185,11 @@ void forward(ah_surrogate* S, const float* x, bool train) {
     c.Cout = C1;
     c.bias = S->theta + OFF_B3;
     c.mode = CONV_GELU_ADD;
-    c.z = train ? w.z3 : nullptr;
+    c.dgelu = train ? w.g3 : nullptr;
     c.a = w.s;
     c.add = w.a1;
     conv(st, pf, c, "surr_conv3");
-    out_layer(st, pf, w.s, S->pw.W4, S->theta + OFF_B4, x, w.r, w.y, N, H, W);
+    out_layer(st, pf, w.s, S->pw.W4x, S->theta + OFF_B4, x, w.r, w.y, N, H, W);
 }
 
 // Loss of work.y against `target` and, when `grad` is set, the full
@@ -218,9 +215,8 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.Cin = C1;
         a.Cout = 1;
         wgrad(st, pf, a, w.part, chunks_for(1, C1), g + OFF_W4, w.bpart, g + OFF_B4);
-        out_input_grad(st, pf, w.dr, S->pw.W4, w.ds, N, H, W);
-        // layer 3
-        dz3(st, pf, w.ds, w.z3, w.dz3, P * C1);
+        // layer 3 (ds, the layer-4 input gradient, is formed from dr where it is used)
+        dz3_from_dr(st, pf, w.dr, S->pw.W4, w.g3, w.dz3, N, H, W);
         a = WGradArgs{};
         a.dz_h = w.dz3;
         a.X = w.a2;
@@ -244,7 +240,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         // dZ2 = (the input gradient summed over 2 x 2 blocks) * gelu'(z2),
         // formed in the convolution's epilogue.
         c.mode = CONV_POOL_DGELU;
-        c.zp = w.z2;
+        c.dgelu_in = w.g2;
         c.dpool = w.dz2;
         conv(st, pf, c, "surr_conv3_dgrad");
         // layer 2
@@ -271,12 +267,12 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         c.z = w.dp;
         conv(st, pf, c, "surr_conv2_dgrad");
         // layer 1
-        dz1(st, pf, w.ds, w.dp, w.z1, w.dz1, N, H, W, C1);
+        dz1_from_dr(st, pf, w.dr, S->pw.W4, w.dp, w.g1, w.dz1, N, H, W);
         a = WGradArgs{};
         a.dz_f = w.dz1;
-        a.X = w.X1;
-        a.xstride = 32;
-        a.xoff = 12;  // im2col channel of tap (0, 0)
+        a.X = w.xb;
+        a.xstride = 1;
+        a.xoff = 0;
         a.N = N;
         a.H = H;
         a.W = W;
@@ -351,6 +347,7 @@ ah_status ah_surrogate_new(uint32_t seed, ah_surrogate** out) {
         S->pw.W2T = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
         S->pw.W3 = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
         S->pw.W3T = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
+        S->pw.W4x = (uint16_t*)S->alloc(2 * 16 * 25 * 32);
         S->pw.W4 = (float*)S->alloc(4 * 25 * 32);
         std::vector<float> th;
         init_params(seed, th);

@@ -21,8 +21,11 @@ constexpr int N_PARAMS = OFF_B4 + 1;  // 104129
 
 // CONV_POOL_DGELU (the layer-3 input gradient): the output, summed over
 // 2 x 2 pixel blocks (the adjoint of the nearest upsampling) and multiplied
-// by gelu'(zp), goes to dpool at half resolution; nothing else is written.
-enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2, CONV_POOL_DGELU = 3 };
+// by dgelu_in (gelu' of layer 2's pre-activation), goes to dpool at half
+// resolution; nothing else is written.
+// CONV_OUT (layer 4, Cout 16 of which only channel 0 has weights): r =
+// channel 0 + bias[0] and y = clip(xin + r, 0, 1), one float each per pixel.
+enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2, CONV_POOL_DGELU = 3, CONV_OUT = 4 };
 
 struct ConvArgs {
     const uint16_t* X;   // bf16 NHWC input, Cin channels (half resolution when up)
@@ -32,11 +35,15 @@ struct ConvArgs {
     int Cout;            // 32 or 64
     const float* bias;   // [Cout] or null
     int mode;            // ConvMode
-    float* z;            // fp32 [N][H][W][Cout] or null
+    float* z;            // fp32 [N][H][W][Cout] or null: the output of CONV_LINEAR
+    float* dgelu;        // fp32 [N][H][W][Cout] or null: gelu'(z) of the GELU modes (kept for training)
     uint16_t* a;         // bf16 [N][H][W][Cout]: gelu(z) (+ add)
     const uint16_t* add; // bf16 [N][H][W][Cout] for CONV_GELU_ADD
-    const float* zp;     // fp32 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
+    const float* dgelu_in;  // fp32 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
     uint16_t* dpool;     // bf16 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
+    const float* xin;    // fp32 [N][H][W] for CONV_OUT: the network's input
+    float* rout;         // fp32 [N][H][W] for CONV_OUT: the residual r
+    float* yout;         // fp32 [N][H][W] for CONV_OUT: clip(xin + r, 0, 1)
 };
 
 struct WGradArgs {
@@ -53,21 +60,28 @@ struct PackedWeights {
     uint16_t* W2T;  // [32][25][64]
     uint16_t* W3;   // [32][25][64]
     uint16_t* W3T;  // [64][25][32]
+    uint16_t* W4x;  // [16][25][32]: W4 in row 0, rows 1..15 zero (the MFMA output layer)
     float* W4;      // [25][32]
 };
 
-void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, int H, int W);
+// Layer 1 (5x5, 1 -> 32, bias, GELU) straight from the fp32 input x; also
+// writes xb = bf16(x), the input the layer-1 weight gradient reads, and the
+// 2 x 2 average pool of the activations (bf16, half resolution).
+void conv1(hipStream_t s, Profiler* p, const ConvArgs& a, const float* x, uint16_t* xb, uint16_t* pool);
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name);
-void pool2(hipStream_t s, Profiler* p, const uint16_t* a, uint16_t* out, int N, int H, int W, int C);
-void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const float* W4, const float* b4, const float* x,
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4x, const float* b4, const float* x,
                float* r, float* y, int N, int H, int W);
 int loss_blocks(size_t total);
 void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const float* x, const float* r, float* dr,
           double* part, size_t total);
-void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W);
-void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total);
-void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
-         int C);
+// The layer-4 input gradient ds = sum over taps of W4 * dr is never stored:
+// dz3_from_dr forms it and writes dz3 = bf16(ds * g3); dz1_from_dr forms the
+// same values again and writes dz1 = (ds + dp / 4) * g1, with g1 and g3 the
+// gelu' values the forward pass kept.
+void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* g3, uint16_t* out, int N,
+                 int H, int W);
+void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const float* g1,
+                 float* out, int N, int H, int W);
 // Weight gradient `out` and bias gradient `bout` of one layer; `part` and
 // `bpart` (at least 2048 * 64 floats) hold per-workgroup partials.
 void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,

@@ -9,8 +9,8 @@
 // conv5 = 5x5 cross-correlation with circular padding, gelu = z*Phi(z).
 //
 // Layout: activations are NHWC, bf16 where a convolution reads them (a1, p,
-// a2, s, dz3, dz2 and the first layer's im2col), FP32 where only elementwise
-// code reads them (pre-activations z and the backward input gradients).
+// a2, s, dz3, dz2 and xb = bf16(x)), FP32 where only elementwise code reads
+// them (gelu' of the pre-activations, kept for the backward pass, and the backward input gradients).
 //
 // Hot kernels run on MFMA as an implicit GEMM, D[co][pixel] = sum over
 // (tap, ci) of W[co][tap][ci] * X[pixel+tap][ci], with
@@ -20,8 +20,10 @@
 // The upsampled input of layer 3 is never stored: the kernel reads the
 // half-resolution tensor at (y/2, x/2).  The backward input gradients of
 // layers 2 and 3 are the same kernel over tap-flipped, channel-swapped
-// weights.  The 1-channel output layer and the weight gradients (reductions
-// over pixels) are VALU kernels with fixed-order partial sums.
+// weights.  Layer 1 (one input channel) forms its 25-tap im2col fragments
+// in registers from an LDS tile of x; layer 4 (one output channel) runs as
+// 16 output channels of which 15 have zero weights.  The weight gradients
+// (reductions over pixels) use fixed-order partial sums.
 #include <algorithm>
 #include <cmath>
 
@@ -44,28 +46,13 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
 }
 __device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.3989422804014327f;
+// gelu(z) = z Phi(z) and, from the same erf, gelu'(z) = Phi(z) + z phi(z).
+__device__ __forceinline__ float gelu_and_grad(float z, float& dg) {
+    const float e = erff(z * 0.70710678118654752f);
+    dg = 0.5f * (1.0f + e) + z * __expf(-0.5f * z * z) * 0.3989422804014327f;
+    return 0.5f * z * (1.0f + e);
 }
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
-
-// ---------------------------------------------------------------- im2col
-// x fp32 [N][H][W] -> X1 bf16 [N][H][W][32]: channel t < 25 holds x at tap t.
-__global__ void k_im2col1(const float* __restrict__ x, uint16_t* __restrict__ X1, int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * 32;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i & 31);
-    const size_t pix = i >> 5;
-    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
-    uint16_t v = 0;
-    if (c < 25) {
-        const int dy = c / 5 - 2, dx = c % 5 - 2;
-        v = (uint16_t)bf16_bits(x[((size_t)n * H + wrap(yy + dy, H)) * W + wrap(xx + dx, W)]);
-    }
-    X1[i] = v;
-}
 
 // ---------------------------------------------------------------- MFMA conv
 // Bias, z, GELU (+ skip) and the bf16 activation of one wave's outputs: lane
@@ -74,6 +61,19 @@ template <int MT, int NB>
 __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NB], int n, int y, int x0,
                                               int q, int l16) {
     const int H = a.H, W = a.W, Cout = a.Cout;
+    if (a.mode == CONV_OUT) {
+        // Channel 0 sits in element 0 of the lanes with q == 0.
+        if (q == 0) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const size_t o = ((size_t)n * H + y) * W + x0 + 16 * b + l16;
+                const float r = acc[0][b][0] + a.bias[0];
+                a.rout[o] = r;
+                a.yout[o] = fminf(fmaxf(a.xin[o] + r, 0.0f), 1.0f);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -83,11 +83,11 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&a
             float z[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) z[r] = acc[m][b][r] + (a.bias ? a.bias[co + r] : 0.0f);
-            if (a.z) *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
             if (a.mode == CONV_GELU || a.mode == CONV_GELU_ADD) {
-                float v[4];
+                float v[4], dg[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu(z[r]);
+                for (int r = 0; r < 4; ++r) v[r] = gelu_and_grad(z[r], dg[r]);
+                if (a.dgelu) *reinterpret_cast<float4*>(a.dgelu + o) = make_float4(dg[0], dg[1], dg[2], dg[3]);
                 if (a.mode == CONV_GELU_ADD) {
                     const uint2 ad = *reinterpret_cast<const uint2*>(a.add + o);
                     v[0] += __uint_as_float(ad.x << 16);
@@ -97,59 +97,123 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&a
                 }
                 *reinterpret_cast<uint2*>(a.a + o) =
                     make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+            } else if (a.z) {
+                *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
             }
         }
 }
 
-// Wave: one output row, 64 consecutive pixels (4 blocks of 16), MT blocks of
-// 16 output channels.  Workgroup: 4 waves = 4 rows.  Grid (W/64, H/4, N).
-// Operands straight from global memory (the 1x1 first layer).
-template <int MT>
-__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
+// Layer 1 (5x5, 1 -> 32): the MFMA product over a 32-channel im2col
+// (channel t < 25 = bf16(x) at tap t, the rest zero), without the im2col in
+// memory.  Wave: one output row, 64 consecutive pixels (4 blocks of 16), 32
+// output channels; workgroup: 4 waves = 4 rows; grid (W/64, H/4, N).  The
+// workgroup stages its 8 x 68 window of bf16(x) in LDS, and lane (q, l16)
+// gathers its B fragment, taps 8q .. 8q+7 of pixel l16 of a block, from
+// there.  Besides gelu'(z1) (when a.dgelu is set) and a1 = bf16(gelu(z1)) it writes
+// xb = bf16(x) for its own 4 x 64 pixels and the 2 x 2 average pool of a1,
+// pool[y/2][x/2] = bf16(((a(x,y) + a(x+1,y)) + a(x,y+1)) + a(x+1,y+1)) / 4):
+// rows y and y+1 (waves 2j, 2j+1) meet in LDS, pixels x and x+1 (lanes l16
+// and l16 ^ 1) by a lane exchange.
+__global__ void __launch_bounds__(256) k_conv1(ConvArgs a, const float* __restrict__ x, uint16_t* __restrict__ xb,
+                                               uint16_t* __restrict__ pool) {
+    constexpr int NB = 4, TR = 8, TC = 16 * NB + 4;
+    __shared__ uint16_t tile[TR * TC];
+    __shared__ float odd_rows[2 * 2 * NB * 4 * 64];  // waves 1 and 3: their bf16 activations
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
-    const int n = blockIdx.z, y = blockIdx.y * 4 + wave, x0 = blockIdx.x * 64;
-    const int H = a.H, W = a.W, Cin = a.Cin, taps = a.ks * a.ks, half = a.ks / 2;
-    const int Hx = a.up ? H / 2 : H, Wx = a.up ? W / 2 : W;
-    const uint16_t* X = a.X + (size_t)n * Hx * Wx * Cin;
-    f32x4 acc[MT][4];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[m][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < taps; ++t) {
-        const int dy = t / a.ks - half, dx = t % a.ks - half;
-        int yy = wrap(y + dy, H);
-        if (a.up) yy >>= 1;
-        int px[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            int xx = wrap(x0 + 16 * b + l16 + dx, W);
-            if (a.up) xx >>= 1;
-            px[b] = (yy * Wx + xx) * Cin;
-        }
-        for (int c0 = 0; c0 < Cin; c0 += 32) {
-            bf16x8 A[MT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-                A[m] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(a.Wt + ((size_t)(16 * m + l16) * taps + t) * Cin + c0 + 8 * q));
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(X + px[b] + c0 + 8 * q));
-#pragma unroll
-                for (int m = 0; m < MT; ++m) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][b], 0, 0, 0);
-            }
-        }
+    const int n = blockIdx.z, y0 = blockIdx.y * 4, y = y0 + wave, x0 = blockIdx.x * (16 * NB);
+    const int H = a.H, W = a.W;
+    const float* xs = x + (size_t)n * H * W;
+    for (int i = threadIdx.x; i < TR * TC; i += 256) {
+        const int r = i / TC, c = i - r * TC;
+        const int yy = wrap(y0 - 2 + r, H), xx = wrap(x0 - 2 + c, W);
+        const uint16_t h = (uint16_t)bf16_bits(xs[(size_t)yy * W + xx]);
+        tile[i] = h;
+        if (r >= 2 && r < 6 && c >= 2 && c < TC - 2) xb[((size_t)n * H + yy) * W + xx] = h;
     }
-    conv_epilogue<MT, 4>(a, acc, n, y, x0, q, l16);
+    __syncthreads();
+    bf16x8 A[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        A[m] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(a.Wt + (size_t)(16 * m + l16) * 32 + 8 * q));
+    f32x4 acc[2][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        // Tap t = (dy + 2) * 5 + (dx + 2) of pixel (wave, 16 b + l16) is
+        // window pixel (wave + t / 5, 16 b + l16 + t % 5).
+        uint32_t h[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t v2[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int t = 8 * q + 2 * j + e;
+                v2[e] = t < 25 ? (uint32_t)tile[(wave + t / 5) * TC + 16 * b + l16 + t % 5] : 0u;
+            }
+            h[j] = v2[0] | (v2[1] << 16);
+        }
+        const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+            acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    }
+    // Lane holds channels 16m + 4q .. +3 of pixel x0 + 16b + l16 of row y.
+    float act[2][NB][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int co = 16 * m + 4 * q;
+            const size_t o = (((size_t)n * H + y) * W + x0 + 16 * b + l16) * 32 + co;
+            float dg[4];
+            uint32_t h[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                h[r] = bf16_bits(gelu_and_grad(acc[m][b][r] + a.bias[co + r], dg[r]));
+                act[m][b][r] = __uint_as_float(h[r] << 16);
+            }
+            if (a.dgelu) *reinterpret_cast<float4*>(a.dgelu + o) = make_float4(dg[0], dg[1], dg[2], dg[3]);
+            *reinterpret_cast<uint2*>(a.a + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+        }
+    float* sc = odd_rows + (wave >> 1) * (2 * NB * 4 * 64);
+    if (wave & 1) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sc[((m * NB + b) * 4 + r) * 64 + lane] = act[m][b][r];
+    }
+    __syncthreads();
+    if (!(wave & 1)) {
+        const int Hh = H / 2, Wh = W / 2, yh = y >> 1;
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                uint32_t h[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float v00 = act[m][b][r], v10 = __shfl_xor(v00, 1);
+                    const float* row1 = sc + ((m * NB + b) * 4 + r) * 64;
+                    const float v01 = row1[lane], v11 = row1[lane ^ 1];
+                    h[r] = bf16_bits(0.25f * (((v00 + v10) + v01) + v11));
+                }
+                if (!(l16 & 1)) {
+                    const int xh = (x0 + 16 * b + l16) >> 1, co = 16 * m + 4 * q;
+                    const size_t o = (((size_t)n * Hh + yh) * Wh + xh) * 32 + co;
+                    *reinterpret_cast<uint2*>(pool + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                }
+            }
+    }
 }
 
-// The 5x5 convolutions: the same waves and MFMAs as k_conv, with the input
-// staged in LDS.  Per 32-channel slice the workgroup loads its 8 x 68-pixel
-// input window (4 output rows plus the 2-row, 2-column halo) once, and every
-// tap's B fragments come from LDS: k_conv read them from the L1 cache for
-// every tap, 25 times per input.  An upsampled layer (UP) stages the 4 x 34
+// The 5x5 convolutions over 32 or 64 input channels.  Wave: one output
+// row, NB blocks of 16 pixels, MT blocks of 16 output channels; workgroup:
+// 4 waves = 4 rows.  Per 32-channel slice the workgroup loads its 8 x (16 NB
+// + 4)-pixel input window (4 output rows plus the 2-row, 2-column halo) once,
+// and every tap's B fragments come from LDS (read from the L1 cache for
+// every tap they were 25 loads per input).  An upsampled layer (UP) stages the 4 x 34
 // half-resolution pixels under that window and reads window pixel (r, c)
 // at (r/2, c/2).  Pixel stride 40 bf16 (80 bytes): a wave's 16-pixel
 // fragment reads fall on distinct banks.
@@ -235,9 +299,8 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
                     if (!(l16 & 1)) {
                         const int xh = (x0 + 16 * b + l16) >> 1, co = 16 * m + 4 * q;
                         const size_t o = (((size_t)n * Hh + yh) * Wh + xh) * a.Cout + co;
-                        const float4 zz = *reinterpret_cast<const float4*>(a.zp + o);
-                        const float g[4] = {v[0] * gelu_grad(zz.x), v[1] * gelu_grad(zz.y), v[2] * gelu_grad(zz.z),
-                                            v[3] * gelu_grad(zz.w)};
+                        const float4 dg = *reinterpret_cast<const float4*>(a.dgelu_in + o);
+                        const float g[4] = {v[0] * dg.x, v[1] * dg.y, v[2] * dg.z, v[3] * dg.w};
                         *reinterpret_cast<uint2*>(a.dpool + o) = make_uint2(bf16_bits(g[0]) | (bf16_bits(g[1]) << 16),
                                                                             bf16_bits(g[2]) | (bf16_bits(g[3]) << 16));
                     }
@@ -246,54 +309,6 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
         return;
     }
     conv_epilogue<MT, NB>(a, acc, n, y, x0, q, l16);
-}
-
-// ---------------------------------------------------------------- pooling
-// a1 bf16 [N][H][W][32] -> p bf16 [N][H/2][W/2][32], mean of the 2x2 block.
-__global__ void k_pool2(const uint16_t* __restrict__ a, uint16_t* __restrict__ p, int N, int H, int W, int C) {
-    const int Hh = H / 2, Wh = W / 2;
-    const size_t total = (size_t)N * Hh * Wh * C;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t pix = i / C;
-    const int xh = (int)(pix % Wh), yh = (int)((pix / Wh) % Hh), n = (int)(pix / ((size_t)Wh * Hh));
-    const size_t b = (((size_t)n * H + 2 * yh) * W + 2 * xh) * C + c;
-    const float s = bf16f(a[b]) + bf16f(a[b + C]) + bf16f(a[b + (size_t)W * C]) + bf16f(a[b + (size_t)W * C + C]);
-    p[i] = (uint16_t)bf16_bits(0.25f * s);
-}
-
-// ---------------------------------------------------------------- layer 4
-// r = conv5(s; W4 32->1) + b4, y = clip(x + r, 0, 1).  W4 fp32 [25][32]
-// (already bf16-rounded values).  One thread per pixel.
-__global__ void k_out_layer(const uint16_t* __restrict__ s, const float* __restrict__ W4, const float* __restrict__ b4p,
-                            const float* __restrict__ x, float* __restrict__ r, float* __restrict__ y, int N, int H,
-                            int W) {
-    __shared__ float w[25 * 32];
-    for (int i = threadIdx.x; i < 25 * 32; i += blockDim.x) w[i] = W4[i];
-    __syncthreads();
-    const size_t total = (size_t)N * H * W;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int xx = (int)(i % W), yy = (int)((i / W) % H), n = (int)(i / ((size_t)W * H));
-    float acc = *b4p;
-    for (int t = 0; t < 25; ++t) {
-        const int dy = t / 5 - 2, dx = t % 5 - 2;
-        const uint4* sp = reinterpret_cast<const uint4*>(
-            s + (((size_t)n * H + wrap(yy + dy, H)) * W + wrap(xx + dx, W)) * 32);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const uint4 v = sp[g];
-            const uint32_t e[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc += w[t * 32 + 8 * g + 2 * j] * __uint_as_float(e[j] << 16);
-                acc += w[t * 32 + 8 * g + 2 * j + 1] * __uint_as_float(e[j] & 0xffff0000u);
-            }
-        }
-    }
-    r[i] = acc;
-    y[i] = fminf(fmaxf(x[i] + acc, 0.0f), 1.0f);
 }
 
 // MSE loss: dr = 2 (y - t) / count where 0 < x + r < 1, else 0; per-block
@@ -321,52 +336,88 @@ __global__ void __launch_bounds__(256) k_loss(const float* __restrict__ y, const
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// ds = d(loss)/ds for layer 4: ds[p][ci] = sum_t W4[t][ci] dr[p - tap_t].
-// One thread per pixel: its 25 dr values serve all 32 channels (taps in the
-// same order as before, so each channel's sum is unchanged).
-__global__ void __launch_bounds__(256) k_out_input_grad(const float* __restrict__ dr, const float* __restrict__ W4,
-                                                        float* __restrict__ ds, int N, int H, int W) {
-    __shared__ float w[25 * 32];
-    for (int i = threadIdx.x; i < 25 * 32; i += blockDim.x) w[i] = W4[i];
+// ds = d(loss)/ds, the layer-4 input gradient: ds[p][ci] = sum over taps t
+// of W4[t][ci] dr[p - tap_t].  It is not stored: the two kernels below form
+// it where they need it.  Workgroup = one 64-pixel row segment (grid W/64,
+// H, N), thread = (pixel tid / 4, channels 8 (tid % 4) .. +7).  The segment's
+// 5 x 68 dr window and W4 sit in LDS; each thread adds its 25 taps in order,
+// one fused multiply-add each, so both kernels form the same values.
+namespace dz {
+constexpr int SEG = 64, COLS = SEG + 4;
+
+__device__ __forceinline__ void stage(float* w, float* drt, const float* __restrict__ W4, const float* __restrict__ dr,
+                                      int n, int y, int x0, int H, int W) {
+    for (int i = threadIdx.x; i < 25 * 32; i += 256) w[i] = W4[i];
+    for (int i = threadIdx.x; i < 5 * COLS; i += 256) {
+        const int r = i / COLS, c = i - r * COLS;
+        drt[i] = dr[((size_t)n * H + wrap(y - 2 + r, H)) * W + wrap(x0 - 2 + c, W)];
+    }
     __syncthreads();
-    const size_t total = (size_t)N * H * W;
-    const size_t pix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= total) return;
-    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
-    float acc[32];
+}
+
+// Tap t = (dy + 2) * 5 + (dx + 2) reads dr at (y - dy, x - dx): window row
+// 2 - dy, column p + 2 - dx.
+__device__ __forceinline__ void ds8(const float* w, const float* drt, int p, int c8, float (&acc)[8]) {
 #pragma unroll
-    for (int ci = 0; ci < 32; ++ci) acc[ci] = 0.0f;
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+#pragma unroll
     for (int t = 0; t < 25; ++t) {
         const int dy = t / 5 - 2, dx = t % 5 - 2;
-        const float d = dr[((size_t)n * H + wrap(yy - dy, H)) * W + wrap(xx - dx, W)];
+        const float d = drt[(2 - dy) * COLS + p + 2 - dx];
 #pragma unroll
-        for (int ci = 0; ci < 32; ++ci) acc[ci] += w[t * 32 + ci] * d;
+        for (int k = 0; k < 8; ++k) acc[k] = __fmaf_rn(w[t * 32 + 8 * c8 + k], d, acc[k]);
     }
-    float4* o = reinterpret_cast<float4*>(ds + pix * 32);
+}
+}  // namespace dz
+
+// dz3 = bf16(ds * gelu'(z3)), g3 = gelu'(z3) as the forward pass stored it.
+__global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, const float* __restrict__ W4,
+                                                const float* __restrict__ g3, uint16_t* __restrict__ dz3, int N, int H,
+                                                int W) {
+    __shared__ float w[25 * 32];
+    __shared__ float drt[5 * dz::COLS];
+    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * dz::SEG;
+    dz::stage(w, drt, W4, dr, n, y, x0, H, W);
+    const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
+    const size_t pix = ((size_t)n * H + y) * W + x0 + p;
+    float ds[8];
+    dz::ds8(w, drt, p, c8, ds);
+    const float4* gp = reinterpret_cast<const float4*>(g3 + pix * 32 + 8 * c8);
+    const float4 ga = gp[0], gb = gp[1];
+    const float g[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+    uint32_t h[8];
 #pragma unroll
-    for (int g = 0; g < 8; ++g) o[g] = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+    for (int k = 0; k < 8; ++k) h[k] = bf16_bits(ds[k] * g[k]);
+    *reinterpret_cast<uint4*>(dz3 + pix * 32 + 8 * c8) =
+        make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
-// dz3 = bf16(ds * gelu'(z3))
-__global__ void k_dz3(const float* __restrict__ ds, const float* __restrict__ z3, uint16_t* __restrict__ dz3,
-                      size_t total) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) dz3[i] = (uint16_t)bf16_bits(ds[i] * gelu_grad(z3[i]));
-}
-
-// dz2 = bf16((sum of du over the 2x2 children) * gelu'(z2)), half resolution, C = 64.
-
-// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, C = 32.
-__global__ void k_dz1(const float* __restrict__ ds, const float* __restrict__ dp, const float* __restrict__ z1,
-                      float* __restrict__ dz1, int N, int H, int W, int C) {
-    const size_t total = (size_t)N * H * W * C;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t pix = i / C;
-    const int xx = (int)(pix % W), yy = (int)((pix / W) % H), n = (int)(pix / ((size_t)W * H));
-    const size_t ph = (((size_t)n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
-    dz1[i] = (ds[i] + 0.25f * dp[ph]) * gelu_grad(z1[i]);
+// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, g1 = gelu'(z1) as stored (the
+// skip connection s = gelu(z3) + a1 passes ds to a1 unchanged; dp is the
+// pooled path's gradient at half resolution).
+__global__ void __launch_bounds__(256) k_dz1_dr(const float* __restrict__ dr, const float* __restrict__ W4,
+                                                const float* __restrict__ dp, const float* __restrict__ g1,
+                                                float* __restrict__ dz1, int N, int H, int W) {
+    __shared__ float w[25 * 32];
+    __shared__ float drt[5 * dz::COLS];
+    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * dz::SEG;
+    dz::stage(w, drt, W4, dr, n, y, x0, H, W);
+    const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
+    const size_t pix = ((size_t)n * H + y) * W + x0 + p;
+    float ds[8];
+    dz::ds8(w, drt, p, c8, ds);
+    const size_t ph = (((size_t)n * (H / 2) + y / 2) * (W / 2) + (x0 + p) / 2) * 32 + 8 * c8;
+    const float4* pp = reinterpret_cast<const float4*>(dp + ph);
+    const float4* gp = reinterpret_cast<const float4*>(g1 + pix * 32 + 8 * c8);
+    const float4 pa = pp[0], pb = pp[1], ga = gp[0], gb = gp[1];
+    const float d[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    const float g[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+    float o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = (ds[k] + 0.25f * d[k]) * g[k];
+    float4* op = reinterpret_cast<float4*>(dz1 + pix * 32 + 8 * c8);
+    op[0] = make_float4(o[0], o[1], o[2], o[3]);
+    op[1] = make_float4(o[4], o[5], o[6], o[7]);
 }
 
 // ---------------------------------------------------------------- weight grads
@@ -374,7 +425,7 @@ __global__ void k_dz1(const float* __restrict__ ds, const float* __restrict__ dp
 // Thread = 4 output channels x one (t, ci); blockIdx.y = pixel chunk; the
 // chunk partials go to part[chunk][co][t][ci] for a fixed-order reduction.
 // dZ is fp32 (dz_f) or bf16 (dz_h); X is bf16 NHWC with channel stride
-// xstride (the first layer reads channel 12, tap (0,0), of its im2col), at
+// xstride (the first layer reads xb = bf16(x), one channel), at
 // half resolution when `up` (the layer-3 input u = upsample2(a2)).
 __global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ part) {
     const int Cq = (a.Cout + 3) / 4;
@@ -526,8 +577,8 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
 
 // Weight gradients of the two thin layers, out[a][t][b] = sum over pixels p
 // of U[p][a] V[p + tap_t][b], with U the fp32 output gradient and V the bf16
-// layer input: layer 1 (A = 32 output channels, B = 1: V is the input x,
-// channel 12 of its im2col) and layer 4 (A = 1, B = 32).  A workgroup walks a
+// layer input: layer 1 (A = 32 output channels, B = 1: V is xb = bf16(x))
+// and layer 4 (A = 1, B = 32).  A workgroup walks a
 // fixed range of 64-pixel row segments, stages U and the 5 x 68 V tile the
 // taps reach in LDS as fp32, and thread (t, quad) accumulates the four
 // outputs a = 4 quad .. (A = 32) or b = 4 quad .. (B = 32) of tap t over the
@@ -678,8 +729,9 @@ __global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* 
 // Device copies of the master weights in the layouts the kernels read:
 // W1x [32][1][32] (im2col channel t), W2 [64][25][32], W3 [32][25][64] as
 // bf16; W2T [32][25][64] and W3T [64][25][32] tap-flipped and
-// channel-swapped for the backward input gradients; W4 [25][32] as fp32 of
-// the bf16-rounded values.
+// channel-swapped for the backward input gradients; W4x [16][25][32] bf16
+// (W4 in output channel 0, the rest zero) for the MFMA output layer; W4
+// [25][32] as fp32 of the bf16-rounded values.
 __global__ void k_pack_weights(const float* __restrict__ theta, PackedWeights w) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 32 * 32) {  // W1x[co][0][t]
@@ -699,6 +751,7 @@ __global__ void k_pack_weights(const float* __restrict__ theta, PackedWeights w)
         w.W3T[((size_t)ci * 25 + (24 - t)) * 32 + co] = h;
     }
     if (i < 25 * 32) w.W4[i] = __uint_as_float(bf16_bits(theta[OFF_W4 + i]) << 16);
+    if (i < 16 * 25 * 32) w.W4x[i] = i < 25 * 32 ? (uint16_t)bf16_bits(theta[OFF_W4 + i]) : (uint16_t)0;
 }
 
 // Crop a (side x side) window at (cx, cy) of a torus state into dst.
@@ -751,20 +804,26 @@ inline unsigned blocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / 
 
 }  // namespace
 
-void im2col1(hipStream_t s, Profiler* p, const float* x, uint16_t* X1, int N, int H, int W) {
-    Launch L(p, s, "surr_im2col1");
-    hipLaunchKernelGGL(k_im2col1, dim3(blocks((size_t)N * H * W * 32)), dim3(256), 0, s, x, X1, N, H, W);
+void conv1(hipStream_t s, Profiler* p, const ConvArgs& a, const float* x, uint16_t* xb, uint16_t* pool) {
+    if (a.W % 64 || a.H % 4 || a.Cout != 32 || !a.bias || !a.a || !xb || !pool)
+        throw Error{AH_ERR_INVALID, "surrogate layer 1: unsupported shape"};
+    Launch L(p, s, "surr_conv1");
+    hipLaunchKernelGGL(k_conv1, dim3(a.W / 64, a.H / 4, a.N), dim3(256), 0, s, a, x, xb, pool);
     AH_HIP(hipGetLastError());
 }
 
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
-    if (a.W % 64 || a.H % 4 || a.Cin % 32 || (a.Cout != 32 && a.Cout != 64) || (a.ks != 1 && a.ks != 5))
+    if (a.W % 64 || a.H % 4 || a.Cin % 32 || a.ks != 5 ||
+        (a.mode == CONV_OUT ? (a.Cout != 16 || a.Cin != 32 || a.up || !a.bias || !a.xin || !a.rout || !a.yout)
+                            : (a.Cout != 32 && a.Cout != 64)))
         throw Error{AH_ERR_INVALID, "surrogate conv: unsupported shape"};
     if (a.mode == CONV_POOL_DGELU && (a.ks != 5 || a.up))
         throw Error{AH_ERR_INVALID, "surrogate conv: the pooled epilogue needs a 5x5 full-resolution input"};
     Launch L(p, s, name);
     dim3 grid(a.W / 64, a.H / 4, a.N);
-    if (a.ks == 5) {
+    if (a.mode == CONV_OUT) {
+        hipLaunchKernelGGL((k_conv_lds<1, false, 4>), grid, dim3(256), 0, s, a);
+    } else {
         // The upsampled 32-channel layer: 128 pixels per wave when the width
         // allows, so each weight fragment feeds 16 MFMAs (1008 -> 803 us per
         // call at batch 64 x 256^2; the half-resolution 32-channel input
@@ -775,25 +834,27 @@ void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
         else if (a.up && a.W % 128 == 0) hipLaunchKernelGGL((k_conv_lds<2, true, 8>), grid8, dim3(256), 0, s, a);
         else if (a.up) hipLaunchKernelGGL((k_conv_lds<2, true, 4>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_conv_lds<2, false, 4>), grid, dim3(256), 0, s, a);
-    } else if (a.Cout == 64) {
-        hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a);
-    } else {
-        hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, a);
     }
     AH_HIP(hipGetLastError());
 }
 
-void pool2(hipStream_t s, Profiler* p, const uint16_t* a, uint16_t* out, int N, int H, int W, int C) {
-    Launch L(p, s, "surr_pool2");
-    hipLaunchKernelGGL(k_pool2, dim3(blocks((size_t)N * H * W * C / 4)), dim3(256), 0, s, a, out, N, H, W, C);
-    AH_HIP(hipGetLastError());
-}
-
-void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const float* W4, const float* b4, const float* x,
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4x, const float* b4, const float* x,
                float* r, float* y, int N, int H, int W) {
-    Launch L(p, s, "surr_out_layer");
-    hipLaunchKernelGGL(k_out_layer, dim3(blocks((size_t)N * H * W)), dim3(256), 0, s, sa, W4, b4, x, r, y, N, H, W);
-    AH_HIP(hipGetLastError());
+    ConvArgs c{};
+    c.X = sa;
+    c.N = N;
+    c.H = H;
+    c.W = W;
+    c.Cin = 32;
+    c.ks = 5;
+    c.Wt = W4x;
+    c.Cout = 16;
+    c.bias = b4;
+    c.mode = CONV_OUT;
+    c.xin = x;
+    c.rout = r;
+    c.yout = y;
+    conv(s, p, c, "surr_out_layer");
 }
 
 int loss_blocks(size_t total) { return (int)std::min<size_t>(1024, (total + 255) / 256); }
@@ -805,22 +866,19 @@ void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const floa
     AH_HIP(hipGetLastError());
 }
 
-void out_input_grad(hipStream_t s, Profiler* p, const float* dr, const float* W4, float* ds, int N, int H, int W) {
-    Launch L(p, s, "surr_out_input_grad");
-    hipLaunchKernelGGL(k_out_input_grad, dim3(blocks((size_t)N * H * W)), dim3(256), 0, s, dr, W4, ds, N, H, W);
-    AH_HIP(hipGetLastError());
-}
-
-void dz3(hipStream_t s, Profiler* p, const float* ds, const float* z3, uint16_t* out, size_t total) {
+void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* g3, uint16_t* out, int N,
+                 int H, int W) {
+    if (W % dz::SEG) throw Error{AH_ERR_INVALID, "surrogate dz3: width % 64 != 0"};
     Launch L(p, s, "surr_dz3");
-    hipLaunchKernelGGL(k_dz3, dim3(blocks(total)), dim3(256), 0, s, ds, z3, out, total);
+    hipLaunchKernelGGL(k_dz3_dr, dim3(W / dz::SEG, H, N), dim3(256), 0, s, dr, W4, g3, out, N, H, W);
     AH_HIP(hipGetLastError());
 }
 
-void dz1(hipStream_t s, Profiler* p, const float* ds, const float* dp, const float* z1, float* out, int N, int H, int W,
-         int C) {
+void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const float* g1,
+                 float* out, int N, int H, int W) {
+    if (W % dz::SEG || H % 2) throw Error{AH_ERR_INVALID, "surrogate dz1: width % 64 != 0 or odd height"};
     Launch L(p, s, "surr_dz1");
-    hipLaunchKernelGGL(k_dz1, dim3(blocks((size_t)N * H * W * C)), dim3(256), 0, s, ds, dp, z1, out, N, H, W, C);
+    hipLaunchKernelGGL(k_dz1_dr, dim3(W / dz::SEG, H, N), dim3(256), 0, s, dr, W4, dp, g1, out, N, H, W);
     AH_HIP(hipGetLastError());
 }
 
