@@ -489,6 +489,74 @@ def torch_leg(tmp: str) -> int:
     return 0
 
 
+def time_torch_surrogate(batch: int = 64, side: int = 256, steps: int = 30):
+    """The surrogate's network written in PyTorch (torch.nn.Conv2d on MIOpen,
+    bf16 weights and activations, circular padding, exact GELU, 2 x 2 average
+    pool, nearest upsampling, the skip and the clipped residual), one Adam
+    step on a batch of random states: ms per step after 5 warm-up steps (the
+    first ones run MIOpen's algorithm search, which takes minutes on a fresh
+    machine).  Child process, as time_torch_fft."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--torch-surrogate-leg",
+                        f"{batch},{side},{steps}"], stdout=subprocess.PIPE, text=True, timeout=1100)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch surrogate leg failed ({r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def torch_surrogate_leg(spec: str) -> int:
+    """Child process of time_torch_surrogate: PyTorch only."""
+    import torch
+    import torch.nn.functional as F
+    batch, side, steps = (int(v) for v in spec.split(","))
+    dev = torch.device("cuda:0")
+    torch.manual_seed(0)
+
+    class Net(torch.nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.c1 = torch.nn.Conv2d(1, 32, 5)
+            self.c2 = torch.nn.Conv2d(32, 64, 5)
+            self.c3 = torch.nn.Conv2d(64, 32, 5)
+            self.c4 = torch.nn.Conv2d(32, 1, 5)
+
+        def forward(self, x):
+            def pad(t):
+                return F.pad(t, (2, 2, 2, 2), mode="circular")
+            a1 = F.gelu(self.c1(pad(x)))
+            a2 = F.gelu(self.c2(pad(F.avg_pool2d(a1, 2))))
+            s = F.gelu(self.c3(pad(F.interpolate(a2, scale_factor=2, mode="nearest")))) + a1
+            return torch.clamp(x + self.c4(pad(s)), 0.0, 1.0)
+
+    net = Net().to(dev).to(torch.bfloat16)
+    params = sum(p.numel() for p in net.parameters())
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    x = torch.rand(batch, 1, side, side, device=dev, dtype=torch.bfloat16)
+    t = torch.rand_like(x)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        loss = F.mse_loss(net(x).float(), t.float())
+        loss.backward()
+        opt.step()
+
+    t0 = time.perf_counter()
+    for i in range(5):
+        step()
+        torch.cuda.synchronize()
+        log(f"torch surrogate warm-up step {i + 1}/5 done at {time.perf_counter() - t0:.1f} s")
+    warm_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    print(json.dumps({"train_ms_per_step": 1e3 * dt_s / steps, "batch": batch, "crop": side, "steps": steps,
+                      "params": params, "warmup_s": warm_s, "torch": torch.__version__,
+                      "dtype": "bf16 weights and activations"}), flush=True)
+    return 0
+
+
 def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 256, steps: int = 1000):
     """Learned 4-step surrogate: pairs from the 4096^2 three-ring trajectory,
     Adam training, then `steps` judged steps at 4096^2 beside plain stepping."""
@@ -679,9 +747,15 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 checks")
     ap.add_argument("--torch-leg", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--torch-surrogate", action="store_true",
+                    help="also train the surrogate's network in PyTorch (MIOpen) for comparison (minutes of "
+                         "MIOpen algorithm search on a fresh machine)")
+    ap.add_argument("--torch-surrogate-leg", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.torch_leg:
         return torch_leg(args.torch_leg)
+    if args.torch_surrogate_leg:
+        return torch_surrogate_leg(args.torch_surrogate_leg)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this engine has no multi-GPU path (asked for {args.gpus}). "
               "A batch of worlds shards by world and one large world by slab FFT, but neither is built.",
@@ -690,7 +764,11 @@ def main() -> int:
     rank = int(os.environ.get("RANK", "0"))
 
     if args.surrogate_only:
-        print(json.dumps({"surrogate": time_surrogate(args.seed, args.surrogate_train_steps)}), flush=True)
+        sur = time_surrogate(args.seed, args.surrogate_train_steps)
+        if args.torch_surrogate:
+            sur["torch_same_network"] = time_torch_surrogate()
+            sur["torch_over_engine_train_ms"] = sur["torch_same_network"]["train_ms_per_step"] / sur["train_ms_per_step"]
+        print(json.dumps({"surrogate": sur}), flush=True)
         return 0
     if args.batch_only:
         print(json.dumps({"batch": batch_profile_run(args.seed, args.steps)}), flush=True)
@@ -774,6 +852,11 @@ def main() -> int:
         out["surrogate"] = time_surrogate(args.seed, args.surrogate_train_steps)
         log(f"surrogate: {out['surrogate']['value']:.3e} cell-updates/s judged, "
             f"{out['surrogate']['advance']['fallbacks']} fallback(s)")
+        if args.torch_surrogate:
+            ts = time_torch_surrogate()
+            out["surrogate"]["torch_same_network"] = ts
+            out["surrogate"]["torch_over_engine_train_ms"] = ts["train_ms_per_step"] / out["surrogate"]["train_ms_per_step"]
+            log(f"surrogate network in PyTorch: {ts['train_ms_per_step']:.2f} ms per Adam step")
     if not args.no_cpu and rank == 0:
         log("FP64 checks along the runs")
         out["run_4096"] = run_4096(args.seed)
