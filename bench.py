@@ -489,6 +489,49 @@ def torch_leg(tmp: str) -> int:
     return 0
 
 
+def time_torch_conv(side: int = 4096, radius: int = 26, steps: int = 20):
+    """The direct path's convolution through the library: torch conv2d
+    (MIOpen) of a side^2 torus with one (2 radius + 1)^2 kernel, circular
+    padding, bf16 and fp32, ms per convolution (the convolution alone, no
+    growth or update) after 3 warm-up calls (MIOpen's algorithm search takes
+    minutes on a fresh machine).  Child process, as time_torch_fft."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--torch-conv-leg", f"{side},{radius},{steps}"],
+                       stdout=subprocess.PIPE, text=True, timeout=1100)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch conv leg failed ({r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def torch_conv_leg(spec: str) -> int:
+    """Child process of time_torch_conv: PyTorch only."""
+    import torch
+    import torch.nn.functional as F
+    side, radius, steps = (int(v) for v in spec.split(","))
+    dev = torch.device("cuda:0")
+    out = {"side": side, "kernel": 2 * radius + 1, "torch": torch.__version__}
+    for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        x = torch.rand(1, 1, side, side, device=dev, dtype=dt)
+        k = torch.rand(1, 1, 2 * radius + 1, 2 * radius + 1, device=dev, dtype=dt)
+
+        def conv():
+            return F.conv2d(F.pad(x, (radius, radius, radius, radius), mode="circular"), k)
+
+        t0 = time.perf_counter()
+        for i in range(3):
+            conv()
+            torch.cuda.synchronize()
+            log(f"torch conv2d {name} warm-up call {i + 1}/3 done at {time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            conv()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_conv": ms, "cell_updates_per_s": side * side / (ms * 1e-3)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def time_torch_surrogate(batch: int = 64, side: int = 256, steps: int = 30):
     """The surrogate's network written in PyTorch (torch.nn.Conv2d on MIOpen,
     bf16 weights and activations, circular padding, exact GELU, 2 x 2 average
@@ -751,11 +794,17 @@ def main() -> int:
                     help="also train the surrogate's network in PyTorch (MIOpen) for comparison (minutes of "
                          "MIOpen algorithm search on a fresh machine)")
     ap.add_argument("--torch-surrogate-leg", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--torch-conv", action="store_true",
+                    help="also time the direct path's 53x53 convolution as torch conv2d (MIOpen) for comparison "
+                         "(minutes of MIOpen algorithm search on a fresh machine)")
+    ap.add_argument("--torch-conv-leg", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.torch_leg:
         return torch_leg(args.torch_leg)
     if args.torch_surrogate_leg:
         return torch_surrogate_leg(args.torch_surrogate_leg)
+    if args.torch_conv_leg:
+        return torch_conv_leg(args.torch_conv_leg)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this engine has no multi-GPU path (asked for {args.gpus}). "
               "A batch of worlds shards by world and one large world by slab FFT, but neither is built.",
@@ -817,6 +866,12 @@ def main() -> int:
                   for p, v in results.items()},
         "direct_over_fft": results["direct_bf16"]["value"] / results["fft_fp32"]["value"],
     }
+    if args.torch_conv:
+        tc = time_torch_conv(args.side)
+        tc["direct_step_over_conv2d_bf16"] = tc["bf16"]["ms_per_conv"] / results["direct_bf16"]["ms_per_step"]
+        out["paths"]["direct_bf16"]["torch_conv2d"] = tc
+        log(f"torch conv2d {tc['kernel']}^2 at {tc['side']}^2: bf16 {tc['bf16']['ms_per_conv']:.2f} ms, fp32 "
+            f"{tc['fp32']['ms_per_conv']:.2f} ms per convolution")
     if args.paths_only:
         out["cpu_baseline"] = None
         print(json.dumps(out), flush=True)
