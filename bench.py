@@ -600,12 +600,21 @@ def torch_surrogate_leg(spec: str) -> int:
     return 0
 
 
-def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 256, steps: int = 1000):
+def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 256, steps: int = 1000,
+                   radius: float = 0.0):
     """Learned 4-step surrogate: pairs from the 4096^2 three-ring trajectory,
-    Adam training, then `steps` judged steps at 4096^2 beside plain stepping."""
+    Adam training, then `steps` judged steps at 4096^2 beside plain stepping.
+    With `radius` set, the world is instead a 1024^2 one-ring world of that
+    radius (the three-ring world's mu, sigma and dt, seeded with uniform noise
+    everywhere), so that 4 steps depend on a neighbourhood the network's
+    receptive field (about 11 cells) covers when radius <= ~3."""
     import numpy as np
     from auto_amd import Sim, Surrogate, workloads as W
-    d = W.rings3_4096()
+    if radius > 0:
+        d = W.desc(1024, 1024, [dict(radius=float(radius), beta=[1.0], mu=0.26, sigma=0.036)], 0.1,
+                   init=W.INIT_PATCH, init_size=1024)
+    else:
+        d = W.rings3_4096()
     gen = Sim(d, seed + 100)
     gen.step(50)
     net = Surrogate(seed=seed)
@@ -636,7 +645,8 @@ def time_surrogate(seed: int, train_steps: int, pairs: int = 20000, crop: int = 
     t4 = time.perf_counter()
     sa, sb = a.read_field("state"), b.read_field("state")
     cells = d.width * d.height
-    return {"pairs": pairs, "crop": crop, "collect_s": t1 - t0, "train_steps": train_steps, "batch": 64,
+    return {"world": f"{d.width}x{d.height}, R={d.kernels[0].radius:g}, {d.kernels[0].rings} ring(s)",
+            "pairs": pairs, "crop": crop, "collect_s": t1 - t0, "train_steps": train_steps, "batch": 64,
             "train_s": t2 - t1, "train_ms_per_step": 1e3 * (t2 - t1) / max(train_steps, 1),
             "loss_first10": float(np.mean(losses[:10])), "loss_last10": float(np.mean(losses[-10:])),
             "held_out_mse": held_mse, "held_out_identity_mse": float(np.mean((xh - yh) ** 2)),
@@ -782,6 +792,9 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned surrogate section")
     ap.add_argument("--surrogate", action="store_true", help=argparse.SUPPRESS)  # on by default now
     ap.add_argument("--surrogate-train-steps", type=int, default=1000)
+    ap.add_argument("--surrogate-radius", type=float, default=0.0,
+                    help="with --surrogate-only: train and judge on a 1024^2 one-ring world of this radius instead "
+                         "of the 4096^2 three-ring world")
     ap.add_argument("--surrogate-only", action="store_true",
                     help="run only the surrogate section and print it (for profiler runs)")
     ap.add_argument("--batch-only", action="store_true",
@@ -813,7 +826,7 @@ def main() -> int:
     rank = int(os.environ.get("RANK", "0"))
 
     if args.surrogate_only:
-        sur = time_surrogate(args.seed, args.surrogate_train_steps)
+        sur = time_surrogate(args.seed, args.surrogate_train_steps, radius=args.surrogate_radius)
         if args.torch_surrogate:
             sur["torch_same_network"] = time_torch_surrogate()
             sur["torch_over_engine_train_ms"] = sur["torch_same_network"]["train_ms_per_step"] / sur["train_ms_per_step"]
