@@ -575,88 +575,149 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
         for (int r = 0; r < 4; ++r) dst[((size_t)(16 * cob + 4 * q + r) * 25 + t) * CIN + 16 * cib + l16] = acc[t][r];
 }
 
-// Weight gradients of the two thin layers, out[a][t][b] = sum over pixels p
-// of U[p][a] V[p + tap_t][b], with U the fp32 output gradient and V the bf16
-// layer input: layer 1 (A = 32 output channels, B = 1: V is xb = bf16(x))
-// and layer 4 (A = 1, B = 32).  A workgroup walks a
-// fixed range of 64-pixel row segments, stages U and the 5 x 68 V tile the
-// taps reach in LDS as fp32, and thread (t, quad) accumulates the four
-// outputs a = 4 quad .. (A = 32) or b = 4 quad .. (B = 32) of tap t over the
-// segment's pixels in order.  Partials go to part[workgroup][a][t][b].
-template <int A, int B>
-__global__ void __launch_bounds__(256) k_wgrad_thin(WGradArgs a, float* __restrict__ part, float* __restrict__ bpart) {
+// The thin layers' weight gradients on MFMA.  With U the fp32 output
+// gradient and V the bf16 layer input, out[a][t][b] = sum over pixels p of
+// U[p][a] V[p + tap_t][b] is a product over K = pixels:
+//   layer 4 (A = 1, B = 32): M = tap t (25 of 32 rows), N = channel b;
+//     A-operand U[q - tap_t] (pixel q = p + tap_t), B-operand V[q][b];
+//   layer 1 (A = 32, B = 1): M = channel a, N = tap t (25 of 32 columns);
+//     A-operand U[p][a], B-operand V[p + tap_t].
+// U enters as two bf16 terms, hi = bf16(U) and lo = bf16(U - hi), so each
+// product keeps about 16 bits of U; V is bf16 already.  A workgroup walks a
+// fixed range of 64-pixel row segments (partials in workgroup order, so the
+// result does not depend on timing); per segment it
+// stages U and V in LDS (the operand read along pixels as [channel][pixel],
+// the shifted one as its 5 x 68 tile) and each of its 4 waves adds two
+// 32-pixel chunks into its own 16 x 16 output block (mb = wave / 2, nb =
+// wave % 2).  The bias gradient's partial (the sum of U over the workgroup's
+// pixels, segment by segment in pixel order) goes to bpart.
+namespace wt {
+constexpr int VS = wg::SEG + 8;  // layer 4: bf16 per channel row of V^T
+constexpr int US = wg::SEG + 4;  // layer 1: fp32 per channel row of U^T
+}
+
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+    uint32_t h[8], l[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        h[j] = bf16_bits(v[j]);
+        l[j] = bf16_bits(v[j] - __uint_as_float(h[j] << 16));
+    }
+    hi = __builtin_bit_cast(bf16x8, make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                                               h[6] | (h[7] << 16)));
+    lo = __builtin_bit_cast(bf16x8, make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16),
+                                               l[6] | (l[7] << 16)));
+}
+
+template <int LAYER>
+__global__ void __launch_bounds__(256) k_wgrad_thin_mfma(WGradArgs a, float* __restrict__ part,
+                                                          float* __restrict__ bpart) {
     using wg::SEG;
     using wg::ROWS;
     using wg::COLS;
-    static_assert((A == 32 && B == 1) || (A == 1 && B == 32), "thin layers");
-    __shared__ __attribute__((aligned(16))) float u[SEG * A];
-    __shared__ __attribute__((aligned(16))) float v[ROWS * COLS * B];
+    static_assert(LAYER == 1 || LAYER == 4, "thin layers");
+    constexpr int A = LAYER == 1 ? 32 : 1;
+    // Layer 4: U tile [ROWS][COLS] fp32 and V^T [32][VS] bf16.
+    // Layer 1: U^T [32][US] fp32 and V tile [ROWS][COLS] bf16.
+    __shared__ __attribute__((aligned(16))) float uf[LAYER == 4 ? ROWS * COLS : 32 * wt::US];
+    __shared__ __attribute__((aligned(16))) uint16_t vh[LAYER == 4 ? 32 * wt::VS : ROWS * COLS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int mb = wave >> 1, nb = wave & 1;
     const int H = a.H, W = a.W, segs_per_row = W / SEG;
     const long nseg = (long)a.N * H * segs_per_row;
     const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
-    const int t = threadIdx.x / 8, quad = threadIdx.x % 8;  // t < 25 for the 200 working threads
-    const bool active = t < 25;
-    const int dy = t / 5, dx = t % 5;  // tile offsets of the tap (tap + 2)
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float bacc = 0.0f;  // bias gradient of output channel threadIdx.x (< A) over this workgroup's pixels
+    // This lane's tap (row m of layer 4's A operand, column n of layer 1's B operand).
+    const int t = LAYER == 4 ? 16 * mb + l16 : 16 * nb + l16;
+    const bool tap_ok = t < 25;
+    const int dy = tap_ok ? t / 5 - 2 : 0, dx = tap_ok ? t % 5 - 2 : 0;
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;
     for (long sg = s0; sg < s1; ++sg) {
         const int xb = (int)(sg % segs_per_row);
         const long ry = sg / segs_per_row;
         const int y = (int)(ry % H), n = (int)(ry / H), x0 = xb * SEG;
         __syncthreads();  // the previous segment's reads are done
-        for (int i = threadIdx.x; i < SEG * A; i += 256)
-            u[i] = a.dz_f[(((size_t)n * H + y) * W + x0) * A + i];
-        if constexpr (B == 1) {
+        if constexpr (LAYER == 4) {
             for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
-                const int r = i / COLS, cc = i % COLS;
-                const int sy = wrap(y + r - 2, H), sx = wrap(x0 + cc - 2, W);
-                v[i] = bf16f(a.X[(((size_t)n * H + sy) * W + sx) * a.xstride + a.xoff]);
+                const int r = i / COLS, c = i % COLS;
+                uf[i] = a.dz_f[((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W)];
+            }
+            {   // V^T: thread = (pixel, 8 channels)
+                const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
+                const uint4 v = *reinterpret_cast<const uint4*>(
+                    a.X + (((size_t)n * H + y) * W + x0 + p) * a.xstride + a.xoff + 8 * c8);
+                const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    vh[(8 * c8 + 2 * j) * wt::VS + p] = (uint16_t)(e[j] & 0xffffu);
+                    vh[(8 * c8 + 2 * j + 1) * wt::VS + p] = (uint16_t)(e[j] >> 16);
+                }
             }
         } else {
-            // 16 bytes (8 channels) per load, widened to two float4.
-            for (int i = threadIdx.x; i < ROWS * COLS * (B / 8); i += 256) {
-                const int pidx = i / (B / 8), c8 = i % (B / 8), r = pidx / COLS, cc = pidx % COLS;
-                const int sy = wrap(y + r - 2, H), sx = wrap(x0 + cc - 2, W);
-                const uint4 h = *reinterpret_cast<const uint4*>(
-                    a.X + (((size_t)n * H + sy) * W + sx) * a.xstride + a.xoff + 8 * c8);
-                float4* o = reinterpret_cast<float4*>(v + pidx * B + 8 * c8);
-                o[0] = make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xffff0000u),
-                                   __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xffff0000u));
-                o[1] = make_float4(__uint_as_float(h.z << 16), __uint_as_float(h.z & 0xffff0000u),
-                                   __uint_as_float(h.w << 16), __uint_as_float(h.w & 0xffff0000u));
+            {   // U^T: thread = (pixel, 8 channels)
+                const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
+                const float4* src = reinterpret_cast<const float4*>(a.dz_f + (((size_t)n * H + y) * W + x0 + p) * 32 +
+                                                                    8 * c8);
+                const float4 u0 = src[0], u1 = src[1];
+                const float e[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) uf[(8 * c8 + j) * wt::US + p] = e[j];
+            }
+            for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
+                const int r = i / COLS, c = i % COLS;
+                vh[i] = a.X[(((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W)) * a.xstride + a.xoff];
             }
         }
         __syncthreads();
-        if ((int)threadIdx.x < A)
-            for (int p = 0; p < SEG; ++p) bacc += u[p * A + threadIdx.x];
-        if (active) {
-            for (int p = 0; p < SEG; ++p) {
-                const int vp = (dy * COLS + p + dx) * B;  // V at pixel p + tap
-                if constexpr (A == 32) {
-                    const float4 uu = *reinterpret_cast<const float4*>(u + p * A + 4 * quad);
-                    const float vv = v[vp];
-                    acc[0] += uu.x * vv;
-                    acc[1] += uu.y * vv;
-                    acc[2] += uu.z * vv;
-                    acc[3] += uu.w * vv;
-                } else {
-                    const float uu = u[p];
-                    const float4 vv = *reinterpret_cast<const float4*>(v + vp + 4 * quad);
-                    acc[0] += uu * vv.x;
-                    acc[1] += uu * vv.y;
-                    acc[2] += uu * vv.z;
-                    acc[3] += uu * vv.w;
+        if constexpr (LAYER == 4) {
+            if (threadIdx.x == 0)
+                for (int p = 0; p < SEG; ++p) bacc += uf[2 * COLS + 2 + p];
+        } else {
+            if ((int)threadIdx.x < A)
+                for (int p = 0; p < SEG; ++p) bacc += uf[threadIdx.x * wt::US + p];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int k0 = 32 * c + 8 * q;  // this lane's first pixel of the chunk
+            float u8[8];
+            bf16x8 B;
+            if constexpr (LAYER == 4) {
+                // A: U at (y - dy, x0 + k0 + j - dx), tile row 2 - dy, column k0 + j - dx + 2.
+#pragma unroll
+                for (int j = 0; j < 8; ++j) u8[j] = tap_ok ? uf[(2 - dy) * COLS + k0 + j - dx + 2] : 0.0f;
+                B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(vh + (16 * nb + l16) * wt::VS + k0));
+            } else {
+                const float4* up = reinterpret_cast<const float4*>(uf + (16 * mb + l16) * wt::US + k0);
+                const float4 u0 = up[0], u1 = up[1];
+                u8[0] = u0.x; u8[1] = u0.y; u8[2] = u0.z; u8[3] = u0.w;
+                u8[4] = u1.x; u8[5] = u1.y; u8[6] = u1.z; u8[7] = u1.w;
+                // B: V at (y + dy, x0 + k0 + j + dx), tile row dy + 2, column k0 + j + dx + 2.
+                uint32_t h[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t v0 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + dx + 2] : 0u;
+                    const uint32_t v1 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + 1 + dx + 2] : 0u;
+                    h[j] = v0 | (v1 << 16);
                 }
+                B = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
             }
+            bf16x8 hi, lo;
+            split8(u8, hi, lo);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B, acc, 0, 0, 0);
         }
     }
     if ((int)threadIdx.x < A) bpart[(size_t)blockIdx.x * A + threadIdx.x] = bacc;
-    if (active) {
-        float* dst = part + (size_t)blockIdx.x * A * 25 * B;
+    // D: rows 16 mb + 4 q + r, column 16 nb + l16.
+    float* dst = part + (size_t)blockIdx.x * 800;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if constexpr (A == 32) dst[(size_t)(4 * quad + k) * 25 + t] = acc[k];
-            else dst[(size_t)t * B + 4 * quad + k] = acc[k];
+    for (int r = 0; r < 4; ++r) {
+        const int m = 16 * mb + 4 * q + r, nn = 16 * nb + l16;
+        if constexpr (LAYER == 4) {
+            if (m < 25) dst[m * 32 + nn] = acc[r];
+        } else {
+            if (nn < 25) dst[m * 25 + nn] = acc[r];
         }
     }
 }
@@ -894,9 +955,9 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         chunks = wg::THIN_GROUPS;
         Launch L(p, s, "surr_wgrad_thin");
         if (a.Cout == 32)
-            hipLaunchKernelGGL((k_wgrad_thin<32, 1>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
+            hipLaunchKernelGGL((k_wgrad_thin_mfma<1>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
         else
-            hipLaunchKernelGGL((k_wgrad_thin<1, 32>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
+            hipLaunchKernelGGL((k_wgrad_thin_mfma<4>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else if (mfma) {
         chunks = wg::GROUPS;
