@@ -502,8 +502,11 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
                                                             float* __restrict__ bpart) {
     using namespace wg;
     static_assert((COUT / 16) * (CIN / 16) == THREADS / 64, "one 16x16 block per wave");
-    constexpr int XS = CIN + 8, ZS = COUT + 8;  // LDS strides per pixel (bf16)
-    __shared__ __attribute__((aligned(16))) uint16_t xt[ROWS * COLS * XS];
+    static_assert(COLS % 2 == 0 && 32 * (SEG / 32 - 1) + 8 * 3 + 16 <= COLS + 4, "tile columns");
+    // X tile transposed, [ci][row][column] (CS bf16 per row), so a lane's 8
+    // consecutive pixels of one channel are contiguous; dZ per pixel (ZS).
+    constexpr int CS = COLS + 4, ZS = COUT + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t xt[CIN * ROWS * CS];
     __shared__ __attribute__((aligned(16))) uint16_t dz[SEG * ZS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
@@ -516,33 +519,78 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
 #pragma unroll
     for (int t = 0; t < 25; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bacc = 0.0f;  // bias gradient of channel threadIdx.x (< COUT) over this workgroup's pixels
-    for (long sg = s0; sg < s1; ++sg) {
+    // A segment's global loads are issued before the previous segment's
+    // MFMAs and written to LDS after them (registers in between), so their
+    // latency hides behind the MFMAs.  dZ: 64 pixels x COUT, contiguous in
+    // global memory; X tile: rows y-2..y+2, columns x0-2..x0+65, wrapped on
+    // the torus, transposed on the way in: a thread takes 8 channels of two
+    // adjacent pixels and writes each channel's pair as one 32-bit word.
+    constexpr int DZ_V = SEG * COUT / 8, X_V = ROWS * (COLS / 2) * (CIN / 8);
+    constexpr int NDZ = (DZ_V + THREADS - 1) / THREADS, NX = (X_V + THREADS - 1) / THREADS;
+    uint4 pdz[NDZ], px0[NX], px1[NX];
+    auto fetch = [&](long sg) {
         const int xb = (int)(sg % segs_per_row);
         const long ry = sg / segs_per_row;
         const int y = (int)(ry % H), n = (int)(ry / H), x0 = xb * SEG;
-        __syncthreads();  // the previous segment's LDS reads are done
-        // dZ: 64 pixels x COUT, contiguous in global memory.
-        {
-            const uint16_t* src = a.dz_h + (((size_t)n * H + y) * W + x0) * COUT;
-            for (int v = threadIdx.x; v < SEG * COUT / 8; v += THREADS) {
+        const uint16_t* src = a.dz_h + (((size_t)n * H + y) * W + x0) * COUT;
+#pragma unroll
+        for (int i = 0; i < NDZ; ++i) {
+            const int v = threadIdx.x + i * THREADS;
+            if (v < DZ_V) {
                 const int px = v / (COUT / 8), part8 = v % (COUT / 8);
-                *reinterpret_cast<uint4*>(dz + px * ZS + 8 * part8) =
-                    *reinterpret_cast<const uint4*>(src + (size_t)px * COUT + 8 * part8);
+                pdz[i] = *reinterpret_cast<const uint4*>(src + (size_t)px * COUT + 8 * part8);
             }
         }
-        // X tile: rows y-2..y+2, columns x0-2..x0+65, wrapped on the torus.
-        for (int v = threadIdx.x; v < ROWS * COLS * CIN / 8; v += THREADS) {
-            const int pidx = v / (CIN / 8), part8 = v % (CIN / 8);
-            const int r = pidx / COLS, c = pidx % COLS;
-            int sy = wrap(y + r - 2, H), sx = wrap(x0 + c - 2, W);
-            if (UP) {
-                sy >>= 1;
-                sx >>= 1;
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            const int v = threadIdx.x + i * THREADS;
+            if (v < X_V) {
+                const int pair = v % (ROWS * (COLS / 2)), part8 = v / (ROWS * (COLS / 2));
+                const int r = pair / (COLS / 2), c = 2 * (pair % (COLS / 2));
+                int sy = wrap(y + r - 2, H), sx0 = wrap(x0 + c - 2, W), sx1 = wrap(x0 + c - 1, W);
+                if (UP) {
+                    sy >>= 1;
+                    sx0 >>= 1;
+                    sx1 >>= 1;
+                }
+                const uint16_t* row = a.X + ((size_t)n * Hx + sy) * Wx * a.xstride + a.xoff + 8 * part8;
+                px0[i] = *reinterpret_cast<const uint4*>(row + (size_t)sx0 * a.xstride);
+                px1[i] = *reinterpret_cast<const uint4*>(row + (size_t)sx1 * a.xstride);
             }
-            *reinterpret_cast<uint4*>(xt + pidx * XS + 8 * part8) = *reinterpret_cast<const uint4*>(
-                a.X + (((size_t)n * Hx + sy) * Wx + sx) * a.xstride + a.xoff + 8 * part8);
         }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < NDZ; ++i) {
+            const int v = threadIdx.x + i * THREADS;
+            if (v < DZ_V) {
+                const int px = v / (COUT / 8), part8 = v % (COUT / 8);
+                *reinterpret_cast<uint4*>(dz + px * ZS + 8 * part8) = pdz[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            const int v = threadIdx.x + i * THREADS;
+            if (v < X_V) {
+                const int pair = v % (ROWS * (COLS / 2)), part8 = v / (ROWS * (COLS / 2));
+                const int r = pair / (COLS / 2), c = 2 * (pair % (COLS / 2));
+                const uint32_t e0[4] = {px0[i].x, px0[i].y, px0[i].z, px0[i].w};
+                const uint32_t e1[4] = {px1[i].x, px1[i].y, px1[i].z, px1[i].w};
+                uint32_t* dst = reinterpret_cast<uint32_t*>(xt + ((size_t)(8 * part8) * ROWS + r) * CS + c);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    dst[(2 * j) * ROWS * CS / 2] = (e0[j] & 0xffffu) | (e1[j] << 16);
+                    dst[(2 * j + 1) * ROWS * CS / 2] = (e0[j] >> 16) | (e1[j] & 0xffff0000u);
+                }
+            }
+        }
+    };
+    if (s0 < s1) fetch(s0);
+    for (long sg = s0; sg < s1; ++sg) {
+        __syncthreads();  // the previous segment's LDS reads are done
+        store();
         __syncthreads();
+        if (sg + 1 < s1) fetch(sg + 1);
         if ((int)threadIdx.x < COUT)
             for (int px = 0; px < SEG; ++px) bacc += bf16f(dz[px * ZS + threadIdx.x]);
 #pragma unroll
@@ -555,14 +603,23 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
                         ((uint32_t)dz[(p0 + 2 * j + 1) * ZS + 16 * cob + l16] << 16);
             const bf16x8 A = __builtin_bit_cast(bf16x8, make_uint4(ab[0], ab[1], ab[2], ab[3]));
 #pragma unroll
-            for (int t = 0; t < 25; ++t) {
-                const int dy = t / 5, dx = t % 5;  // tile row and column offsets (tap + 2)
-                const uint16_t* xb0 = xt + (dy * COLS + p0 + dx) * XS + 16 * cib + l16;
-                uint32_t bb[4];
+            for (int dy = 0; dy < 5; ++dy) {
+                // Columns p0 .. p0+15 of channel 16 cib + l16, tile row dy (two
+                // aligned 16-byte reads) serve the 5 taps of this row: tap dx
+                // (tile column offset, tap + 2) needs columns p0+dx .. p0+dx+7.
+                const uint4* src =
+                    reinterpret_cast<const uint4*>(xt + ((size_t)(16 * cib + l16) * ROWS + dy) * CS + p0);
+                const uint4 w0 = src[0], w1 = src[1];
+                const uint32_t w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) bb[j] = (uint32_t)xb0[(2 * j) * XS] | ((uint32_t)xb0[(2 * j + 1) * XS] << 16);
-                const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(bb[0], bb[1], bb[2], bb[3]));
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+                for (int dx = 0; dx < 5; ++dx) {
+                    uint32_t bb[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        bb[k] = dx % 2 == 0 ? w[dx / 2 + k] : (w[(dx - 1) / 2 + k] >> 16) | (w[(dx + 1) / 2 + k] << 16);
+                    const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(bb[0], bb[1], bb[2], bb[3]));
+                    acc[dy * 5 + dx] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[dy * 5 + dx], 0, 0, 0);
+                }
             }
         }
     }
