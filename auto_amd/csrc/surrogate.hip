@@ -250,14 +250,32 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             const uint4 v = *reinterpret_cast<const uint4*>(X + ((size_t)yy * Wx + xx) * Cin + c0 + 8 * part);
             *reinterpret_cast<uint4*>(tile + pix * PS + 8 * part) = v;
         }
-        __syncthreads();
-        for (int t = 0; t < 25; ++t) {
-            const int dy = t / 5 - 2, dx = t % 5 - 2;
-            bf16x8 A[MT];
+        // With MT <= 2 the weight fragments load two taps ahead, so a whole
+        // tap's MFMAs lie between a load and its use (a load at the top of
+        // each tap, or one tap ahead, left every tap waiting on global
+        // latency: layer 3 forward 808 -> 769 us).  With MT = 4 the 16 extra
+        // registers cost a workgroup per CU, which lost more (696 -> 768 us),
+        // so those kernels load at the top of each tap.
+        constexpr bool AHEAD = MT <= 2;
+        auto weights = [&](int t, bf16x8 (&A)[MT]) {
 #pragma unroll
             for (int m = 0; m < MT; ++m)
                 A[m] = __builtin_bit_cast(
                     bf16x8, *reinterpret_cast<const uint4*>(a.Wt + ((size_t)(16 * m + l16) * 25 + t) * Cin + c0 + 8 * q));
+        };
+        bf16x8 A[MT], A1[MT], A2[MT];
+        if constexpr (AHEAD) {
+            weights(0, A);
+            weights(1, A1);
+        }
+        __syncthreads();
+        for (int t = 0; t < 25; ++t) {
+            const int dy = t / 5 - 2, dx = t % 5 - 2;
+            if constexpr (AHEAD) {
+                if (t + 2 < 25) weights(t + 2, A2);
+            } else {
+                weights(t, A);
+            }
             const int r = wave + 2 + dy, c = 2 + dx + l16;  // window pixel of block 0
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -265,6 +283,13 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
                 const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(tile + pix * PS + 8 * q));
 #pragma unroll
                 for (int m = 0; m < MT; ++m) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][b], 0, 0, 0);
+            }
+            if constexpr (AHEAD) {
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    A[m] = A1[m];
+                    A1[m] = A2[m];
+                }
             }
         }
     }
