@@ -54,7 +54,7 @@ struct Work {
 constexpr int WG_CHUNKS = 64;  // pixel chunks of the weight-gradient reductions
 // Partials: the VALU reduction's chunks, or the MFMA reduction's 256
 // workgroups of one 64 x 25 x 32 weight block each.
-constexpr size_t PART_FLOATS = (size_t)256 * 64 * 25 * 32;
+constexpr size_t PART_FLOATS = WGRAD_PART_FLOATS;
 
 // More pixel chunks for the small layers so the reduction still fills the chip.
 int chunks_for(int cout, int cin) {
@@ -136,7 +136,7 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
         w.dp = w.get<float>(Ph * C1);
         w.dz1 = w.get<float>(P * C1);
         w.part = w.get<float>(PART_FLOATS);
-        w.bpart = w.get<float>(2048 * 64);
+        w.bpart = w.get<float>(WGRAD_BPART_FLOATS);
     }
 }
 

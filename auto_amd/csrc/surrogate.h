@@ -82,8 +82,11 @@ void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, c
                  int H, int W);
 void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const float* g1,
                  float* out, int N, int H, int W);
-// Weight gradient `out` and bias gradient `bout` of one layer; `part` and
-// `bpart` (at least 2048 * 64 floats) hold per-workgroup partials.
+// Weight gradient `out` and bias gradient `bout` of one layer; `part`
+// (WGRAD_PART_FLOATS) and `bpart` (WGRAD_BPART_FLOATS) hold per-workgroup
+// partials.
+constexpr size_t WGRAD_PART_FLOATS = (size_t)256 * 64 * 25 * 32;
+constexpr size_t WGRAD_BPART_FLOATS = (size_t)2048 * 64;
 void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,
            float* bout);
 void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,

@@ -657,6 +657,172 @@ __global__ void __launch_bounds__(wg::THREADS) k_wgrad_mfma(WGradArgs a, float* 
         for (int r = 0; r < 4; ++r) dst[((size_t)(16 * cob + 4 * q + r) * 25 + t) * CIN + 16 * cib + l16] = acc[t][r];
 }
 
+// Layer 3's weight gradient through the parity split of its nearest-
+// upsampled input u (u[y][x] = a2[y >> 1][x >> 1]).  Output pixel (2i + py,
+// 2j + px) meets a2 at (i + A(py, dy), j + A(px, dx)) through tap (dy, dx)
+// (0..4, tap + 2), A(p, d) = (p + d - 2) >> 1 in {-1, 0, 1}, so
+//   dW3[co][dy][dx][ci] = sum over parities (py, px) of
+//                         G[py][px][A(py, dy)][A(px, dx)][co][ci],
+//   G[py][px][a][b] = sum over half-resolution (i, j) of
+//                     dz3[2i + py][2j + px][co] * a2[i + a][j + b][ci]:
+// 36 blocks over a quarter of the pixels each, 9 MFMA blocks per pixel
+// where the direct form takes 25; the inputs are the same bf16 values.
+// Grid (UPG, 4 parities); a workgroup walks a fixed range of 64-pixel
+// half-resolution row segments of its parity class, stages that row's dz3
+// pixels transposed ([co][k]) and a2 rows i-1..i+1, columns j0-1..j0+64
+// transposed ([ci][row][column]), prefetching the next segment into
+// registers during the MFMAs; wave w owns the (co, ci) block (w / 4, w % 4)
+// for all 9 (a, b).  Partials: part[workgroup][parity][a][b][co][ci],
+// bias partials bpart[workgroup][parity][co].
+namespace wu {
+constexpr int SEG = 64, COUT = 32, CIN = 64, THREADS = 512, UPG = 128;
+static_assert((size_t)UPG * 4 * 9 * COUT * CIN <= WGRAD_PART_FLOATS && UPG * 4 * COUT <= WGRAD_BPART_FLOATS,
+              "partials fit the workspace");
+constexpr int ROWS = 3, COLS = SEG + 2, CS = COLS + 6, DS = SEG + 8;  // a2 tile; dz3 row stride
+static_assert(8 * 3 + 32 + 16 <= CS, "B windows stay inside a tile row");
+}
+
+__global__ void __launch_bounds__(wu::THREADS) k_wgrad_up(WGradArgs a, float* __restrict__ part,
+                                                          float* __restrict__ bpart) {
+    using namespace wu;
+    __shared__ __attribute__((aligned(16))) uint16_t xt[CIN * ROWS * CS];
+    __shared__ __attribute__((aligned(16))) uint16_t dzt[COUT * DS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int cob = wave / (CIN / 16), cib = wave % (CIN / 16);
+    const int par = blockIdx.y, py = par >> 1, px = par & 1;
+    const int H = a.H, W = a.W, Hh = H / 2, Wh = W / 2;
+    const int segs_per_row = Wh / SEG;
+    const long nseg = (long)a.N * Hh * segs_per_row;
+    const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
+    f32x4 acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;
+    // dz3: SEG pixels (2i + py, 2(j0 + k) + px) x 32 channels, 8 channels
+    // per load; a2: ROWS x COLS pixels x 64 channels, 8 channels of two
+    // adjacent pixels per task, written as one 32-bit word per channel.
+    constexpr int DZ_V = SEG * COUT / 8, X_V = ROWS * (COLS / 2) * (CIN / 8);
+    constexpr int NDZ = (DZ_V + THREADS - 1) / THREADS, NX = (X_V + THREADS - 1) / THREADS;
+    uint4 pdz[NDZ], px0[NX], px1[NX];
+    auto fetch = [&](long sg) {
+        const int jb = (int)(sg % segs_per_row);
+        const long ri = sg / segs_per_row;
+        const int i = (int)(ri % Hh), n = (int)(ri / Hh), j0 = jb * SEG;
+        const uint16_t* zrow = a.dz_h + (((size_t)n * H + 2 * i + py) * W) * COUT;
+#pragma unroll
+        for (int u = 0; u < NDZ; ++u) {
+            const int v = threadIdx.x + u * THREADS;
+            if (v < DZ_V) {
+                const int k = v / (COUT / 8), part8 = v % (COUT / 8);
+                pdz[u] = *reinterpret_cast<const uint4*>(zrow + (size_t)(2 * (j0 + k) + px) * COUT + 8 * part8);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int v = threadIdx.x + u * THREADS;
+            if (v < X_V) {
+                const int pair = v % (ROWS * (COLS / 2)), part8 = v / (ROWS * (COLS / 2));
+                const int r = pair / (COLS / 2), c = 2 * (pair % (COLS / 2));
+                const int sy = wrap(i + r - 1, Hh), sx0 = wrap(j0 + c - 1, Wh), sx1 = wrap(j0 + c, Wh);
+                const uint16_t* row = a.X + ((size_t)n * Hh + sy) * Wh * a.xstride + a.xoff + 8 * part8;
+                px0[u] = *reinterpret_cast<const uint4*>(row + (size_t)sx0 * a.xstride);
+                px1[u] = *reinterpret_cast<const uint4*>(row + (size_t)sx1 * a.xstride);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < NDZ; ++u) {
+            const int v = threadIdx.x + u * THREADS;
+            if (v < DZ_V) {
+                const int k = v / (COUT / 8), part8 = v % (COUT / 8);
+                const uint32_t e[4] = {pdz[u].x, pdz[u].y, pdz[u].z, pdz[u].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    dzt[(8 * part8 + 2 * j) * DS + k] = (uint16_t)(e[j] & 0xffffu);
+                    dzt[(8 * part8 + 2 * j + 1) * DS + k] = (uint16_t)(e[j] >> 16);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int v = threadIdx.x + u * THREADS;
+            if (v < X_V) {
+                const int pair = v % (ROWS * (COLS / 2)), part8 = v / (ROWS * (COLS / 2));
+                const int r = pair / (COLS / 2), c = 2 * (pair % (COLS / 2));
+                const uint32_t e0[4] = {px0[u].x, px0[u].y, px0[u].z, px0[u].w};
+                const uint32_t e1[4] = {px1[u].x, px1[u].y, px1[u].z, px1[u].w};
+                uint32_t* dst = reinterpret_cast<uint32_t*>(xt + ((size_t)(8 * part8) * ROWS + r) * CS + c);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    dst[(2 * j) * ROWS * CS / 2] = (e0[j] & 0xffffu) | (e1[j] << 16);
+                    dst[(2 * j + 1) * ROWS * CS / 2] = (e0[j] >> 16) | (e1[j] & 0xffff0000u);
+                }
+            }
+        }
+    };
+    if (s0 < s1) fetch(s0);
+    for (long sg = s0; sg < s1; ++sg) {
+        __syncthreads();  // the previous segment's LDS reads are done
+        store();
+        __syncthreads();
+        if (sg + 1 < s1) fetch(sg + 1);
+        if ((int)threadIdx.x < COUT)
+            for (int k = 0; k < SEG; ++k) bacc += bf16f(dzt[threadIdx.x * DS + k]);
+#pragma unroll
+        for (int kc = 0; kc < SEG / 32; ++kc) {
+            const int p0 = 32 * kc + 8 * q;  // this lane's 8 half-resolution pixels
+            const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(dzt + (16 * cob + l16) * DS + p0));
+#pragma unroll
+            for (int ar = 0; ar < 3; ++ar) {
+                // Tile row ar (a = ar - 1), columns p0 .. p0 + 15; offset b
+                // (= bc - 1) needs columns p0 + bc .. p0 + bc + 7.
+                const uint4* src =
+                    reinterpret_cast<const uint4*>(xt + ((size_t)(16 * cib + l16) * ROWS + ar) * CS + p0);
+                const uint4 w0 = src[0], w1 = src[1];
+                const uint32_t w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+                for (int bc = 0; bc < 3; ++bc) {
+                    uint32_t bb[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        bb[k] = bc % 2 == 0 ? w[bc / 2 + k] : (w[(bc - 1) / 2 + k] >> 16) | (w[(bc + 1) / 2 + k] << 16);
+                    const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(bb[0], bb[1], bb[2], bb[3]));
+                    acc[ar * 3 + bc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[ar * 3 + bc], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const size_t grp = (size_t)blockIdx.x * 4 + par;
+    if ((int)threadIdx.x < COUT) bpart[grp * COUT + threadIdx.x] = bacc;
+    // Lane holds G[a][b][co = 16 cob + 4q + r][ci = 16 cib + l16].
+    float* dst = part + grp * 9 * COUT * CIN;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[((size_t)k * COUT + 16 * cob + 4 * q + r) * CIN + 16 * cib + l16] = acc[k][r];
+}
+
+// dW3[co][t][ci] from the G partials: thread = output (co, t, ci); sums over
+// workgroups, then parities, in that fixed order.
+__global__ void k_reduce_up(const float* __restrict__ part, int groups, float* __restrict__ out) {
+    using namespace wu;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= COUT * 25 * CIN) return;
+    const int ci = o % CIN, t = (o / CIN) % 25, co = o / (CIN * 25);
+    const int dy = t / 5, dx = t % 5;
+    float acc = 0.0f;
+    for (int g = 0; g < groups; ++g)
+#pragma unroll
+        for (int par = 0; par < 4; ++par) {
+            const int py = par >> 1, px = par & 1;
+            const int ab = (((py + dy - 2) >> 1) + 1) * 3 + (((px + dx - 2) >> 1) + 1);
+            acc += part[((((size_t)g * 4 + par) * 9 + ab) * COUT + co) * CIN + ci];
+        }
+    out[o] = acc;
+}
+
 // The thin layers' weight gradients on MFMA.  With U the fp32 output
 // gradient and V the bf16 layer input, out[a][t][b] = sum over pixels p of
 // U[p][a] V[p + tap_t][b] is a product over K = pixels:
@@ -1029,8 +1195,9 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
            float* bout) {
     const int Cq = (a.Cout + 3) / 4;
     const int nout = Cq * 25 * a.Cin, nw = a.Cout * 25 * a.Cin;
-    const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 &&
-                      ((a.Cout == 32 && a.Cin == 64 && a.up) || (a.Cout == 64 && a.Cin == 32 && !a.up));
+    const bool up = a.dz_h && a.xoff == 0 && a.up && a.Cout == wu::COUT && a.Cin == wu::CIN && a.H % 2 == 0 &&
+                    (a.W / 2) % wu::SEG == 0;
+    const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 && a.Cout == 64 && a.Cin == 32 && !a.up;
     const bool thin = a.dz_f && a.W % wg::SEG == 0 && !a.up &&
                       ((a.Cout == 32 && a.Cin == 1) || (a.Cout == 1 && a.Cin == 32));
     if (thin) {
@@ -1041,29 +1208,36 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         else
             hipLaunchKernelGGL((k_wgrad_thin_mfma<4>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
+    } else if (up) {
+        chunks = 4 * wu::UPG;  // bias partials: workgroup x parity
+        {
+            Launch L(p, s, "surr_wgrad_up");
+            hipLaunchKernelGGL(k_wgrad_up, dim3(wu::UPG, 4), dim3(wu::THREADS), 0, s, a, part, bpart);
+            AH_HIP(hipGetLastError());
+        }
+        Launch L(p, s, "surr_reduce");
+        hipLaunchKernelGGL(k_reduce_up, dim3(blocks(nw)), dim3(256), 0, s, part, wu::UPG, out);
+        AH_HIP(hipGetLastError());
     } else if (mfma) {
         chunks = wg::GROUPS;
         Launch L(p, s, "surr_wgrad_mfma");
-        if (a.Cout == 32)
-            hipLaunchKernelGGL((k_wgrad_mfma<32, 64, 1>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
-        else
-            hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
+        hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else {
         Launch L(p, s, "surr_wgrad");
         hipLaunchKernelGGL(k_wgrad, dim3(blocks(nout), chunks), dim3(256), 0, s, a, part);
         AH_HIP(hipGetLastError());
     }
-    {
+    if (!up) {
         Launch L(p, s, "surr_reduce");
         if (thin) hipLaunchKernelGGL(k_reduce_few, dim3(nw), dim3(256), 0, s, part, chunks, nw, out);
         else hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
         AH_HIP(hipGetLastError());
     }
-    // The bias gradient (the sum of dZ over the pixels): the thin and MFMA
-    // kernels add it up from the dZ they stage (per-workgroup partials in
-    // bpart); the generic kernel leaves it to a pass of its own.
-    if (thin || mfma) {
+    // The bias gradient (the sum of dZ over the pixels): the thin, MFMA and
+    // parity-split kernels add it up from the dZ they stage (per-workgroup
+    // partials in bpart); the generic kernel leaves it to a pass of its own.
+    if (thin || mfma || up) {
         Launch L(p, s, "surr_reduce_bias");
         hipLaunchKernelGGL(k_reduce_few, dim3(a.Cout), dim3(256), 0, s, bpart, chunks, a.Cout, bout);
         AH_HIP(hipGetLastError());
