@@ -51,20 +51,6 @@ struct Work {
     }
 };
 
-constexpr int WG_CHUNKS = 64;  // pixel chunks of the weight-gradient reductions
-// Partials: the VALU reduction's chunks, or the MFMA reduction's 256
-// workgroups of one 64 x 25 x 32 weight block each.
-constexpr size_t PART_FLOATS = WGRAD_PART_FLOATS;
-
-// More pixel chunks for the small layers so the reduction still fills the chip.
-int chunks_for(int cout, int cin) {
-    const int nout = ((cout + 3) / 4) * 25 * cin;
-    const int blocks = (nout + 255) / 256;
-    int c = WG_CHUNKS;
-    while (c < 4096 && blocks * c < 4096 && (size_t)(2 * c) * cout * 25 * cin <= PART_FLOATS) c *= 2;
-    return c;
-}
-
 }  // namespace
 
 struct ah_surrogate {
@@ -135,7 +121,7 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
         w.dz2 = w.get<uint16_t>(Ph * C2);
         w.dp = w.get<float>(Ph * C1);
         w.dz1 = w.get<float>(P * C1);
-        w.part = w.get<float>(PART_FLOATS);
+        w.part = w.get<float>(WGRAD_PART_FLOATS);
         w.bpart = w.get<float>(WGRAD_BPART_FLOATS);
     }
 }
@@ -214,7 +200,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W;
         a.Cin = C1;
         a.Cout = 1;
-        wgrad(st, pf, a, w.part, chunks_for(1, C1), g + OFF_W4, w.bpart, g + OFF_B4);
+        wgrad(st, pf, a, w.part, g + OFF_W4, w.bpart, g + OFF_B4);
         // layer 3 (ds, the layer-4 input gradient, is formed from dr where it is used)
         dz3_from_dr(st, pf, w.dr, S->pw.W4, w.g3, w.dz3, N, H, W);
         a = WGradArgs{};
@@ -227,7 +213,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.Cin = C2;
         a.Cout = C1;
         a.up = 1;
-        wgrad(st, pf, a, w.part, chunks_for(C1, C2), g + OFF_W3, w.bpart, g + OFF_B3);
+        wgrad(st, pf, a, w.part, g + OFF_W3, w.bpart, g + OFF_B3);
         ConvArgs c{};
         c.X = w.dz3;
         c.N = N;
@@ -253,7 +239,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W / 2;
         a.Cin = C1;
         a.Cout = C2;
-        wgrad(st, pf, a, w.part, chunks_for(C2, C1), g + OFF_W2, w.bpart, g + OFF_B2);
+        wgrad(st, pf, a, w.part, g + OFF_W2, w.bpart, g + OFF_B2);
         c = ConvArgs{};
         c.X = w.dz2;
         c.N = N;
@@ -278,7 +264,7 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         a.W = W;
         a.Cin = 1;
         a.Cout = C1;
-        wgrad(st, pf, a, w.part, chunks_for(C1, 1), g + OFF_W1, w.bpart, g + OFF_B1);
+        wgrad(st, pf, a, w.part, g + OFF_W1, w.bpart, g + OFF_B1);
     }
     std::vector<double> part(lb);
     AH_HIP(hipMemcpyAsync(part.data(), w.lpart, sizeof(double) * lb, hipMemcpyDeviceToHost, st));

@@ -84,13 +84,12 @@ void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, c
                  float* out, int N, int H, int W);
 // Weight gradient `out` and bias gradient `bout` of one layer; `part`
 // (WGRAD_PART_FLOATS) and `bpart` (WGRAD_BPART_FLOATS) hold per-workgroup
-// partials.
+// partials.  Layers 1 and 4 (one channel on one side), layer 2 (32 -> 64)
+// and layer 3 (64 -> 32 over the upsampled input) each have their kernel;
+// other shapes are an error.
 constexpr size_t WGRAD_PART_FLOATS = (size_t)256 * 64 * 25 * 32;
 constexpr size_t WGRAD_BPART_FLOATS = (size_t)2048 * 64;
-void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,
-           float* bout);
-void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
-           int chunks, float* out);
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, float* out, float* bpart, float* bout);
 void adam(hipStream_t s, Profiler* p, float* theta, float* m, float* v, const float* g, int n, float lr, float b1,
           float b2, float eps, float c1, float c2);
 void pack_weights(hipStream_t s, Profiler* p, const float* theta, const PackedWeights& w);

@@ -446,80 +446,17 @@ __global__ void __launch_bounds__(256) k_dz1_dr(const float* __restrict__ dr, co
 }
 
 // ---------------------------------------------------------------- weight grads
-// dW[co][t][ci] = sum over pixels of dZ[pix][co] * X[pix + tap_t][ci].
-// Thread = 4 output channels x one (t, ci); blockIdx.y = pixel chunk; the
-// chunk partials go to part[chunk][co][t][ci] for a fixed-order reduction.
-// dZ is fp32 (dz_f) or bf16 (dz_h); X is bf16 NHWC with channel stride
-// xstride (the first layer reads xb = bf16(x), one channel), at
-// half resolution when `up` (the layer-3 input u = upsample2(a2)).
-__global__ void __launch_bounds__(256) k_wgrad(WGradArgs a, float* __restrict__ part) {
-    const int Cq = (a.Cout + 3) / 4;
-    const int nout = Cq * 25 * a.Cin;
-    const int o = blockIdx.x * 256 + threadIdx.x;
-    if (o >= nout) return;
-    const int ci = o % a.Cin, t = (o / a.Cin) % 25, cq = o / (a.Cin * 25);
-    const int dy = t / 5 - 2, dx = t % 5 - 2;
-    const size_t npix = (size_t)a.N * a.H * a.W;
-    const size_t per = (npix + gridDim.y - 1) / gridDim.y;
-    const size_t p0 = per * blockIdx.y, p1 = p0 + per < npix ? p0 + per : npix;
-    const int Hx = a.up ? a.H / 2 : a.H, Wx = a.up ? a.W / 2 : a.W;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    // Pixel coordinates advance with the pixel index (one division per chunk).
-    int xx = (int)(p0 % a.W), yy = (int)((p0 / a.W) % a.H), n = (int)(p0 / ((size_t)a.W * a.H));
-    for (size_t pix = p0; pix < p1; ++pix) {
-        int sy = wrap(yy + dy, a.H), sx = wrap(xx + dx, a.W);
-        if (a.up) {
-            sy >>= 1;
-            sx >>= 1;
-        }
-        const float xv = bf16f(a.X[(((size_t)n * Hx + sy) * Wx + sx) * a.xstride + a.xoff + ci]);
-        if (a.Cout == 1) {
-            acc[0] += a.dz_f[pix] * xv;
-        } else if (a.dz_f) {
-            const float4 g = *reinterpret_cast<const float4*>(a.dz_f + pix * a.Cout + 4 * cq);
-            acc[0] += g.x * xv;
-            acc[1] += g.y * xv;
-            acc[2] += g.z * xv;
-            acc[3] += g.w * xv;
-        } else {
-            const uint2 g = *reinterpret_cast<const uint2*>(a.dz_h + pix * a.Cout + 4 * cq);
-            acc[0] += __uint_as_float(g.x << 16) * xv;
-            acc[1] += __uint_as_float(g.x & 0xffff0000u) * xv;
-            acc[2] += __uint_as_float(g.y << 16) * xv;
-            acc[3] += __uint_as_float(g.y & 0xffff0000u) * xv;
-        }
-        if (++xx == a.W) {
-            xx = 0;
-            if (++yy == a.H) {
-                yy = 0;
-                ++n;
-            }
-        }
-    }
-    float* dst = part + (size_t)blockIdx.y * a.Cout * 25 * a.Cin;
-    const int nco = a.Cout == 1 ? 1 : 4;
-    for (int j = 0; j < nco; ++j) dst[((size_t)(4 * cq + j) * 25 + t) * a.Cin + ci] = acc[j];
-}
-
-// Weight gradients of the two 32 <-> 64 channel layers on MFMA: a
-// transposed implicit GEMM, dW[co][t][ci] = sum over pixels p of
-// dZ[p][co] X[p + tap_t][ci], with the pixels as the MFMA's K dimension.
-// v_mfma_f32_16x16x32_bf16: A = 16 output channels x 32 pixels of dZ, B = 32
-// pixels x 16 input channels of X shifted by the tap, D = 16 x 16 of dW.
-// A workgroup (8 waves) walks a fixed range of 64-pixel row segments; per
-// segment it stages dZ (64 pixels x COUT) and the X tile the 25 taps reach
-// (5 rows x 68 columns x CIN, read at half resolution when UP) in LDS, and
-// wave w accumulates the (16 co x 16 ci) block w of all 25 taps.  A lane's
-// operands are 8 pixels of one channel, read from the pixel-major LDS copies
-// 2 bytes at a time (the row strides are padded by 8 channels so the 4 pixel
-// groups of a read land on distinct banks).  Partials go to
-// part[workgroup][co][t][ci] and are summed in workgroup order afterwards:
-// the result does not depend on timing.
+// dW[co][t][ci] = sum over pixels of dZ[pix][co] * X[pix + tap_t][ci], the
+// bias gradient the sum of dZ; per-workgroup partials in fixed ranges of
+// pixels, summed in workgroup order (results do not depend on timing).
 namespace wg {
 constexpr int SEG = 64, ROWS = 5, COLS = SEG + 4, THREADS = 512, GROUPS = 256;
 // Workgroups of the thin-layer kernels: 800 outputs each, so many more
 // workgroups than the MFMA kernels' (256 left 4 waves per CU).
 constexpr int THIN_GROUPS = 2048;
+static_assert((size_t)GROUPS * 64 * 25 * 32 <= WGRAD_PART_FLOATS && (size_t)THIN_GROUPS * 800 <= WGRAD_PART_FLOATS &&
+                  GROUPS * 64 <= WGRAD_BPART_FLOATS && THIN_GROUPS * 32 <= WGRAD_BPART_FLOATS,
+              "partials fit the workspace");
 }
 
 template <int COUT, int CIN, int UP>
@@ -970,28 +907,6 @@ __global__ void __launch_bounds__(256) k_wgrad_thin_mfma(WGradArgs a, float* __r
     }
 }
 
-// Bias gradient partials: part[chunk][c] = sum over the chunk's pixels of
-// dZ[pix][c].  C divides 256: thread j sums channel j % C over the chunk's
-// pixels j / C, j / C + 256 / C, ...; the 256 / C lanes of a channel are then
-// added in lane order through LDS (a fixed order: bitwise reproducible).
-__global__ void __launch_bounds__(256) k_bgrad(const float* __restrict__ dz_f, const uint16_t* __restrict__ dz_h,
-                                               int C, size_t npix, float* __restrict__ part) {
-    __shared__ float sh[256];
-    const size_t per = (npix + gridDim.x - 1) / gridDim.x;
-    const size_t p0 = per * blockIdx.x, p1 = p0 + per < npix ? p0 + per : npix;
-    const int c = threadIdx.x % C, stride = 256 / C;
-    float acc = 0.0f;
-    for (size_t pix = p0 + threadIdx.x / C; pix < p1; pix += stride)
-        acc += dz_f ? dz_f[pix * C + c] : bf16f(dz_h[pix * C + c]);
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    if ((int)threadIdx.x < C) {
-        float t = 0.0f;
-        for (int j = 0; j < stride; ++j) t += sh[j * C + threadIdx.x];
-        part[(size_t)blockIdx.x * C + threadIdx.x] = t;
-    }
-}
-
 // Few outputs over many chunks (biases, the thin layers' weights):
 // out[c] = sum over chunks of part[chunk][c], one workgroup per output:
 // thread t adds chunks t, t + 256, ... in order, then a fixed pairwise tree
@@ -1191,10 +1106,9 @@ void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, c
     AH_HIP(hipGetLastError());
 }
 
-void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chunks, float* out, float* bpart,
-           float* bout) {
-    const int Cq = (a.Cout + 3) / 4;
-    const int nout = Cq * 25 * a.Cin, nw = a.Cout * 25 * a.Cin;
+void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, float* out, float* bpart, float* bout) {
+    const int nw = a.Cout * 25 * a.Cin;
+    int chunks = 0;  // partial blocks of the chosen kernel
     const bool up = a.dz_h && a.xoff == 0 && a.up && a.Cout == wu::COUT && a.Cin == wu::CIN && a.H % 2 == 0 &&
                     (a.W / 2) % wu::SEG == 0;
     const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 && a.Cout == 64 && a.Cin == 32 && !a.up;
@@ -1224,9 +1138,7 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         hipLaunchKernelGGL((k_wgrad_mfma<64, 32, 0>), dim3(chunks), dim3(wg::THREADS), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else {
-        Launch L(p, s, "surr_wgrad");
-        hipLaunchKernelGGL(k_wgrad, dim3(blocks(nout), chunks), dim3(256), 0, s, a, part);
-        AH_HIP(hipGetLastError());
+        throw Error{AH_ERR_INVALID, "surrogate weight gradient: unsupported layer shape"};
     }
     if (!up) {
         Launch L(p, s, "surr_reduce");
@@ -1234,30 +1146,11 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, int chun
         else hipLaunchKernelGGL(k_reduce_parts, dim3(blocks(nw)), dim3(256), 0, s, part, chunks, nw, out);
         AH_HIP(hipGetLastError());
     }
-    // The bias gradient (the sum of dZ over the pixels): the thin, MFMA and
-    // parity-split kernels add it up from the dZ they stage (per-workgroup
-    // partials in bpart); the generic kernel leaves it to a pass of its own.
-    if (thin || mfma || up) {
-        Launch L(p, s, "surr_reduce_bias");
-        hipLaunchKernelGGL(k_reduce_few, dim3(a.Cout), dim3(256), 0, s, bpart, chunks, a.Cout, bout);
-        AH_HIP(hipGetLastError());
-    } else {
-        bgrad(s, p, a.dz_f, a.dz_h, a.Cout, (size_t)a.N * a.H * a.W, part, 1024, bout);
-    }
-}
-
-void bgrad(hipStream_t s, Profiler* p, const float* dz_f, const uint16_t* dz_h, int C, size_t npix, float* part,
-           int chunks, float* out) {
-    {
-        Launch L(p, s, "surr_bgrad");
-        if (C < 1 || 256 % C) throw Error{AH_ERR_INVALID, "bias gradient: channels must divide 256"};
-        hipLaunchKernelGGL(k_bgrad, dim3(chunks), dim3(256), 0, s, dz_f, dz_h, C, npix, part);
-        AH_HIP(hipGetLastError());
-    }
-    // One workgroup per channel: a lone thread per channel walking all the
-    // chunks in turn took 183 us per call on its chain of dependent loads.
+    // The bias gradient (the sum of dZ over the pixels): each kernel adds it
+    // up from the dZ it stages (per-workgroup partials in bpart), one
+    // reducing workgroup per channel.
     Launch L(p, s, "surr_reduce_bias");
-    hipLaunchKernelGGL(k_reduce_few, dim3(C), dim3(256), 0, s, part, chunks, C, out);
+    hipLaunchKernelGGL(k_reduce_few, dim3(a.Cout), dim3(256), 0, s, bpart, chunks, a.Cout, bout);
     AH_HIP(hipGetLastError());
 }
 
