@@ -18,7 +18,14 @@ dominant kernel's algorithmic bytes against the 8 TB/s HBM peak.  The CPU
 baseline times the oracle's FP32 restatement of the reference step on host
 cores (one thread and all threads, 256^2 and 4096^2, calibrate-then-median),
 and the error legs check GPU steps against the oracle's FP64 step from the
-same input along the runs.
+same input along the runs.  The 384-world batch is timed beside the same
+worlds stepped one by one as standalone sims.
+
+Opt-in comparisons with the library route on the same GPU (each spends
+minutes in MIOpen's algorithm search on a fresh machine): --torch-conv (the
+direct path's 53x53 convolution as torch conv2d) and --torch-surrogate (the
+surrogate's network trained in PyTorch).  --surrogate-only
+[--surrogate-radius R] runs the surrogate section alone.
 """
 from __future__ import annotations
 
