@@ -250,48 +250,51 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
             const uint4 v = *reinterpret_cast<const uint4*>(X + ((size_t)yy * Wx + xx) * Cin + c0 + 8 * part);
             *reinterpret_cast<uint4*>(tile + pix * PS + 8 * part) = v;
         }
-        // With MT <= 2 the weight fragments load two taps ahead, so a whole
-        // tap's MFMAs lie between a load and its use (a load at the top of
-        // each tap, or one tap ahead, left every tap waiting on global
-        // latency: layer 3 forward 808 -> 769 us).  With MT = 4 the 16 extra
-        // registers cost a workgroup per CU, which lost more (696 -> 768 us),
-        // so those kernels load at the top of each tap.
-        constexpr bool AHEAD = MT <= 2;
+        // Weight fragments rotate through NS register sets: tap t uses set
+        // t % NS while the loads for tap t + NS - 1 are in flight, so NS - 1
+        // taps of MFMAs hide each load.  Taps 0..23 run in groups of NS (NS
+        // divides 24) with the set index known at compile time and no branch
+        // in the group (a rolled loop that copied set to set waited for the
+        // newest load before every copy); tap 24 follows the loop.
+        constexpr int NS = MT <= 2 ? 3 : 2;
+        static_assert(24 % NS == 0, "tap groups");
         auto weights = [&](int t, bf16x8 (&A)[MT]) {
 #pragma unroll
             for (int m = 0; m < MT; ++m)
                 A[m] = __builtin_bit_cast(
                     bf16x8, *reinterpret_cast<const uint4*>(a.Wt + ((size_t)(16 * m + l16) * 25 + t) * Cin + c0 + 8 * q));
         };
-        bf16x8 A[MT], A1[MT], A2[MT];
-        if constexpr (AHEAD) {
-            weights(0, A);
-            weights(1, A1);
-        }
-        __syncthreads();
-        for (int t = 0; t < 25; ++t) {
-            const int dy = t / 5 - 2, dx = t % 5 - 2;
-            if constexpr (AHEAD) {
-                if (t + 2 < 25) weights(t + 2, A2);
-            } else {
-                weights(t, A);
-            }
+        // Block b of the window lies 16 b pixels (8 b half-resolution
+        // pixels) right of block 0.
+        constexpr int BSTEP = (UP ? 8 : 16) * PS;
+        auto tap = [&](int t, const bf16x8 (&A)[MT]) {
+            const int dy = t / 5 - 2, dx = t - 5 * (t / 5) - 2;
             const int r = wave + 2 + dy, c = 2 + dx + l16;  // window pixel of block 0
+            const uint16_t* b0 = tile + (UP ? (r >> 1) * SC + (c >> 1) : r * SC + c) * PS + 8 * q;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const int pix = UP ? (r >> 1) * SC + ((c + 16 * b) >> 1) : r * SC + c + 16 * b;
-                const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(tile + pix * PS + 8 * q));
+                const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b0 + b * BSTEP));
 #pragma unroll
                 for (int m = 0; m < MT; ++m) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[m][b], 0, 0, 0);
             }
-            if constexpr (AHEAD) {
+        };
+        bf16x8 Aw[NS][MT];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) {
-                    A[m] = A1[m];
-                    A1[m] = A2[m];
-                }
+        for (int j = 0; j < NS - 1; ++j) weights(j, Aw[j]);
+        __syncthreads();
+#pragma unroll 1
+        for (int t0 = 0; t0 < 24; t0 += NS) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                // With NS = 3 the last group's final load repeats tap 24 (unused).
+                weights(min(t0 + j + NS - 1, 24), Aw[(j + NS - 1) % NS]);
+                // Keep the loads here: the scheduler would otherwise sink them
+                // next to their first use, a tap or less ahead.
+                __builtin_amdgcn_sched_barrier(0);
+                tap(t0 + j, Aw[j]);
             }
         }
+        tap(24, Aw[0]);
     }
     constexpr bool can_pool = !UP && MT * NB * 64 * 4 * 2 * sizeof(float) <= sizeof(tile);
     if (can_pool && a.mode == CONV_POOL_DGELU) {
