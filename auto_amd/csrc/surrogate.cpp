@@ -175,7 +175,7 @@ void forward(ah_surrogate* S, const float* x, bool train) {
     c.a = w.s;
     c.add = w.a1;
     conv(st, pf, c, "surr_conv3");
-    out_layer(st, pf, w.s, S->pw.W4x, S->theta + OFF_B4, x, w.r, w.y, N, H, W);
+    out_layer(st, pf, w.s, S->pw.W4b, S->theta + OFF_B4, x, w.r, w.y, N, H, W);
 }
 
 // Loss of work.y against `target` and, when `grad` is set, the full
@@ -333,7 +333,7 @@ ah_status ah_surrogate_new(uint32_t seed, ah_surrogate** out) {
         S->pw.W2T = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
         S->pw.W3 = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
         S->pw.W3T = (uint16_t*)S->alloc(2 * 64 * 25 * 32);
-        S->pw.W4x = (uint16_t*)S->alloc(2 * 16 * 25 * 32);
+        S->pw.W4b = (uint16_t*)S->alloc(2 * 25 * 32);
         S->pw.W4 = (float*)S->alloc(4 * 25 * 32);
         std::vector<float> th;
         init_params(seed, th);

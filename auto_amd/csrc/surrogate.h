@@ -23,9 +23,7 @@ constexpr int N_PARAMS = OFF_B4 + 1;  // 104129
 // 2 x 2 pixel blocks (the adjoint of the nearest upsampling) and multiplied
 // by dgelu_in (gelu' of layer 2's pre-activation), goes to dpool at half
 // resolution; nothing else is written.
-// CONV_OUT (layer 4, Cout 16 of which only channel 0 has weights): r =
-// channel 0 + bias[0] and y = clip(xin + r, 0, 1), one float each per pixel.
-enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2, CONV_POOL_DGELU = 3, CONV_OUT = 4 };
+enum ConvMode { CONV_LINEAR = 0, CONV_GELU = 1, CONV_GELU_ADD = 2, CONV_POOL_DGELU = 3 };
 
 struct ConvArgs {
     const uint16_t* X;   // bf16 NHWC input, Cin channels (half resolution when up)
@@ -41,9 +39,6 @@ struct ConvArgs {
     const uint16_t* add; // bf16 [N][H][W][Cout] for CONV_GELU_ADD
     const float* dgelu_in;  // fp32 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
     uint16_t* dpool;     // bf16 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
-    const float* xin;    // fp32 [N][H][W] for CONV_OUT: the network's input
-    float* rout;         // fp32 [N][H][W] for CONV_OUT: the residual r
-    float* yout;         // fp32 [N][H][W] for CONV_OUT: clip(xin + r, 0, 1)
 };
 
 struct WGradArgs {
@@ -60,8 +55,8 @@ struct PackedWeights {
     uint16_t* W2T;  // [32][25][64]
     uint16_t* W3;   // [32][25][64]
     uint16_t* W3T;  // [64][25][32]
-    uint16_t* W4x;  // [16][25][32]: W4 in row 0, rows 1..15 zero (the MFMA output layer)
-    float* W4;      // [25][32]
+    uint16_t* W4b;  // [25][32]
+    float* W4;      // [25][32], the same bf16 values as fp32
 };
 
 // Layer 1 (5x5, 1 -> 32, bias, GELU) straight from the fp32 input x; also
@@ -69,7 +64,9 @@ struct PackedWeights {
 // 2 x 2 average pool of the activations (bf16, half resolution).
 void conv1(hipStream_t s, Profiler* p, const ConvArgs& a, const float* x, uint16_t* xb, uint16_t* pool);
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name);
-void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4x, const float* b4, const float* x,
+// Layer 4 (5x5, 32 -> 1) from the bf16 NHWC activations sa: r = conv + b4
+// and y = clip(x + r, 0, 1), fp32 [N][H][W] each.
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4b, const float* b4, const float* x,
                float* r, float* y, int N, int H, int W);
 int loss_blocks(size_t total);
 void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const float* x, const float* r, float* dr,

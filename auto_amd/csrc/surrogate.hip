@@ -21,8 +21,8 @@
 // half-resolution tensor at (y/2, x/2).  The backward input gradients of
 // layers 2 and 3 are the same kernel over tap-flipped, channel-swapped
 // weights.  Layer 1 (one input channel) forms its 25-tap im2col fragments
-// in registers from an LDS tile of x; layer 4 (one output channel) runs as
-// 16 output channels of which 15 have zero weights.  The weight gradients
+// in registers from an LDS tile of x; layer 4 (one output channel) puts the
+// 25 taps in the MFMA's N dimension and adds the shifted products.  The weight gradients
 // (reductions over pixels) use fixed-order partial sums.
 #include <algorithm>
 #include <cmath>
@@ -61,19 +61,6 @@ template <int MT, int NB>
 __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&acc)[MT][NB], int n, int y, int x0,
                                               int q, int l16) {
     const int H = a.H, W = a.W, Cout = a.Cout;
-    if (a.mode == CONV_OUT) {
-        // Channel 0 sits in element 0 of the lanes with q == 0.
-        if (q == 0) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const size_t o = ((size_t)n * H + y) * W + x0 + 16 * b + l16;
-                const float r = acc[0][b][0] + a.bias[0];
-                a.rout[o] = r;
-                a.yout[o] = fminf(fmaxf(a.xin[o] + r, 0.0f), 1.0f);
-            }
-        }
-        return;
-    }
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -337,6 +324,91 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
         return;
     }
     conv_epilogue<MT, NB>(a, acc, n, y, x0, q, l16);
+}
+
+// Layer 4 (5x5, 32 -> 1) and the output: r = conv(a3, W4) + b4 and
+// y = clip(x + r, 0, 1).  With one output channel, a product over (pixel,
+// channel) x (channel, output) would leave 15 of an MFMA's 16 columns zero;
+// here the taps are the N dimension instead: Y[p][t] = sum over ci of
+// a3[p][ci] W4[t][ci] for every window pixel p (16 pixels x 16 taps per
+// MFMA, the 32 channels in K), and r(y, x) = sum over taps (dy, dx) of
+// Y[(y + dy, x + dx)][(dy, dx)], added from LDS in tap order.  Workgroup:
+// 4 output rows x 64 columns; window rows y0 - 2 .. y0 + 5 and columns
+// x0 - 2 .. x0 + 77 (5 blocks of 16, of which 68 columns are read).  LDS
+// keeps only the (window row, dy) pairs some output row of the tile uses:
+// window row r serves dy + 2 = e in [max(0, r - 3), min(4, r)], 5 taps per
+// e, 100 slots in all.
+namespace ot {
+constexpr int ROWS = 4, COLS = 64, WR = ROWS + 4, NBLK = 5, UNITS = WR * NBLK, PER = UNITS / 4;
+constexpr int YS = 84;  // floats per slot: 80 columns, padded so 8 slots of one store meet distinct banks
+__host__ __device__ constexpr int e_lo(int r) { return r > 3 ? r - 3 : 0; }
+__host__ __device__ constexpr int e_hi(int r) { return r < 4 ? r : 4; }
+constexpr int slot0_sum(int r) {
+    int s = 0;
+    for (int i = 0; i < r; ++i) s += 5 * (e_hi(i) - e_lo(i) + 1);
+    return s;
+}
+constexpr int SLOTS = slot0_sum(WR);
+// First slot of window row r (0 <= r <= 8), in closed form.
+__host__ __device__ constexpr int slot0(int r) { return r <= 4 ? 5 * r * (r + 1) / 2 : SLOTS - 5 * (8 - r) * (9 - r) / 2; }
+static_assert(SLOTS == 100 && UNITS % 4 == 0, "layer-4 window");
+static_assert(slot0(1) == slot0_sum(1) && slot0(3) == slot0_sum(3) && slot0(4) == slot0_sum(4) &&
+                  slot0(5) == slot0_sum(5) && slot0(7) == slot0_sum(7) && slot0(8) == slot0_sum(8),
+              "slot table");
+}  // namespace ot
+
+__global__ void __launch_bounds__(256) k_out_taps(const uint16_t* __restrict__ a3, const uint16_t* __restrict__ W4b,
+                                                  const float* __restrict__ b4, const float* __restrict__ x,
+                                                  float* __restrict__ r, float* __restrict__ y, int H, int W) {
+    using namespace ot;
+    __shared__ __align__(16) float Y[SLOTS * YS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int n = blockIdx.z, y0 = blockIdx.y * ROWS, x0 = blockIdx.x * COLS;
+    const uint16_t* A3 = a3 + (size_t)n * H * W * 32;
+    // A: 16 window pixels x channels 8q..8q+7; unit u = wave + 4i is window
+    // row u / NBLK, block u % NBLK.
+    bf16x8 A[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int u = wave + 4 * i, rw = u / NBLK, b = u - NBLK * rw;
+        const int gy = wrap(y0 - 2 + rw, H), gx = wrap(x0 - 2 + 16 * b + l16, W);
+        A[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(A3 + ((size_t)gy * W + gx) * 32 + 8 * q));
+    }
+    // B: W4 transposed, tap 16g + l16 in N (taps 25..31 zero), channels in K.
+    bf16x8 B[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const int t = 16 * g + l16;
+        const uint4 v = t < 25 ? *reinterpret_cast<const uint4*>(W4b + t * 32 + 8 * q) : make_uint4(0, 0, 0, 0);
+        B[g] = __builtin_bit_cast(bf16x8, v);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int u = wave + 4 * i, rw = u / NBLK, b = u - NBLK * rw;
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            // Lane holds Y of window pixels 16b + 4q .. +3 for tap 16g + l16.
+            const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[g], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            const int t = 16 * g + l16, e = t / 5;
+            if (t < 25 && e >= e_lo(rw) && e <= e_hi(rw))
+                *reinterpret_cast<f32x4*>(Y + (slot0(rw) + t - 5 * e_lo(rw)) * YS + 16 * b + 4 * q) = d;
+        }
+    }
+    __syncthreads();
+    // Output row y0 + wave, column x0 + lane: taps in order, dy then dx.
+    float s = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        const int rw = wave + e;
+        const float* row = Y + (slot0(rw) + 5 * (e - e_lo(rw))) * YS + lane;
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) s += row[dx * YS + dx];
+    }
+    const size_t o = ((size_t)n * H + y0 + wave) * W + x0 + lane;
+    const float rr = s + b4[0];
+    r[o] = rr;
+    y[o] = fminf(fmaxf(x[o] + rr, 0.0f), 1.0f);
 }
 
 // MSE loss: dr = 2 (y - t) / count where 0 < x + r < 1, else 0; per-block
@@ -956,9 +1028,8 @@ __global__ void k_adam(float* __restrict__ theta, float* __restrict__ m, float* 
 // Device copies of the master weights in the layouts the kernels read:
 // W1x [32][1][32] (im2col channel t), W2 [64][25][32], W3 [32][25][64] as
 // bf16; W2T [32][25][64] and W3T [64][25][32] tap-flipped and
-// channel-swapped for the backward input gradients; W4x [16][25][32] bf16
-// (W4 in output channel 0, the rest zero) for the MFMA output layer; W4
-// [25][32] as fp32 of the bf16-rounded values.
+// channel-swapped for the backward input gradients; W4b [25][32] bf16 for
+// the output layer and W4 [25][32] as fp32 of the same bf16 values.
 __global__ void k_pack_weights(const float* __restrict__ theta, PackedWeights w) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 32 * 32) {  // W1x[co][0][t]
@@ -977,8 +1048,11 @@ __global__ void k_pack_weights(const float* __restrict__ theta, PackedWeights w)
         w.W3[i] = h;
         w.W3T[((size_t)ci * 25 + (24 - t)) * 32 + co] = h;
     }
-    if (i < 25 * 32) w.W4[i] = __uint_as_float(bf16_bits(theta[OFF_W4 + i]) << 16);
-    if (i < 16 * 25 * 32) w.W4x[i] = i < 25 * 32 ? (uint16_t)bf16_bits(theta[OFF_W4 + i]) : (uint16_t)0;
+    if (i < 25 * 32) {
+        const uint32_t h = bf16_bits(theta[OFF_W4 + i]);
+        w.W4[i] = __uint_as_float(h << 16);
+        w.W4b[i] = (uint16_t)h;
+    }
 }
 
 // Crop a (side x side) window at (cx, cy) of a torus state into dst.
@@ -1040,17 +1114,13 @@ void conv1(hipStream_t s, Profiler* p, const ConvArgs& a, const float* x, uint16
 }
 
 void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
-    if (a.W % 64 || a.H % 4 || a.Cin % 32 || a.ks != 5 ||
-        (a.mode == CONV_OUT ? (a.Cout != 16 || a.Cin != 32 || a.up || !a.bias || !a.xin || !a.rout || !a.yout)
-                            : (a.Cout != 32 && a.Cout != 64)))
+    if (a.W % 64 || a.H % 4 || a.Cin % 32 || a.ks != 5 || (a.Cout != 32 && a.Cout != 64))
         throw Error{AH_ERR_INVALID, "surrogate conv: unsupported shape"};
     if (a.mode == CONV_POOL_DGELU && (a.ks != 5 || a.up))
         throw Error{AH_ERR_INVALID, "surrogate conv: the pooled epilogue needs a 5x5 full-resolution input"};
     Launch L(p, s, name);
     dim3 grid(a.W / 64, a.H / 4, a.N);
-    if (a.mode == CONV_OUT) {
-        hipLaunchKernelGGL((k_conv_lds<1, false, 4>), grid, dim3(256), 0, s, a);
-    } else {
+    {
         // The upsampled 32-channel layer: 128 pixels per wave when the width
         // allows, so each weight fragment feeds 16 MFMAs (1008 -> 803 us per
         // call at batch 64 x 256^2; the half-resolution 32-channel input
@@ -1065,23 +1135,12 @@ void conv(hipStream_t s, Profiler* p, const ConvArgs& a, const char* name) {
     AH_HIP(hipGetLastError());
 }
 
-void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4x, const float* b4, const float* x,
+void out_layer(hipStream_t s, Profiler* p, const uint16_t* sa, const uint16_t* W4b, const float* b4, const float* x,
                float* r, float* y, int N, int H, int W) {
-    ConvArgs c{};
-    c.X = sa;
-    c.N = N;
-    c.H = H;
-    c.W = W;
-    c.Cin = 32;
-    c.ks = 5;
-    c.Wt = W4x;
-    c.Cout = 16;
-    c.bias = b4;
-    c.mode = CONV_OUT;
-    c.xin = x;
-    c.rout = r;
-    c.yout = y;
-    conv(s, p, c, "surr_out_layer");
+    if (W % ot::COLS || H % ot::ROWS || N < 1) throw Error{AH_ERR_INVALID, "surrogate layer 4: unsupported shape"};
+    Launch L(p, s, "surr_out_layer");
+    hipLaunchKernelGGL(k_out_taps, dim3(W / ot::COLS, H / ot::ROWS, N), dim3(256), 0, s, sa, W4b, b4, x, r, y, H, W);
+    AH_HIP(hipGetLastError());
 }
 
 int loss_blocks(size_t total) { return (int)std::min<size_t>(1024, (total + 255) / 256); }
