@@ -29,7 +29,7 @@ struct Work {
     bool train = false;
     std::vector<void*> bufs;
     uint16_t *xb = nullptr, *a1 = nullptr, *p = nullptr, *a2 = nullptr, *s = nullptr;
-    float *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;  // gelu' of each layer's pre-activation (training)
+    uint16_t *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;  // bf16 gelu' of each layer's pre-activation (training)
     float *r = nullptr, *y = nullptr;
     float *dr = nullptr, *dp = nullptr, *dz1 = nullptr;
     uint16_t *dz3 = nullptr, *dz2 = nullptr;
@@ -113,9 +113,9 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
     w.y = w.get<float>(P);
     w.lpart = w.get<double>(1024 * 2);
     if (train) {
-        w.g1 = w.get<float>(P * C1);
-        w.g2 = w.get<float>(Ph * C2);
-        w.g3 = w.get<float>(P * C1);
+        w.g1 = w.get<uint16_t>(P * C1);
+        w.g2 = w.get<uint16_t>(Ph * C2);
+        w.g3 = w.get<uint16_t>(P * C1);
         w.dr = w.get<float>(P);
         w.dz3 = w.get<uint16_t>(P * C1);
         w.dz2 = w.get<uint16_t>(Ph * C2);

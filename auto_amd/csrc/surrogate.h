@@ -34,10 +34,10 @@ struct ConvArgs {
     const float* bias;   // [Cout] or null
     int mode;            // ConvMode
     float* z;            // fp32 [N][H][W][Cout] or null: the output of CONV_LINEAR
-    float* dgelu;        // fp32 [N][H][W][Cout] or null: gelu'(z) of the GELU modes (kept for training)
+    uint16_t* dgelu;     // bf16 [N][H][W][Cout] or null: gelu'(z) of the GELU modes (kept for training)
     uint16_t* a;         // bf16 [N][H][W][Cout]: gelu(z) (+ add)
     const uint16_t* add; // bf16 [N][H][W][Cout] for CONV_GELU_ADD
-    const float* dgelu_in;  // fp32 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
+    const uint16_t* dgelu_in;  // bf16 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
     uint16_t* dpool;     // bf16 [N][H/2][W/2][Cout] for CONV_POOL_DGELU
 };
 
@@ -74,10 +74,10 @@ void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const floa
 // The layer-4 input gradient ds = sum over taps of W4 * dr is never stored:
 // dz3_from_dr forms it and writes dz3 = bf16(ds * g3); dz1_from_dr forms the
 // same values again and writes dz1 = (ds + dp / 4) * g1, with g1 and g3 the
-// gelu' values the forward pass kept.
-void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* g3, uint16_t* out, int N,
+// gelu' values the forward pass kept (bf16).
+void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const uint16_t* g3, uint16_t* out, int N,
                  int H, int W);
-void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const float* g1,
+void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
                  float* out, int N, int H, int W);
 // Weight gradient `out` and bias gradient `bout` of one layer; `part`
 // (WGRAD_PART_FLOATS) and `bpart` (WGRAD_BPART_FLOATS) hold per-workgroup

@@ -9,8 +9,9 @@
 // conv5 = 5x5 cross-correlation with circular padding, gelu = z*Phi(z).
 //
 // Layout: activations are NHWC, bf16 where a convolution reads them (a1, p,
-// a2, s, dz3, dz2 and xb = bf16(x)), FP32 where only elementwise code reads
-// them (gelu' of the pre-activations, kept for the backward pass, and the backward input gradients).
+// a2, s, dz3, dz2 and xb = bf16(x)) and for gelu' of the pre-activations,
+// kept for the backward pass (half the bytes of the largest tensors the
+// step moves); FP32 for the backward input gradients dp and dz1.
 //
 // Hot kernels run on MFMA as an implicit GEMM, D[co][pixel] = sum over
 // (tap, ci) of W[co][tap][ci] * X[pixel+tap][ci], with
@@ -45,6 +46,19 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
     return u >> 16;
 }
 __device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Four floats as bf16 (round to nearest even), element 0 in the low half.
+__device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
+    return make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+}
+// Eight bf16 (one 16-byte load) as floats.
+__device__ __forceinline__ void unpack8(const uint4& u, float (&v)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[2 * k] = __uint_as_float(w[k] << 16);
+        v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+    }
+}
 
 // gelu(z) = z Phi(z) and, from the same erf, gelu'(z) = Phi(z) + z phi(z).
 __device__ __forceinline__ float gelu_and_grad(float z, float& dg) {
@@ -74,7 +88,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&a
                 float v[4], dg[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[r] = gelu_and_grad(z[r], dg[r]);
-                if (a.dgelu) *reinterpret_cast<float4*>(a.dgelu + o) = make_float4(dg[0], dg[1], dg[2], dg[3]);
+                if (a.dgelu) *reinterpret_cast<uint2*>(a.dgelu + o) = pack4(dg);
                 if (a.mode == CONV_GELU_ADD) {
                     const uint2 ad = *reinterpret_cast<const uint2*>(a.add + o);
                     v[0] += __uint_as_float(ad.x << 16);
@@ -159,7 +173,7 @@ __global__ void __launch_bounds__(256) k_conv1(ConvArgs a, const float* __restri
                 h[r] = bf16_bits(gelu_and_grad(acc[m][b][r] + a.bias[co + r], dg[r]));
                 act[m][b][r] = __uint_as_float(h[r] << 16);
             }
-            if (a.dgelu) *reinterpret_cast<float4*>(a.dgelu + o) = make_float4(dg[0], dg[1], dg[2], dg[3]);
+            if (a.dgelu) *reinterpret_cast<uint2*>(a.dgelu + o) = pack4(dg);
             *reinterpret_cast<uint2*>(a.a + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
         }
     float* sc = odd_rows + (wave >> 1) * (2 * NB * 4 * 64);
@@ -314,8 +328,9 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
                     if (!(l16 & 1)) {
                         const int xh = (x0 + 16 * b + l16) >> 1, co = 16 * m + 4 * q;
                         const size_t o = (((size_t)n * Hh + yh) * Wh + xh) * a.Cout + co;
-                        const float4 dg = *reinterpret_cast<const float4*>(a.dgelu_in + o);
-                        const float g[4] = {v[0] * dg.x, v[1] * dg.y, v[2] * dg.z, v[3] * dg.w};
+                        const uint2 dg = *reinterpret_cast<const uint2*>(a.dgelu_in + o);
+                        const float g[4] = {v[0] * bf16f(dg.x), v[1] * bf16f(dg.x >> 16), v[2] * bf16f(dg.y),
+                                            v[3] * bf16f(dg.y >> 16)};
                         *reinterpret_cast<uint2*>(a.dpool + o) = make_uint2(bf16_bits(g[0]) | (bf16_bits(g[1]) << 16),
                                                                             bf16_bits(g[2]) | (bf16_bits(g[3]) << 16));
                     }
@@ -470,10 +485,10 @@ __device__ __forceinline__ void ds8(const float* w, const float* drt, int p, int
 }
 }  // namespace dz
 
-// dz3 = bf16(ds * gelu'(z3)), g3 = gelu'(z3) as the forward pass stored it.
+// dz3 = bf16(ds * gelu'(z3)), g3 = gelu'(z3) as the forward pass stored it (bf16).
 __global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, const float* __restrict__ W4,
-                                                const float* __restrict__ g3, uint16_t* __restrict__ dz3, int N, int H,
-                                                int W) {
+                                                const uint16_t* __restrict__ g3, uint16_t* __restrict__ dz3, int N,
+                                                int H, int W) {
     __shared__ float w[25 * 32];
     __shared__ float drt[5 * dz::COLS];
     const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * dz::SEG;
@@ -482,9 +497,8 @@ __global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, co
     const size_t pix = ((size_t)n * H + y) * W + x0 + p;
     float ds[8];
     dz::ds8(w, drt, p, c8, ds);
-    const float4* gp = reinterpret_cast<const float4*>(g3 + pix * 32 + 8 * c8);
-    const float4 ga = gp[0], gb = gp[1];
-    const float g[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+    float g[8];
+    unpack8(*reinterpret_cast<const uint4*>(g3 + pix * 32 + 8 * c8), g);
     uint32_t h[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) h[k] = bf16_bits(ds[k] * g[k]);
@@ -492,11 +506,11 @@ __global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, co
         make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
-// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, g1 = gelu'(z1) as stored (the
-// skip connection s = gelu(z3) + a1 passes ds to a1 unchanged; dp is the
+// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, g1 = gelu'(z1) as stored (bf16;
+// the skip connection s = gelu(z3) + a1 passes ds to a1 unchanged; dp is the
 // pooled path's gradient at half resolution).
 __global__ void __launch_bounds__(256) k_dz1_dr(const float* __restrict__ dr, const float* __restrict__ W4,
-                                                const float* __restrict__ dp, const float* __restrict__ g1,
+                                                const float* __restrict__ dp, const uint16_t* __restrict__ g1,
                                                 float* __restrict__ dz1, int N, int H, int W) {
     __shared__ float w[25 * 32];
     __shared__ float drt[5 * dz::COLS];
@@ -508,10 +522,10 @@ __global__ void __launch_bounds__(256) k_dz1_dr(const float* __restrict__ dr, co
     dz::ds8(w, drt, p, c8, ds);
     const size_t ph = (((size_t)n * (H / 2) + y / 2) * (W / 2) + (x0 + p) / 2) * 32 + 8 * c8;
     const float4* pp = reinterpret_cast<const float4*>(dp + ph);
-    const float4* gp = reinterpret_cast<const float4*>(g1 + pix * 32 + 8 * c8);
-    const float4 pa = pp[0], pb = pp[1], ga = gp[0], gb = gp[1];
+    const float4 pa = pp[0], pb = pp[1];
     const float d[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
-    const float g[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+    float g[8];
+    unpack8(*reinterpret_cast<const uint4*>(g1 + pix * 32 + 8 * c8), g);
     float o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = (ds[k] + 0.25f * d[k]) * g[k];
@@ -1152,7 +1166,7 @@ void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const floa
     AH_HIP(hipGetLastError());
 }
 
-void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* g3, uint16_t* out, int N,
+void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const uint16_t* g3, uint16_t* out, int N,
                  int H, int W) {
     if (W % dz::SEG) throw Error{AH_ERR_INVALID, "surrogate dz3: width % 64 != 0"};
     Launch L(p, s, "surr_dz3");
@@ -1160,7 +1174,7 @@ void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, c
     AH_HIP(hipGetLastError());
 }
 
-void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const float* g1,
+void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
                  float* out, int N, int H, int W) {
     if (W % dz::SEG || H % 2) throw Error{AH_ERR_INVALID, "surrogate dz1: width % 64 != 0 or odd height"};
     Launch L(p, s, "surr_dz1");

@@ -26,8 +26,9 @@ W2 [64][25][32], b2 [64], W3 [32][25][64], b3 [32], W4 [1][25][32], b4 [1]
 
 With bf16_points=True the oracle rounds to bfloat16 where the GPU stores
 bf16: the weights, the input x of the first convolution, the activations a1,
-p, a2 and s (s = bf16(gelu(z3) + a1)), and the gradients dz3, dz2 that feed
-MFMA convolutions.  Everything else (pre-activations z, r, the backward
+p, a2 and s (s = bf16(gelu(z3) + a1)), the gelu'(z) values the forward pass
+keeps for the backward pass, and the gradients dz3, dz2 that feed MFMA
+convolutions.  Everything else (pre-activations z, r, the backward
 input gradients ds, du, dp, dz1) stays in FP32 on the GPU and FP64 here.  The analytic backward is pinned by central finite differences of the
 FP64 loss (tests/test_surrogate_oracle.py).
 """
@@ -148,17 +149,17 @@ def loss_and_grad(theta: np.ndarray, x: np.ndarray, target: np.ndarray, bf16_poi
     g["b4"] = dr.sum(axis=(0, 1, 2), keepdims=False).reshape(1)
     g["W4"] = conv5_weight_grad(dr4, c["s"])
     ds = conv5_input_grad(dr4, W["W4"])
-    dz3 = rd(ds * gelu_grad(c["z3"]))
+    dz3 = rd(ds * rd(gelu_grad(c["z3"])))
     g["b3"] = dz3.sum(axis=(0, 2, 3))
     g["W3"] = conv5_weight_grad(dz3, c["u"])
     du = conv5_input_grad(dz3, W["W3"])
     n, ch, h, w = du.shape
     da2 = du.reshape(n, ch, h // 2, 2, w // 2, 2).sum(axis=(3, 5))
-    dz2 = rd(da2 * gelu_grad(c["z2"]))
+    dz2 = rd(da2 * rd(gelu_grad(c["z2"])))
     g["b2"] = dz2.sum(axis=(0, 2, 3))
     g["W2"] = conv5_weight_grad(dz2, c["p"])
     dp = conv5_input_grad(dz2, W["W2"])
-    dz1 = (ds + up2(dp) / 4.0) * gelu_grad(c["z1"])
+    dz1 = (ds + up2(dp) / 4.0) * rd(gelu_grad(c["z1"]))
     g["b1"] = dz1.sum(axis=(0, 2, 3))
     g["W1"] = conv5_weight_grad(dz1, c["x_in"])
     return loss, pack(g)
