@@ -61,10 +61,20 @@ __device__ __forceinline__ void unpack8(const uint4& u, float (&v)[8]) {
 }
 
 // gelu(z) = z Phi(z) and, from the same erf, gelu'(z) = Phi(z) + z phi(z).
+// erf by Abramowitz & Stegun 7.1.26 (Hastings): erf(x) = 1 - P(t) exp(-x^2),
+// t = 1 / (1 + 0.3275911 x), |error| <= 1.5e-7 for x >= 0.  With x = |z|/sqrt(2)
+// the exponential is exp(-z^2/2), which gelu' needs as well, so one exp
+// serves both.
 __device__ __forceinline__ float gelu_and_grad(float z, float& dg) {
-    const float e = erff(z * 0.70710678118654752f);
-    dg = 0.5f * (1.0f + e) + z * __expf(-0.5f * z * z) * 0.3989422804014327f;
-    return 0.5f * z * (1.0f + e);
+    const float E = __expf(-0.5f * z * z);
+    const float x = fabsf(z) * 0.70710678118654752f;
+    const float t = __frcp_rn(fmaf(0.3275911f, x, 1.0f));
+    const float P = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                             0.254829592f);
+    const float h = 0.5f * P * E;                 // 1 - Phi(|z|)
+    const float phi = z >= 0.0f ? 1.0f - h : h;  // Phi(z)
+    dg = fmaf(z * E, 0.3989422804014327f, phi);
+    return z * phi;
 }
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
