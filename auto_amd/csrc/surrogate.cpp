@@ -31,7 +31,7 @@ struct Work {
     uint16_t *xb = nullptr, *a1 = nullptr, *p = nullptr, *a2 = nullptr, *s = nullptr;
     uint16_t *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;  // bf16 gelu' of each layer's pre-activation (training)
     float *r = nullptr, *y = nullptr;
-    float *dr = nullptr, *dp = nullptr, *dz1 = nullptr;
+    float *dr = nullptr, *dp = nullptr;
     uint16_t *dz3 = nullptr, *dz2 = nullptr;
     float* part = nullptr;
     float* bpart = nullptr;  // bias-gradient partials, 2048 workgroups x 64 channels
@@ -120,7 +120,6 @@ void ensure_work(ah_surrogate* S, int N, int H, int W, bool train) {
         w.dz3 = w.get<uint16_t>(P * C1);
         w.dz2 = w.get<uint16_t>(Ph * C2);
         w.dp = w.get<float>(Ph * C1);
-        w.dz1 = w.get<float>(P * C1);
         w.part = w.get<float>(WGRAD_PART_FLOATS);
         w.bpart = w.get<float>(WGRAD_BPART_FLOATS);
     }
@@ -252,19 +251,8 @@ double backward(ah_surrogate* S, const float* x, const float* target, bool grad)
         c.mode = CONV_LINEAR;
         c.z = w.dp;
         conv(st, pf, c, "surr_conv2_dgrad");
-        // layer 1
-        dz1_from_dr(st, pf, w.dr, S->pw.W4, w.dp, w.g1, w.dz1, N, H, W);
-        a = WGradArgs{};
-        a.dz_f = w.dz1;
-        a.X = w.xb;
-        a.xstride = 1;
-        a.xoff = 0;
-        a.N = N;
-        a.H = H;
-        a.W = W;
-        a.Cin = 1;
-        a.Cout = C1;
-        wgrad(st, pf, a, w.part, g + OFF_W1, w.bpart, g + OFF_B1);
+        // layer 1 (dz1 is formed inside the weight-gradient kernel)
+        dz1_wgrad(st, pf, w.dr, S->pw.W4, w.dp, w.g1, w.xb, N, H, W, w.part, g + OFF_W1, w.bpart, g + OFF_B1);
     }
     std::vector<double> part(lb);
     AH_HIP(hipMemcpyAsync(part.data(), w.lpart, sizeof(double) * lb, hipMemcpyDeviceToHost, st));

@@ -72,18 +72,19 @@ int loss_blocks(size_t total);
 void loss(hipStream_t s, Profiler* p, const float* y, const float* t, const float* x, const float* r, float* dr,
           double* part, size_t total);
 // The layer-4 input gradient ds = sum over taps of W4 * dr is never stored:
-// dz3_from_dr forms it and writes dz3 = bf16(ds * g3); dz1_from_dr forms the
-// same values again and writes dz1 = (ds + dp / 4) * g1, with g1 and g3 the
-// gelu' values the forward pass kept (bf16).
+// dz3_from_dr forms it and writes dz3 = bf16(ds * g3); dz1_wgrad forms the
+// same values again, dz1 = (ds + dp / 4) * g1 (never stored), and from it
+// the layer-1 weight and bias gradients into out and bout, with g1 and g3
+// the gelu' values the forward pass kept (bf16) and xb = bf16(x).
 void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const uint16_t* g3, uint16_t* out, int N,
                  int H, int W);
-void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
-                 float* out, int N, int H, int W);
+void dz1_wgrad(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
+               const uint16_t* xb, int N, int H, int W, float* part, float* out, float* bpart, float* bout);
 // Weight gradient `out` and bias gradient `bout` of one layer; `part`
 // (WGRAD_PART_FLOATS) and `bpart` (WGRAD_BPART_FLOATS) hold per-workgroup
-// partials.  Layers 1 and 4 (one channel on one side), layer 2 (32 -> 64)
-// and layer 3 (64 -> 32 over the upsampled input) each have their kernel;
-// other shapes are an error.
+// partials.  Layer 4 (32 -> 1, fp32 dz_f), layer 2 (32 -> 64) and layer 3
+// (64 -> 32 over the upsampled input) each have their kernel (layer 1's is
+// dz1_wgrad); other shapes are an error.
 constexpr size_t WGRAD_PART_FLOATS = (size_t)256 * 64 * 25 * 32;
 constexpr size_t WGRAD_BPART_FLOATS = (size_t)2048 * 64;
 void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, float* out, float* bpart, float* bout);

@@ -516,34 +516,6 @@ __global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, co
         make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
-// dz1 = (ds + dp[p/2] / 4) * gelu'(z1), fp32, g1 = gelu'(z1) as stored (bf16;
-// the skip connection s = gelu(z3) + a1 passes ds to a1 unchanged; dp is the
-// pooled path's gradient at half resolution).
-__global__ void __launch_bounds__(256) k_dz1_dr(const float* __restrict__ dr, const float* __restrict__ W4,
-                                                const float* __restrict__ dp, const uint16_t* __restrict__ g1,
-                                                float* __restrict__ dz1, int N, int H, int W) {
-    __shared__ float w[25 * 32];
-    __shared__ float drt[5 * dz::COLS];
-    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * dz::SEG;
-    dz::stage(w, drt, W4, dr, n, y, x0, H, W);
-    const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
-    const size_t pix = ((size_t)n * H + y) * W + x0 + p;
-    float ds[8];
-    dz::ds8(w, drt, p, c8, ds);
-    const size_t ph = (((size_t)n * (H / 2) + y / 2) * (W / 2) + (x0 + p) / 2) * 32 + 8 * c8;
-    const float4* pp = reinterpret_cast<const float4*>(dp + ph);
-    const float4 pa = pp[0], pb = pp[1];
-    const float d[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
-    float g[8];
-    unpack8(*reinterpret_cast<const uint4*>(g1 + pix * 32 + 8 * c8), g);
-    float o[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) o[k] = (ds[k] + 0.25f * d[k]) * g[k];
-    float4* op = reinterpret_cast<float4*>(dz1 + pix * 32 + 8 * c8);
-    op[0] = make_float4(o[0], o[1], o[2], o[3]);
-    op[1] = make_float4(o[4], o[5], o[6], o[7]);
-}
-
 // ---------------------------------------------------------------- weight grads
 // dW[co][t][ci] = sum over pixels of dZ[pix][co] * X[pix + tap_t][ci], the
 // bias gradient the sum of dZ; per-workgroup partials in fixed ranges of
@@ -874,7 +846,8 @@ __global__ void k_reduce_up(const float* __restrict__ part, int groups, float* _
 // the shifted one as its 5 x 68 tile) and each of its 4 waves adds two
 // 32-pixel chunks into its own 16 x 16 output block (mb = wave / 2, nb =
 // wave % 2).  The bias gradient's partial (the sum of U over the workgroup's
-// pixels, segment by segment in pixel order) goes to bpart.
+// pixels, segment by segment in pixel order) goes to bpart.  Layer 1's U,
+// dz1, is formed in LDS by the same kernel (k_dz1_wgrad) and never stored.
 namespace wt {
 constexpr int VS = wg::SEG + 8;  // layer 4: bf16 per channel row of V^T
 constexpr int US = wg::SEG + 4;  // layer 1: fp32 per channel row of U^T
@@ -893,26 +866,22 @@ __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& 
                                                l[6] | (l[7] << 16)));
 }
 
-template <int LAYER>
-__global__ void __launch_bounds__(256) k_wgrad_thin_mfma(WGradArgs a, float* __restrict__ part,
-                                                          float* __restrict__ bpart) {
+// Layer 4: U = dr (one channel), V = s (32 channels).
+__global__ void __launch_bounds__(256) k_wgrad_thin4(WGradArgs a, float* __restrict__ part, float* __restrict__ bpart) {
     using wg::SEG;
     using wg::ROWS;
     using wg::COLS;
-    static_assert(LAYER == 1 || LAYER == 4, "thin layers");
-    constexpr int A = LAYER == 1 ? 32 : 1;
-    // Layer 4: U tile [ROWS][COLS] fp32 and V^T [32][VS] bf16.
-    // Layer 1: U^T [32][US] fp32 and V tile [ROWS][COLS] bf16.
-    __shared__ __attribute__((aligned(16))) float uf[LAYER == 4 ? ROWS * COLS : 32 * wt::US];
-    __shared__ __attribute__((aligned(16))) uint16_t vh[LAYER == 4 ? 32 * wt::VS : ROWS * COLS];
+    // U tile [ROWS][COLS] fp32 and V^T [32][VS] bf16.
+    __shared__ __attribute__((aligned(16))) float uf[ROWS * COLS];
+    __shared__ __attribute__((aligned(16))) uint16_t vh[32 * wt::VS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, l16 = lane & 15;
     const int mb = wave >> 1, nb = wave & 1;
     const int H = a.H, W = a.W, segs_per_row = W / SEG;
     const long nseg = (long)a.N * H * segs_per_row;
     const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
-    // This lane's tap (row m of layer 4's A operand, column n of layer 1's B operand).
-    const int t = LAYER == 4 ? 16 * mb + l16 : 16 * nb + l16;
+    // This lane's tap: row m of the A operand.
+    const int t = 16 * mb + l16;
     const bool tap_ok = t < 25;
     const int dy = tap_ok ? t / 5 - 2 : 0, dx = tap_ok ? t % 5 - 2 : 0;
     f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -922,87 +891,136 @@ __global__ void __launch_bounds__(256) k_wgrad_thin_mfma(WGradArgs a, float* __r
         const long ry = sg / segs_per_row;
         const int y = (int)(ry % H), n = (int)(ry / H), x0 = xb * SEG;
         __syncthreads();  // the previous segment's reads are done
-        if constexpr (LAYER == 4) {
-            for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
-                const int r = i / COLS, c = i % COLS;
-                uf[i] = a.dz_f[((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W)];
-            }
-            {   // V^T: thread = (pixel, 8 channels)
-                const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
-                const uint4 v = *reinterpret_cast<const uint4*>(
-                    a.X + (((size_t)n * H + y) * W + x0 + p) * a.xstride + a.xoff + 8 * c8);
-                const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+        for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
+            const int r = i / COLS, c = i % COLS;
+            uf[i] = a.dz_f[((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W)];
+        }
+        {   // V^T: thread = (pixel, 8 channels)
+            const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(a.X + (((size_t)n * H + y) * W + x0 + p) * a.xstride +
+                                                            a.xoff + 8 * c8);
+            const uint32_t e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    vh[(8 * c8 + 2 * j) * wt::VS + p] = (uint16_t)(e[j] & 0xffffu);
-                    vh[(8 * c8 + 2 * j + 1) * wt::VS + p] = (uint16_t)(e[j] >> 16);
-                }
-            }
-        } else {
-            {   // U^T: thread = (pixel, 8 channels)
-                const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;
-                const float4* src = reinterpret_cast<const float4*>(a.dz_f + (((size_t)n * H + y) * W + x0 + p) * 32 +
-                                                                    8 * c8);
-                const float4 u0 = src[0], u1 = src[1];
-                const float e[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-#pragma unroll
-                for (int j = 0; j < 8; ++j) uf[(8 * c8 + j) * wt::US + p] = e[j];
-            }
-            for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
-                const int r = i / COLS, c = i % COLS;
-                vh[i] = a.X[(((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W)) * a.xstride + a.xoff];
+            for (int j = 0; j < 4; ++j) {
+                vh[(8 * c8 + 2 * j) * wt::VS + p] = (uint16_t)(e[j] & 0xffffu);
+                vh[(8 * c8 + 2 * j + 1) * wt::VS + p] = (uint16_t)(e[j] >> 16);
             }
         }
         __syncthreads();
-        if constexpr (LAYER == 4) {
-            if (threadIdx.x == 0)
-                for (int p = 0; p < SEG; ++p) bacc += uf[2 * COLS + 2 + p];
-        } else {
-            if ((int)threadIdx.x < A)
-                for (int p = 0; p < SEG; ++p) bacc += uf[threadIdx.x * wt::US + p];
-        }
+        if (threadIdx.x == 0)
+            for (int p = 0; p < SEG; ++p) bacc += uf[2 * COLS + 2 + p];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const int k0 = 32 * c + 8 * q;  // this lane's first pixel of the chunk
+            // A: U at (y - dy, x0 + k0 + j - dx), tile row 2 - dy, column k0 + j - dx + 2.
             float u8[8];
-            bf16x8 B;
-            if constexpr (LAYER == 4) {
-                // A: U at (y - dy, x0 + k0 + j - dx), tile row 2 - dy, column k0 + j - dx + 2.
 #pragma unroll
-                for (int j = 0; j < 8; ++j) u8[j] = tap_ok ? uf[(2 - dy) * COLS + k0 + j - dx + 2] : 0.0f;
-                B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(vh + (16 * nb + l16) * wt::VS + k0));
-            } else {
-                const float4* up = reinterpret_cast<const float4*>(uf + (16 * mb + l16) * wt::US + k0);
-                const float4 u0 = up[0], u1 = up[1];
-                u8[0] = u0.x; u8[1] = u0.y; u8[2] = u0.z; u8[3] = u0.w;
-                u8[4] = u1.x; u8[5] = u1.y; u8[6] = u1.z; u8[7] = u1.w;
-                // B: V at (y + dy, x0 + k0 + j + dx), tile row dy + 2, column k0 + j + dx + 2.
-                uint32_t h[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t v0 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + dx + 2] : 0u;
-                    const uint32_t v1 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + 1 + dx + 2] : 0u;
-                    h[j] = v0 | (v1 << 16);
-                }
-                B = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
-            }
+            for (int j = 0; j < 8; ++j) u8[j] = tap_ok ? uf[(2 - dy) * COLS + k0 + j - dx + 2] : 0.0f;
+            const bf16x8 B =
+                __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(vh + (16 * nb + l16) * wt::VS + k0));
             bf16x8 hi, lo;
             split8(u8, hi, lo);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B, acc, 0, 0, 0);
         }
     }
-    if ((int)threadIdx.x < A) bpart[(size_t)blockIdx.x * A + threadIdx.x] = bacc;
-    // D: rows 16 mb + 4 q + r, column 16 nb + l16.
+    if (threadIdx.x == 0) bpart[blockIdx.x] = bacc;
+    // D: rows 16 mb + 4 q + r (tap), column 16 nb + l16 (channel).
     float* dst = part + (size_t)blockIdx.x * 800;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int m = 16 * mb + 4 * q + r, nn = 16 * nb + l16;
-        if constexpr (LAYER == 4) {
-            if (m < 25) dst[m * 32 + nn] = acc[r];
-        } else {
-            if (nn < 25) dst[m * 25 + nn] = acc[r];
+        const int m = 16 * mb + 4 * q + r;
+        if (m < 25) dst[m * 32 + 16 * nb + l16] = acc[r];
+    }
+}
+
+// Layer 1: per segment, dz1 = (ds + dp[p/2] / 4) * gelu'(z1) (ds formed from
+// the segment's 5 x 68 tile of dr as in k_dz3_dr; dp the pooled path's
+// gradient at half resolution; gelu'(z1) as the forward pass kept it) goes
+// into LDS as U^T [channel][pixel]; V = bf16(x) from its 5 x 68 tile.
+__global__ void __launch_bounds__(256) k_dz1_wgrad(const float* __restrict__ dr, const float* __restrict__ W4,
+                                                   const float* __restrict__ dp, const uint16_t* __restrict__ g1,
+                                                   const uint16_t* __restrict__ xb, int N, int H, int W,
+                                                   float* __restrict__ part, float* __restrict__ bpart) {
+    using wg::SEG;
+    using wg::ROWS;
+    using wg::COLS;
+    static_assert(dz::SEG == SEG && dz::COLS == COLS && ROWS == 5, "segment tiles");
+    __shared__ float w4[25 * 32];
+    __shared__ float drt[ROWS * COLS];
+    __shared__ __attribute__((aligned(16))) float uf[32 * wt::US];
+    __shared__ __attribute__((aligned(16))) uint16_t vh[ROWS * COLS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const int mb = wave >> 1, nb = wave & 1;
+    const int p = threadIdx.x >> 2, c8 = threadIdx.x & 3;  // dz1: pixel p, channels 8 c8 .. +7
+    const int segs_per_row = W / SEG;
+    const long nseg = (long)N * H * segs_per_row;
+    const long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
+    // This lane's tap: column n of the B operand.
+    const int t = 16 * nb + l16;
+    const bool tap_ok = t < 25;
+    const int dy = tap_ok ? t / 5 - 2 : 0, dx = tap_ok ? t % 5 - 2 : 0;
+    for (int i = threadIdx.x; i < 25 * 32; i += 256) w4[i] = W4[i];
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;
+    for (long sg = s0; sg < s1; ++sg) {
+        const int xs = (int)(sg % segs_per_row);
+        const long ry = sg / segs_per_row;
+        const int y = (int)(ry % H), n = (int)(ry / H), x0 = xs * SEG;
+        // This thread's dp and gelu'(z1), in flight during the staging.
+        const size_t pix = ((size_t)n * H + y) * W + x0 + p;
+        const float4* pp =
+            reinterpret_cast<const float4*>(dp + (((size_t)n * (H / 2) + y / 2) * (W / 2) + (x0 + p) / 2) * 32 + 8 * c8);
+        const float4 pa = pp[0], pb = pp[1];
+        const uint4 gu = *reinterpret_cast<const uint4*>(g1 + pix * 32 + 8 * c8);
+        __syncthreads();  // the previous segment's reads are done (and w4 is written)
+        for (int i = threadIdx.x; i < ROWS * COLS; i += 256) {
+            const int r = i / COLS, c = i - r * COLS;
+            const size_t o = ((size_t)n * H + wrap(y + r - 2, H)) * W + wrap(x0 + c - 2, W);
+            drt[i] = dr[o];
+            vh[i] = xb[o];
         }
+        __syncthreads();
+        {
+            float ds[8], g[8];
+            dz::ds8(w4, drt, p, c8, ds);
+            unpack8(gu, g);
+            const float d[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) uf[(8 * c8 + k) * wt::US + p] = (ds[k] + 0.25f * d[k]) * g[k];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < 32)
+            for (int i = 0; i < SEG; ++i) bacc += uf[threadIdx.x * wt::US + i];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int k0 = 32 * c + 8 * q;  // this lane's first pixel of the chunk
+            const float4* up = reinterpret_cast<const float4*>(uf + (16 * mb + l16) * wt::US + k0);
+            const float4 u0 = up[0], u1 = up[1];
+            const float u8[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+            // B: V at (y + dy, x0 + k0 + j + dx), tile row dy + 2, column k0 + j + dx + 2.
+            uint32_t h[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t v0 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + dx + 2] : 0u;
+                const uint32_t v1 = tap_ok ? vh[(dy + 2) * COLS + k0 + 2 * j + 1 + dx + 2] : 0u;
+                h[j] = v0 | (v1 << 16);
+            }
+            const bf16x8 B = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+            bf16x8 hi, lo;
+            split8(u8, hi, lo);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B, acc, 0, 0, 0);
+        }
+    }
+    if ((int)threadIdx.x < 32) bpart[(size_t)blockIdx.x * 32 + threadIdx.x] = bacc;
+    // D: rows 16 mb + 4 q + r (channel), column 16 nb + l16 (tap).
+    float* dst = part + (size_t)blockIdx.x * 800;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int nn = 16 * nb + l16;
+        if (nn < 25) dst[(16 * mb + 4 * q + r) * 25 + nn] = acc[r];
     }
 }
 
@@ -1184,11 +1202,22 @@ void dz3_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, c
     AH_HIP(hipGetLastError());
 }
 
-void dz1_from_dr(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
-                 float* out, int N, int H, int W) {
-    if (W % dz::SEG || H % 2) throw Error{AH_ERR_INVALID, "surrogate dz1: width % 64 != 0 or odd height"};
-    Launch L(p, s, "surr_dz1");
-    hipLaunchKernelGGL(k_dz1_dr, dim3(W / dz::SEG, H, N), dim3(256), 0, s, dr, W4, dp, g1, out, N, H, W);
+void dz1_wgrad(hipStream_t s, Profiler* p, const float* dr, const float* W4, const float* dp, const uint16_t* g1,
+               const uint16_t* xb, int N, int H, int W, float* part, float* out, float* bpart, float* bout) {
+    if (W % wg::SEG || H % 2 || N < 1) throw Error{AH_ERR_INVALID, "surrogate layer 1: width % 64 != 0 or odd height"};
+    const int chunks = wg::THIN_GROUPS;
+    {
+        Launch L(p, s, "surr_dz1_wgrad");
+        hipLaunchKernelGGL(k_dz1_wgrad, dim3(chunks), dim3(256), 0, s, dr, W4, dp, g1, xb, N, H, W, part, bpart);
+        AH_HIP(hipGetLastError());
+    }
+    {
+        Launch L(p, s, "surr_reduce");
+        hipLaunchKernelGGL(k_reduce_few, dim3(32 * 25), dim3(256), 0, s, part, chunks, 32 * 25, out);
+        AH_HIP(hipGetLastError());
+    }
+    Launch L(p, s, "surr_reduce_bias");
+    hipLaunchKernelGGL(k_reduce_few, dim3(32), dim3(256), 0, s, bpart, chunks, 32, bout);
     AH_HIP(hipGetLastError());
 }
 
@@ -1198,15 +1227,11 @@ void wgrad(hipStream_t s, Profiler* p, const WGradArgs& a, float* part, float* o
     const bool up = a.dz_h && a.xoff == 0 && a.up && a.Cout == wu::COUT && a.Cin == wu::CIN && a.H % 2 == 0 &&
                     (a.W / 2) % wu::SEG == 0;
     const bool mfma = a.dz_h && a.xoff == 0 && a.W % wg::SEG == 0 && a.Cout == 64 && a.Cin == 32 && !a.up;
-    const bool thin = a.dz_f && a.W % wg::SEG == 0 && !a.up &&
-                      ((a.Cout == 32 && a.Cin == 1) || (a.Cout == 1 && a.Cin == 32));
+    const bool thin = a.dz_f && a.W % wg::SEG == 0 && !a.up && a.Cout == 1 && a.Cin == 32;
     if (thin) {
         chunks = wg::THIN_GROUPS;
         Launch L(p, s, "surr_wgrad_thin");
-        if (a.Cout == 32)
-            hipLaunchKernelGGL((k_wgrad_thin_mfma<1>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
-        else
-            hipLaunchKernelGGL((k_wgrad_thin_mfma<4>), dim3(chunks), dim3(256), 0, s, a, part, bpart);
+        hipLaunchKernelGGL(k_wgrad_thin4, dim3(chunks), dim3(256), 0, s, a, part, bpart);
         AH_HIP(hipGetLastError());
     } else if (up) {
         chunks = 4 * wu::UPG;  // bias partials: workgroup x parity
