@@ -39,17 +39,17 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// Two floats as packed bf16 (element a in the low half), rounded to nearest
+// even by one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
+__device__ __forceinline__ uint32_t bf16_bits(float f) { return pack2(f, 0.0f) & 0xffffu; }
 __device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // Four floats as bf16 (round to nearest even), element 0 in the low half.
-__device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
-    return make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
-}
+__device__ __forceinline__ uint2 pack4(const float (&v)[4]) { return make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3])); }
 // Eight bf16 (one 16-byte load) as floats.
 __device__ __forceinline__ void unpack8(const uint4& u, float (&v)[8]) {
     const uint32_t w[4] = {u.x, u.y, u.z, u.w};
@@ -106,8 +106,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, const f32x4 (&a
                     v[2] += __uint_as_float(ad.y << 16);
                     v[3] += __uint_as_float(ad.y & 0xffff0000u);
                 }
-                *reinterpret_cast<uint2*>(a.a + o) =
-                    make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16), bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+                *reinterpret_cast<uint2*>(a.a + o) = pack4(v);
             } else if (a.z) {
                 *reinterpret_cast<float4*>(a.z + o) = make_float4(z[0], z[1], z[2], z[3]);
             }
@@ -176,15 +175,16 @@ __global__ void __launch_bounds__(256) k_conv1(ConvArgs a, const float* __restri
         for (int b = 0; b < NB; ++b) {
             const int co = 16 * m + 4 * q;
             const size_t o = (((size_t)n * H + y) * W + x0 + 16 * b + l16) * 32 + co;
-            float dg[4];
-            uint32_t h[4];
+            float dg[4], v[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                h[r] = bf16_bits(gelu_and_grad(acc[m][b][r] + a.bias[co + r], dg[r]));
-                act[m][b][r] = __uint_as_float(h[r] << 16);
-            }
+            for (int r = 0; r < 4; ++r) v[r] = gelu_and_grad(acc[m][b][r] + a.bias[co + r], dg[r]);
+            const uint2 h = pack4(v);
+            act[m][b][0] = __uint_as_float(h.x << 16);
+            act[m][b][1] = __uint_as_float(h.x & 0xffff0000u);
+            act[m][b][2] = __uint_as_float(h.y << 16);
+            act[m][b][3] = __uint_as_float(h.y & 0xffff0000u);
             if (a.dgelu) *reinterpret_cast<uint2*>(a.dgelu + o) = pack4(dg);
-            *reinterpret_cast<uint2*>(a.a + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            *reinterpret_cast<uint2*>(a.a + o) = h;
         }
     float* sc = odd_rows + (wave >> 1) * (2 * NB * 4 * 64);
     if (wave & 1) {
@@ -202,18 +202,19 @@ __global__ void __launch_bounds__(256) k_conv1(ConvArgs a, const float* __restri
         for (int m = 0; m < 2; ++m)
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                uint32_t h[4];
+                float pv[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const float v00 = act[m][b][r], v10 = __shfl_xor(v00, 1);
                     const float* row1 = sc + ((m * NB + b) * 4 + r) * 64;
                     const float v01 = row1[lane], v11 = row1[lane ^ 1];
-                    h[r] = bf16_bits(0.25f * (((v00 + v10) + v01) + v11));
+                    pv[r] = 0.25f * (((v00 + v10) + v01) + v11);
                 }
+                const uint2 h = pack4(pv);
                 if (!(l16 & 1)) {
                     const int xh = (x0 + 16 * b + l16) >> 1, co = 16 * m + 4 * q;
                     const size_t o = (((size_t)n * Hh + yh) * Wh + xh) * 32 + co;
-                    *reinterpret_cast<uint2*>(pool + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                    *reinterpret_cast<uint2*>(pool + o) = h;
                 }
             }
     }
@@ -341,8 +342,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
                         const uint2 dg = *reinterpret_cast<const uint2*>(a.dgelu_in + o);
                         const float g[4] = {v[0] * bf16f(dg.x), v[1] * bf16f(dg.x >> 16), v[2] * bf16f(dg.y),
                                             v[3] * bf16f(dg.y >> 16)};
-                        *reinterpret_cast<uint2*>(a.dpool + o) = make_uint2(bf16_bits(g[0]) | (bf16_bits(g[1]) << 16),
-                                                                            bf16_bits(g[2]) | (bf16_bits(g[3]) << 16));
+                        *reinterpret_cast<uint2*>(a.dpool + o) = pack4(g);
                     }
                 }
         }
@@ -509,11 +509,10 @@ __global__ void __launch_bounds__(256) k_dz3_dr(const float* __restrict__ dr, co
     dz::ds8(w, drt, p, c8, ds);
     float g[8];
     unpack8(*reinterpret_cast<const uint4*>(g3 + pix * 32 + 8 * c8), g);
-    uint32_t h[8];
+    uint32_t h[4];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) h[k] = bf16_bits(ds[k] * g[k]);
-    *reinterpret_cast<uint4*>(dz3 + pix * 32 + 8 * c8) =
-        make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+    for (int k = 0; k < 4; ++k) h[k] = pack2(ds[2 * k] * g[2 * k], ds[2 * k + 1] * g[2 * k + 1]);
+    *reinterpret_cast<uint4*>(dz3 + pix * 32 + 8 * c8) = make_uint4(h[0], h[1], h[2], h[3]);
 }
 
 // ---------------------------------------------------------------- weight grads
@@ -854,16 +853,14 @@ constexpr int US = wg::SEG + 4;  // layer 1: fp32 per channel row of U^T
 }
 
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
-    uint32_t h[8], l[8];
+    uint32_t h[4], l[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        h[j] = bf16_bits(v[j]);
-        l[j] = bf16_bits(v[j] - __uint_as_float(h[j] << 16));
+    for (int j = 0; j < 4; ++j) {
+        h[j] = pack2(v[2 * j], v[2 * j + 1]);
+        l[j] = pack2(v[2 * j] - __uint_as_float(h[j] << 16), v[2 * j + 1] - __uint_as_float(h[j] & 0xffff0000u));
     }
-    hi = __builtin_bit_cast(bf16x8, make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
-                                               h[6] | (h[7] << 16)));
-    lo = __builtin_bit_cast(bf16x8, make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16),
-                                               l[6] | (l[7] << 16)));
+    hi = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+    lo = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
 // Layer 4: U = dr (one channel), V = s (32 channels).
