@@ -44,14 +44,21 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int DTHREADS = 512;   // 8 waves: strip parity = wave & 1, row group = wave >> 1
 // Tile: DX = 32*S columns (2S strips of 16), DY = 64*NB rows (4 groups of NB blocks).
 
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;  // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                      // round to nearest even
-    return u >> 16;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// Two floats as packed bf16 (a in the low half), rounded to nearest even by
+// one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
-
-__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+// Four floats as bf16 hi terms, or (lo) the bf16 of what the hi terms leave.
+__device__ __forceinline__ uint2 pack4(const float (&e)[4], bool lo) {
+    uint2 h = make_uint2(pack2(e[0], e[1]), pack2(e[2], e[3]));
+    if (lo)
+        h = make_uint2(pack2(e[0] - __uint_as_float(h.x << 16), e[1] - __uint_as_float(h.x & 0xffff0000u)),
+                       pack2(e[2] - __uint_as_float(h.y << 16), e[3] - __uint_as_float(h.y & 0xffff0000u)));
+    return h;
+}
 
 // Stage rows [y0-hy, y0+DY+hy) x cols [x0-hx, x0-hx+tc) of one channel as
 // bf16, wrapped on the torus: the value's bf16 rounding (hi), or with lo set
@@ -89,17 +96,13 @@ __device__ __forceinline__ void stage_tile(uint16_t* tile, const float* src, int
             for (int u = 0; u < STAGE_BATCH; ++u) {
                 const int r = r0 + 8 * u;
                 if (r < tr) {
-                    const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-                    uint32_t h[4];
+                    float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int c = cx + t;
-                        uint32_t hv = bf16_bits(e[t]);
-                        if (lo) hv = bf16_bits(e[t] - bf16_value(hv));
-                        h[t] = (c >= -halo_y && c < DX + halo_y) ? hv : 0u;
+                        if (!(c >= -halo_y && c < DX + halo_y)) e[t] = 0.0f;  // bf16 0 for hi and lo
                     }
-                    *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) =
-                        make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                    *reinterpret_cast<uint2*>(tile + (size_t)r * tstride + j) = pack4(e, lo);
                 }
             }
         }
@@ -490,15 +493,13 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
 #pragma unroll
             for (int u = 0; u < NEW_PER_WAVE; ++u) {
                 const int v = DY * i + a.tr + wave + 8 * u;
-                const float e[4] = {nv[u].x, nv[u].y, nv[u].z, nv[u].w};
-                uint32_t h[4];
+                float e[4] = {nv[u].x, nv[u].y, nv[u].z, nv[u].w};
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int cx = cxn + c;
-                    h[c] = (cx >= -a.halo_y && cx < DX + a.halo_y) ? bf16_bits(e[c]) : 0u;
+                    if (!(cx >= -a.halo_y && cx < DX + a.halo_y)) e[c] = 0.0f;
                 }
-                *reinterpret_cast<uint2*>(ring + (size_t)(v % RT) * a.tstride + jn) =
-                    make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                *reinterpret_cast<uint2*>(ring + (size_t)(v % RT) * a.tstride + jn) = pack4(e, false);
             }
         }
 #pragma unroll
