@@ -465,13 +465,18 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
         const int i = t - t0;
         const bool more = t + 1 < t1;
         // Rows 64i + tr + wave + 8u (u < 8) of the run: the next tile's new rows.
+        // Their world rows and ring slots step by 8 from u = 0, so each
+        // takes one conditional subtraction (8u < 64 <= H, 8u < RT), not a
+        // division; the row indexes are wave-uniform.
+        const int v0 = DY * i + a.tr + __builtin_amdgcn_readfirstlane(wave);
+        const int gy0 = (ybase - a.halo_y + v0) % H;  // the argument is positive (tr > halo_y)
+        const int slot0 = v0 % RT;
         float4 nv[NEW_PER_WAVE];
         if (more && col_ok) {
 #pragma unroll
             for (int u = 0; u < NEW_PER_WAVE; ++u) {
-                const int v = DY * i + a.tr + wave + 8 * u;
-                int gy = (ybase - a.halo_y + v) % H;
-                if (gy < 0) gy += H;
+                int gy = gy0 + 8 * u;
+                if (gy >= H) gy -= H;
                 nv[u] = *reinterpret_cast<const float4*>(a.state_in + (size_t)gy * W + gxn);
             }
         }
@@ -492,14 +497,15 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
         if (more && col_ok) {
 #pragma unroll
             for (int u = 0; u < NEW_PER_WAVE; ++u) {
-                const int v = DY * i + a.tr + wave + 8 * u;
+                int slot = slot0 + 8 * u;
+                if (slot >= RT) slot -= RT;
                 float e[4] = {nv[u].x, nv[u].y, nv[u].z, nv[u].w};
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int cx = cxn + c;
                     if (!(cx >= -a.halo_y && cx < DX + a.halo_y)) e[c] = 0.0f;
                 }
-                *reinterpret_cast<uint2*>(ring + (size_t)(v % RT) * a.tstride + jn) = pack4(e, false);
+                *reinterpret_cast<uint2*>(ring + (size_t)slot * a.tstride + jn) = pack4(e, false);
             }
         }
 #pragma unroll
