@@ -188,12 +188,81 @@ __device__ __forceinline__ void conv_rows(f32x4 (&acc)[S][NB], const uint16_t* t
     }
 }
 
+// The rows [i0, i1) of the row list (NCH chunks each) for NB = 1, where the
+// list holds kernel rows dy and -dy next to each other (row 0 first when the
+// group has it): the two rows of a pair share their A fragments, so a pair
+// reads NCH A fragments instead of 2 NCH and runs 2 S NCH MFMAs between
+// loads.  The MFMAs per accumulator run in the order of conv_rows (row by
+// row, chunks in increasing order), so both give the same bits.
+template <int S, int NCH>
+struct PairFrags {
+    bf16x8 A[NCH];
+    bf16x8 B[2][NCH + S - 1];
+};
+
+template <int S, int NCH>
+__device__ __forceinline__ void mfma_pair(f32x4 (&acc)[S][1], const PairFrags<S, NCH>& f) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < NCH + S - 1; ++u)
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const int c = u - k;
+                if (c >= 0 && c < NCH)
+                    acc[k][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.A[c], f.B[h][u], acc[k][0], 0, 0, 0);
+            }
+}
+
+template <int S, int NCH>
+__device__ __forceinline__ void conv_row_pairs(f32x4 (&acc)[S][1], const uint16_t* tile, const uint4* frag_lds,
+                                               const int4& ent, int i0, int i1, int base, int ring_rows, int colP,
+                                               int tstride, int lane) {
+    if ((i1 - i0) & 1) {  // kernel row 0, unpaired
+        conv_rows<S, 1, NCH>(acc, tile, frag_lds, ent, i0, i0 + 1, base, ring_rows, colP, tstride, lane);
+        ++i0;
+    }
+    if (i0 >= i1) return;
+    auto brow_of = [&](int i) {
+        const int dy = __builtin_amdgcn_readlane(ent.x, i);
+        int slot = base + dy;
+        if (slot < 0) slot += ring_rows;
+        else if (slot >= ring_rows) slot -= ring_rows;
+        return tile + (size_t)slot * tstride + colP;
+    };
+    auto load = [&](PairFrags<S, NCH>& f, int i) {
+        const int e = __builtin_amdgcn_readlane(ent.y, i);
+        const int fr = __builtin_amdgcn_readlane(ent.z, i);
+        const uint16_t* b0 = brow_of(i) + e;
+        const uint16_t* b1 = brow_of(i + 1) + e;
+        const uint4* afrag = frag_lds + fr * 64;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) f.A[c] = __builtin_bit_cast(bf16x8, afrag[c * 64 + lane]);
+#pragma unroll
+        for (int u = 0; u < NCH + S - 1; ++u) {
+            f.B[0][u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b0 + 32 * u));
+            f.B[1][u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(b1 + 32 * u));
+        }
+    };
+    PairFrags<S, NCH> f0, f1;
+    load(f0, i0);
+    for (int i = i0; i < i1; i += 4) {
+        // f0 holds the pair at i.
+        if (i + 2 < i1) load(f1, i + 2);
+        mfma_pair(acc, f0);
+        if (i + 2 < i1) {
+            if (i + 4 < i1) load(f0, i + 4);
+            mfma_pair(acc, f1);
+        }
+    }
+}
+
 // All MFMAs of one kernel term over this wave's S strips x NB blocks,
 // accumulated into acc, row group by row group (3, 2, then 1 chunks).
 // Tile row r of the lane sits in LDS row (vbase + r) mod ring_rows: the
 // plain tile passes vbase 0 and a ring_rows no row reaches; the ring kernel
 // (NB = 1) passes its ring.
-template <int S, int NB>
+template <int S, int NB, bool PAIRS = false>
 __device__ __forceinline__ void conv_tile(f32x4 (&acc)[S][NB], const uint16_t* tile, const uint4* frag_lds,
                                           const int4* rows, const int (&nrow)[3], int halo_y, int halo_x,
                                           int tstride, int ring_rows, int vbase) {
@@ -204,9 +273,15 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[S][NB], const uint16_t* t
     const int colP = 16 * par + halo_x + 8 * q;
     const int a3 = nrow[0], a2 = a3 + nrow[1], a1 = a2 + nrow[2];
     const int4 ent = lane < a1 ? rows[lane] : int4{0, 0, 0, 0};  // at most 63 rows
-    conv_rows<S, NB, 3>(acc, tile, frag_lds, ent, 0, a3, base, ring_rows, colP, tstride, lane);
-    conv_rows<S, NB, 2>(acc, tile, frag_lds, ent, a3, a2, base, ring_rows, colP, tstride, lane);
-    conv_rows<S, NB, 1>(acc, tile, frag_lds, ent, a2, a1, base, ring_rows, colP, tstride, lane);
+    if constexpr (PAIRS && NB == 1) {
+        conv_row_pairs<S, 3>(acc, tile, frag_lds, ent, 0, a3, base, ring_rows, colP, tstride, lane);
+        conv_row_pairs<S, 2>(acc, tile, frag_lds, ent, a3, a2, base, ring_rows, colP, tstride, lane);
+        conv_row_pairs<S, 1>(acc, tile, frag_lds, ent, a2, a1, base, ring_rows, colP, tstride, lane);
+    } else {
+        conv_rows<S, NB, 3>(acc, tile, frag_lds, ent, 0, a3, base, ring_rows, colP, tstride, lane);
+        conv_rows<S, NB, 2>(acc, tile, frag_lds, ent, a3, a2, base, ring_rows, colP, tstride, lane);
+        conv_rows<S, NB, 1>(acc, tile, frag_lds, ent, a2, a1, base, ring_rows, colP, tstride, lane);
+    }
 }
 
 // One kernel's potential over the tile: K_hi s_hi, plus K_lo s_hi and
@@ -493,7 +568,7 @@ __global__ void __launch_bounds__(DTHREADS) k_direct_ring(DirectArgs a) {
         f32x4 acc[S][1];
 #pragma unroll
         for (int k = 0; k < S; ++k) acc[k][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        conv_tile<S, 1>(acc, ring, frag_lds, rows_lds, kk.nrow, a.halo_y, a.halo_x, a.tstride, RT, (DY * i) % RT);
+        conv_tile<S, 1, true>(acc, ring, frag_lds, rows_lds, kk.nrow, a.halo_y, a.halo_x, a.tstride, RT, (DY * i) % RT);
         if (more && col_ok) {
 #pragma unroll
             for (int u = 0; u < NEW_PER_WAVE; ++u) {

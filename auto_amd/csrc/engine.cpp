@@ -104,7 +104,7 @@ double bf16_to_f64(uint16_t h) {
 // element t holds bf16(K[dy][e + 32c + 8q + t - i]) with i = l%16, q = l/16.
 // Fragments are numbered by |dy| (rows dy and -dy share them).  Appends the
 // row list (dy, e, first fragment, 0) of the rows with 3, then 2, then 1
-// chunks to `rows`, the hi fragments to `frags` and, for three terms, the lo
+// chunks (each group: row 0 if present, then the pairs -|dy|, +|dy|) to `rows`, the hi fragments to `frags` and, for three terms, the lo
 // fragments bf16(K - bf16(K)) right after them (same numbering, offset nfrag).
 double build_fragments(const std::vector<double>& t, int R, int terms, std::vector<int>& rowlist,
                        std::vector<uint16_t>& frags, int& nfrag, int nrow[3]) {
@@ -132,13 +132,18 @@ double build_fragments(const std::vector<double>& t, int R, int terms, std::vect
         rows.insert(rows.end(), {e, chunks, nf});
         nf += chunks;
     }
+    // Within a group, row 0 (if the group has it) comes first and rows -|dy|
+    // and +|dy| follow as adjacent pairs, which share their fragments.
     for (int c = 3; c >= 1; --c) {
         nrow[3 - c] = 0;
-        for (int dy = -R; dy <= R; ++dy) {
-            const int* r = &rows[3 * std::abs(dy)];
+        for (int ady = 0; ady <= R; ++ady) {
+            const int* r = &rows[3 * ady];
             if (r[1] != c) continue;
-            rowlist.insert(rowlist.end(), {dy, r[0], r[2], 0});
-            ++nrow[3 - c];
+            for (int dy : {-ady, ady}) {
+                rowlist.insert(rowlist.end(), {dy, r[0], r[2], 0});
+                ++nrow[3 - c];
+                if (ady == 0) break;  // row 0 once
+            }
         }
     }
     for (int part = 0; part < (terms == 3 ? 2 : 1); ++part) {
