@@ -4,10 +4,12 @@
 // Training data are (x_t, x_{t+k}) crops taken from a sim's own trajectory
 // (the plain step generates them); training is Adam on the MSE loss with the
 // project's forward and backward kernels; inference advances a sim k steps
-// per network call, and every `judge_every` calls runs k plain steps from the
-// same input and compares (relative L2).  Past the tolerance the run falls
-// back to plain stepping for the rest of the call.  At a judged call the sim
-// continues from the plain result.
+// per network call.  The first call of every window of `judge_every` calls is
+// judged before the window is trusted: k plain steps from the same input,
+// compared by relative L2.  Past the tolerance the run falls back to plain
+// stepping for the rest of the call.  At a judged call the sim continues from
+// the plain result, so a network that fails its first judge never touches the
+// state and the run is the plain trajectory bit for bit.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -544,8 +546,9 @@ ah_status ah_surrogate_advance(ah_surrogate* S, ah_sim* sim, uint32_t steps, uin
                 float* state = sim_state(sim);
                 forward(S, state, false);
                 ++calls;
-                if (calls % judge_every == 0) {
-                    // Judge: k plain steps from the same input, on the sim's path.
+                if ((calls - 1) % judge_every == 0) {
+                    // Judge (calls 1, 1 + judge_every, ...): k plain steps from
+                    // the same input, on the sim's path.
                     AH_HIP(hipStreamSynchronize(S->stream));
                     sim_advance(sim, k);
                     AH_HIP(hipStreamSynchronize(sst));

@@ -407,9 +407,12 @@ ah_status ah_surrogate_train(ah_surrogate* s, uint32_t steps, uint32_t batch, ui
 ah_status ah_surrogate_eval(ah_surrogate* s, uint32_t first, uint32_t count, double* mse);
 /*
  * Advance `sim` by `steps` steps with the network (horizon steps per call).
- * Every judge_every-th call is judged: `horizon` plain steps from the same
- * input, relative L2 of the network's output against them; the sim keeps the
- * plain result, and past `tol` the rest of the call falls back to plain steps.
+ * The first call of every window of judge_every calls (calls 1, 1 +
+ * judge_every, ...) is judged before the window runs: `horizon` plain steps
+ * from the same input, relative L2 of the network's output against them; the
+ * sim keeps the plain result, and past `tol` the rest of the call falls back
+ * to plain steps (a network that fails its first judge leaves the plain
+ * trajectory unchanged).
  */
 ah_status ah_surrogate_advance(ah_surrogate* s, ah_sim* sim, uint32_t steps, uint32_t judge_every, float tol,
                                ah_surrogate_report* report);
