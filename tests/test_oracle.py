@@ -27,6 +27,18 @@ def test_splitmix64_published_vector(O):
     assert [r.next() for _ in range(3)] == want
 
 
+def test_pcg_rand3_unit_f_golden_vectors(O):
+    """pcg / rand3 / unit_f against vectors evaluated from the reference's WGSL
+    text (program.rs:139-146; constants parsed, not retyped: see
+    tests/golden/make_pcg_vectors.py)."""
+    g = np.load(os.path.join(GOLDEN, "pcg_vectors.npz"))
+    assert [O.pcg(int(v)) for v in g["v"]] == [int(x) for x in g["pcg"]]
+    got = [O.rand3(int(a), int(b), int(c)) for a, b, c in zip(g["a"], g["b"], g["c"])]
+    assert got == [int(x) for x in g["rand3"]]
+    got = np.array([O.unit_f(int(r)) for r in g["r"]], np.float32)
+    assert np.array_equal(got.view(np.uint32), g["unit_f"].view(np.uint32))
+
+
 def test_pcg_is_a_bijection_on_a_sample(O):
     """Every step of the PCG hash is invertible on u32, so distinct inputs give distinct outputs."""
     outs = {O.pcg(v) for v in range(0, 1 << 16)}
