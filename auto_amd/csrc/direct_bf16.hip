@@ -202,6 +202,10 @@ struct PairFrags {
 
 template <int S, int NCH>
 __device__ __forceinline__ void mfma_pair(f32x4 (&acc)[S][1], const PairFrags<S, NCH>& f) {
+    // Raised issue priority while this wave's MFMAs run: with two waves per
+    // SIMD, the other wave's LDS reads and VALU work fill the gaps between
+    // them instead of delaying them.
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -212,6 +216,7 @@ __device__ __forceinline__ void mfma_pair(f32x4 (&acc)[S][1], const PairFrags<S,
                 if (c >= 0 && c < NCH)
                     acc[k][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.A[c], f.B[h][u], acc[k][0], 0, 0, 0);
             }
+    __builtin_amdgcn_s_setprio(0);
 }
 
 template <int S, int NCH>
